@@ -1,0 +1,119 @@
+"""Closed-form dense matrices of the hot-path operators (test infrastructure).
+
+Each function restates one class's ``_reference()`` of the reference
+(fastmat 0.2.2.post0): the dense matrix built without any fast algorithm.
+These are the oracle at small sizes; ``oracle.fast64`` covers large sizes.
+All float results are FP64 / complex128.
+"""
+import numpy as np
+
+
+def fourier(order):
+    """Unnormalised DFT matrix, F[j, l] = exp(-2 pi i j l / N).
+
+    Follows Fourier.pyx:241-246.  The product j*l is reduced modulo N in int64
+    before forming the angle (the same matrix; fewer rounding errors at large N
+    than the reference's direct ``j*l*-2j*pi/N``).
+    """
+    n = int(order)
+    idx = np.arange(n, dtype=np.int64)
+    prod = np.mod(np.outer(idx, idx), n)
+    return np.exp(prod * (-2j * np.pi / n))
+
+
+def circulant(vecC):
+    """C[i, j] = c[(i - j) mod N], the 1-D branch of Circulant._refRecursion
+    (Circulant.pyx:414-425)."""
+    c = np.asarray(vecC)
+    n = c.size
+    i = np.arange(n)
+    return c[np.mod(i[:, None] - i[None, :], n)]
+
+
+def toeplitz_tensor(tenT, numRows):
+    """1-D Toeplitz from the defining vector tenT = [vecC (m), vecR (n-1)].
+
+    Follows Toeplitz._refRecursion, single-level branch (Toeplitz.pyx:719-732):
+    the lower triangle holds tenT[:m] down the columns, the upper triangle holds
+    ``tenT[m:][::-1]`` along the rows, i.e. T[i, j] = tenT[(i - j) mod (m+n-1)].
+    """
+    t = np.asarray(tenT)
+    m = int(numRows)
+    n = t.size - m + 1
+    T = np.zeros((m, n), dtype=np.result_type(t.dtype, np.int8))
+    for ii in range(min(m, n)):
+        T[ii:m, ii] = t[:m - ii]
+    vecR = t[m:][::-1]
+    for ii in range(min(m, n - 1)):
+        T[ii, ii + 1:n] = vecR[:n - ii - 1]
+    return T
+
+
+def toeplitz(vecC, vecR):
+    """Toeplitz(vecC, vecR): tenT = hstack(vecC, vecR), split = len(vecC)
+    (Toeplitz.pyx:182-201)."""
+    vecC = np.asarray(vecC)
+    vecR = np.asarray(vecR)
+    return toeplitz_tensor(np.hstack((vecC, vecR)), vecC.size)
+
+
+def hadamard(order, dtype=np.int64):
+    """Sylvester Hadamard matrix H[i, j] = (-1)^popcount(i & j)
+    (Hadamard.pyx:242-248 builds it with scipy.linalg.hadamard)."""
+    n = 1 << int(order)
+    i = np.arange(n, dtype=np.int64)
+    a = np.bitwise_and(i[:, None], i[None, :])
+    # parity of popcount via xor-folding
+    p = a.copy()
+    for s in (32, 16, 8, 4, 2, 1):
+        p ^= p >> s
+    return (1 - 2 * (p & 1)).astype(dtype)
+
+
+def permutation(sigma):
+    """P = eye(N)[sigma, :] so that P x = x[sigma] (Permutation.pyx:121-129)."""
+    sigma = np.asarray(sigma)
+    return np.eye(sigma.size, dtype=np.int8)[sigma, :]
+
+
+def diag(vecD):
+    """Diag.reference: np.diag(vecD) (Diag.pyx:170-171)."""
+    return np.diag(np.asarray(vecD))
+
+
+def kron(*refs):
+    """np.kron chain, first factor = slowest index (Kron.pyx:344-353)."""
+    dt = np.promote_types(np.asarray(refs[0]).dtype, np.float64)
+    res = np.asarray(refs[0]).astype(dt)
+    for r in refs[1:]:
+        res = np.kron(res, r)
+    return res
+
+
+def product(refs, scalar=1):
+    """Product reference: scalar * R0 @ R1 @ ... (Product.pyx:243-257)."""
+    res = np.asarray(refs[-1]) * scalar
+    for r in refs[-2::-1]:
+        res = np.asarray(r).dot(res)
+    return res
+
+
+def sum_(refs):
+    """Sum reference (Sum.pyx:178-186)."""
+    res = np.zeros_like(np.asarray(refs[0]), dtype=np.result_type(*refs))
+    for r in refs:
+        res = res + r
+    return res
+
+
+def partial(ref, rows=None, cols=None):
+    """Partial reference: full[rows, :][:, cols] (Partial.pyx:297-303).
+
+    ``rows``/``cols`` may be int index arrays or bool masks, as in
+    Partial.pyx:141-166."""
+    a = np.asarray(ref)
+    if rows is not None:
+        a = a[np.asarray(rows), :]
+    if cols is not None:
+        a = a[:, np.asarray(cols)]
+    return a

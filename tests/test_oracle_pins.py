@@ -1,0 +1,271 @@
+"""Pins for the CPU oracle (runs without a GPU).
+
+Each oracle function is checked against something that does not restate its
+own formula: a library routine (scipy.linalg / numpy), a closed form or
+invariant, brute force on tiny inputs, or a statement in the reference's own
+test definitions.  A plausible mistake (wrong sign, transposed operand, index
+off by one, dropped term) fails at least one of these.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import accept, cpu_path, dense, fast64, fwht_c, heuristics
+
+RNG = np.random.default_rng(20261202)
+
+
+def crandn(*shape, rng=RNG):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# ------------------------------------------------------------------ Fourier
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 35, 127, 128])
+def test_dense_fourier_vs_numpy_fft(n):
+    x = crandn(n, 3)
+    np.testing.assert_allclose(dense.fourier(n) @ x, np.fft.fft(x, axis=0),
+                               rtol=1e-12, atol=1e-12 * n)
+
+
+@pytest.mark.parametrize("n", [5, 64, 97])
+def test_dense_fourier_invariants(n):
+    F = dense.fourier(n)
+    # F^H F = N I (Fourier.pyx:188-189) and F symmetric
+    np.testing.assert_allclose(F.conj().T @ F, n * np.eye(n), atol=1e-9 * n)
+    np.testing.assert_allclose(F, F.T, atol=1e-12)
+    # impulse -> all ones; F F x = N x[(-n) mod N]
+    e0 = np.zeros(n); e0[0] = 1
+    np.testing.assert_allclose(F @ e0, np.ones(n), atol=1e-12)
+    x = crandn(n)
+    np.testing.assert_allclose(F @ (F @ x), n * x[np.mod(-np.arange(n), n)],
+                               atol=1e-9 * n)
+
+
+def test_fast64_fourier_matches_direct_bins_large():
+    n = 1 << 14
+    x = crandn(n, 2)
+    bins = [0, 1, 7, 4095, n // 2, n - 1]
+    y = fast64.fourier(x)
+    yb = fast64.fourier_bins(x, bins)
+    np.testing.assert_allclose(y[bins], yb, rtol=1e-10, atol=1e-9)
+    ya = fast64.fourier(x, adjoint=True)
+    yba = fast64.fourier_bins(x, bins, adjoint=True)
+    np.testing.assert_allclose(ya[bins], yba, rtol=1e-10, atol=1e-9)
+    # Parseval: ||F x||^2 = N ||x||^2
+    np.testing.assert_allclose(np.linalg.norm(y, axis=0) ** 2,
+                               n * np.linalg.norm(x, axis=0) ** 2, rtol=1e-10)
+
+
+@pytest.mark.parametrize("n", [35, 127, 1000])
+def test_fast64_fourier_vs_dense(n):
+    x = crandn(n, 2)
+    F = dense.fourier(n)
+    np.testing.assert_allclose(fast64.fourier(x), F @ x, atol=1e-9 * n)
+    np.testing.assert_allclose(fast64.fourier(x, adjoint=True), F.conj().T @ x,
+                               atol=1e-9 * n)
+
+
+# ----------------------------------------------------------------- Circulant
+@pytest.mark.parametrize("n", [1, 2, 7, 41, 64])
+def test_dense_circulant_vs_scipy(n):
+    c = crandn(n)
+    np.testing.assert_array_equal(dense.circulant(c), sla.circulant(c))
+
+
+@pytest.mark.parametrize("n,L", [(7, 7), (7, 13), (7, 16), (41, 96), (64, 64),
+                                 (64, 128), (100, 256)])
+def test_fast64_circulant_vs_dense(n, L):
+    c = crandn(n)
+    x = crandn(n, 3)
+    C = sla.circulant(c)
+    np.testing.assert_allclose(fast64.circulant(c, x, L=L), C @ x, atol=1e-10 * n)
+    np.testing.assert_allclose(fast64.circulant(c, x, adjoint=True, L=L),
+                               C.conj().T @ x, atol=1e-10 * n)
+
+
+# ------------------------------------------------------------------ Toeplitz
+@pytest.mark.parametrize("m,n", [(4, 3), (4, 5), (4, 41), (1, 1), (1, 6), (6, 1),
+                                 (17, 9)])
+def test_dense_toeplitz_vs_scipy(m, n):
+    vecC = crandn(m)
+    vecR = crandn(n - 1)
+    # vecR is indexed backwards: the first row is [c0, vecR[-1], ..., vecR[0]]
+    ref = sla.toeplitz(vecC, np.r_[vecC[0], vecR[::-1]])
+    np.testing.assert_array_equal(dense.toeplitz(vecC, vecR), ref)
+
+
+@pytest.mark.parametrize("m,n,L", [(4, 3, 6), (4, 5, 8), (4, 41, 64), (17, 9, 25),
+                                   (300, 500, 1024), (1, 1, 1)])
+def test_fast64_toeplitz_vs_dense(m, n, L):
+    vecC = crandn(m)
+    vecR = crandn(n - 1)
+    T = sla.toeplitz(vecC, np.r_[vecC[0], vecR[::-1]])
+    t = np.hstack([vecC, vecR])
+    x = crandn(n, 2)
+    yv = crandn(m, 2)
+    np.testing.assert_allclose(fast64.toeplitz(t, m, x, L=L), T @ x, atol=1e-10 * L)
+    np.testing.assert_allclose(fast64.toeplitz(t, m, yv, adjoint=True, L=L),
+                               T.conj().T @ yv, atol=1e-10 * L)
+
+
+# ------------------------------------------------------------------ Hadamard
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 6, 10])
+def test_dense_hadamard_vs_scipy(order):
+    np.testing.assert_array_equal(dense.hadamard(order), sla.hadamard(1 << order))
+
+
+@pytest.mark.parametrize("order", [1, 4, 6, 10])
+@pytest.mark.parametrize("dtype", [np.int8, np.int16, np.int32, np.int64])
+def test_hadamard_int_wraps_like_c(order, dtype):
+    n = 1 << order
+    info = np.iinfo(dtype)
+    x = RNG.integers(info.min, info.max, size=(n, 3), endpoint=True).astype(dtype)
+    H = sla.hadamard(n).astype(object)
+    exact = (H.dot(x.astype(object)))
+    bits = np.dtype(dtype).itemsize * 8
+    wrapped = np.array([[((int(v) + (1 << (bits - 1))) % (1 << bits)) - (1 << (bits - 1))
+                         for v in row] for row in exact], dtype=dtype)
+    np.testing.assert_array_equal(fast64.hadamard(x), wrapped)
+    np.testing.assert_array_equal(fwht_c.hadamard(x), wrapped)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.complex128])
+def test_fwht_c_float(dtype):
+    n = 1 << 9
+    x = RNG.standard_normal((n, 2)).astype(dtype)
+    if np.iscomplexobj(x):
+        x = x + 1j * RNG.standard_normal((n, 2)).astype(dtype)
+    y = fwht_c.hadamard(x)
+    assert y.dtype == x.dtype
+    ref = sla.hadamard(n).astype(np.float64) @ x.astype(np.complex128)
+    err = accept.max_rel_error(y, ref)
+    real = np.float32 if dtype in (np.float32, np.complex64) else np.float64
+    assert err < 50 * np.finfo(real).eps
+    # integer-valued floats are exact in any order -> bit-exact vs int64 path
+    xi = RNG.integers(-8, 9, size=(n, 2))
+    np.testing.assert_array_equal(fwht_c.hadamard(xi.astype(dtype)).real,
+                                  fast64.hadamard(xi.astype(np.int64)))
+
+
+def test_hadamard_involution():
+    n = 1 << 12
+    x = RNG.integers(-8, 9, size=(n, 2)).astype(np.int64)
+    np.testing.assert_array_equal(fast64.hadamard(fast64.hadamard(x)), n * x)
+
+
+# ---------------------------------------------------------------------- Kron
+def test_dense_kron_index_formula():
+    A = RNG.standard_normal((3, 3))
+    B = crandn(4, 4)
+    K = dense.kron(A, B)
+    for i1 in range(3):
+        for i2 in range(4):
+            for j1 in range(3):
+                for j2 in range(4):
+                    assert K[i1 * 4 + i2, j1 * 4 + j2] == A[i1, j1] * B[i2, j2]
+
+
+def test_fast64_kron_modewise_vs_dense():
+    dims = (4, 6, 5)
+    mats = [crandn(d, d) for d in dims]
+    x = crandn(int(np.prod(dims)), 3)
+    ops = [lambda v, M=M: M @ v for M in mats]
+    np.testing.assert_allclose(fast64.kron_apply(ops, dims, x),
+                               dense.kron(*mats) @ x, atol=1e-10)
+    # a Fourier factor through fast64.fourier gives the same as its dense matrix
+    ops[0] = lambda v: fast64.fourier(v)
+    mats[0] = dense.fourier(4)
+    np.testing.assert_allclose(fast64.kron_apply(ops, dims, x),
+                               dense.kron(*mats) @ x, atol=1e-10)
+
+
+# ---------------------------------------------------------------- heuristics
+@pytest.mark.parametrize("n,val", [(7, 63), (16, 176), (96, 1728), (41, 1763),
+                                   (35, 525), (72, 1368), (127, 16383),
+                                   (256, 5376), (1 << 20, (1 << 20) * 51)])
+def test_fft_complexity_hand_values(n, val):
+    # hand-evaluated from core/cmath.pyx:156-214 (e.g. 72 = 4*2*9 and the
+    # trial division stops at ii*ii < nn, so 9 costs 10: 72*(5+3+10+1))
+    assert float(heuristics.fft_complexity(n)) == float(val)
+
+
+def test_circulant_padding_first_at_41():
+    # Circulant.pyx:436-437: "7 is just any number that causes no padding",
+    # "41 is the first size bluestein is faster for"
+    assert heuristics.circulant_embedding_length(7) == 7
+    for n in range(1, 41):
+        assert heuristics.circulant_embedding_length(n) == n, n
+    L = heuristics.circulant_embedding_length(41)
+    assert L >= 81 and L != 41
+
+
+def test_fourier_path_choice():
+    # Fourier.pyx:256-257: "127 is a size for which bluestein is faster"
+    assert heuristics.fourier_bluestein_length(127) > 0
+    assert heuristics.fourier_bluestein_length(127, optimize=False) == 0
+    assert heuristics.fourier_bluestein_length(1 << 20) == 0
+    assert heuristics.fourier_bluestein_length(1000003) == 2097152
+
+
+@pytest.mark.parametrize("order", [1, 2, 13, 64, 65, 81, 1000, 2 * 1000003 - 1])
+def test_optimal_size_properties(order):
+    L = heuristics.find_optimal_fft_size(order, 4)
+    assert L >= order
+    r = L
+    for p in (2, 3):
+        while r % p == 0:
+            r //= p
+    assert r == 1  # maxStage=4 only allows factors 2, 3 and 4
+
+
+# ------------------------------------------------------------------ cpu path
+@pytest.mark.parametrize("n", [35, 127, 128])
+def test_cpu_path_fourier(n):
+    x = crandn(n, 2)
+    f = cpu_path.FourierCPU(n)
+    F = dense.fourier(n)
+    np.testing.assert_allclose(f.forward(x), F @ x, atol=1e-9 * n)
+    np.testing.assert_allclose(f.backward(x), F.conj().T @ x, atol=1e-9 * n)
+
+
+def test_cpu_path_fourier_complex64_dtype():
+    # numpy 2.x keeps complex64 through np.fft: the radix path returns it
+    x = crandn(64, 2).astype(np.complex64)
+    assert cpu_path.FourierCPU(64).forward(x).dtype == np.complex64
+    # the Bluestein path computes in a complex128 buffer (Fourier.pyx:216-220)
+    x127 = crandn(127, 1).astype(np.complex64)
+    assert cpu_path.FourierCPU(127).forward(x127).dtype == np.complex128
+
+
+@pytest.mark.parametrize("n", [7, 41, 64])
+def test_cpu_path_circulant(n):
+    c = crandn(n)
+    x = crandn(n, 2)
+    op = cpu_path.circulant(c)
+    C = sla.circulant(c)
+    np.testing.assert_allclose(op.forward(x), C @ x, atol=1e-10 * n)
+    np.testing.assert_allclose(op.backward(x), C.conj().T @ x, atol=1e-10 * n)
+
+
+@pytest.mark.parametrize("m,n", [(4, 3), (4, 41), (30, 20)])
+def test_cpu_path_toeplitz(m, n):
+    vecC = crandn(m)
+    vecR = crandn(n - 1)
+    T = sla.toeplitz(vecC, np.r_[vecC[0], vecR[::-1]])
+    op = cpu_path.toeplitz(vecC, vecR)
+    x = crandn(n, 2)
+    yv = crandn(m, 2)
+    np.testing.assert_allclose(op.forward(x), T @ x, atol=1e-10 * (m + n))
+    np.testing.assert_allclose(op.backward(yv), T.conj().T @ yv, atol=1e-10 * (m + n))
+
+
+def test_acceptance_rule_rejects_planted_error():
+    n = 64
+    ref = crandn(n, 2)
+    tol = accept.tolerance(n, 3, [np.complex64])
+    ok, err, _ = accept.passes(ref + 0.1 * tol * np.abs(ref).max(), ref, n, 3,
+                               [np.complex64])
+    assert ok
+    ok, err, _ = accept.passes(ref + 10 * tol * np.abs(ref).max(), ref, n, 3,
+                               [np.complex64])
+    assert not ok
