@@ -1,0 +1,262 @@
+"""Circulant and Toeplitz matrices via circulant embedding, GPU-backed.
+
+Reference: fastmat/Circulant.pyx (1-D branch 104-137, reference 414-425) and
+fastmat/Toeplitz.pyx (constructor 146-322, embedding 357-366, reference
+719-732).  The reference builds ``Partial(Product(F.H, Diag(fft(t')/L), F))``;
+here one native plan holds the spectrum fft(t')/L (computed on the device in
+FP64) and applies pad -> FFT -> spectrum -> inverse FFT -> crop in one fused
+kernel (L up to the LDS tile) or three passes (larger L).  Embedding lengths
+are powers of two: any L >= 2N-1 (Circulant) or L >= m+n-1 (Toeplitz) embeds
+the matrix exactly, so y = A x is unchanged.
+
+dtype: promote(generator dtype, complex64) unless ``dtype=`` is given; the
+computation precision follows promote(input, dtype) like ``Fourier``.  The
+reference reports complex128 always (Circulant.pyx:223-227,
+Toeplitz.pyx:318-322).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _device as D
+from . import _lib
+from .fourier import _PlanCache, complex_of, precision_of
+from .matrix import Matrix
+
+
+class _ConvBase(Matrix):
+    """Shared native-plan handling of Circulant and Toeplitz."""
+
+    def _setup(self, generator, dtype, options):
+        gen = np.asarray(generator)
+        if dtype is None:
+            dtype = np.promote_types(gen.dtype if gen.dtype.kind in "biufc" else np.float64,
+                                     np.complex64)
+        dtype = D.np_dtype(dtype)
+        if dtype not in (np.dtype(np.complex64), np.dtype(np.complex128)):
+            raise TypeError("dtype must be complex64 or complex128")
+        self._gen128 = gen.astype(np.complex128)
+        self._plans = _PlanCache(self._make_plan, "fm_conv_plan_destroy")
+        self._content_cache = None
+        return dtype
+
+    def __del__(self):
+        try:
+            self._plans.drop()
+        except Exception:
+            pass
+
+    def plan_info(self, dtype=np.complex64):
+        """(rows, cols, fft_length, passes) of the native plan."""
+        h = self._plans.get(precision_of(dtype))
+        r, c, L = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        p = ctypes.c_int()
+        _lib.call("fm_conv_plan_info", h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(L),
+                  ctypes.byref(p))
+        return r.value, c.value, L.value, p.value
+
+    def plan_handle(self, dtype):
+        return self._plans.get(precision_of(dtype))
+
+    def compute_dtype(self, in_dtype):
+        return complex_of(D.promote(in_dtype, self.dtype))
+
+    def _run(self, X, adjoint):
+        cdt = self.compute_dtype(X.dtype)
+        if D.np_dtype(X.dtype) != cdt:
+            X = X.to(D.torch_dtype(cdt))
+        h = self._plans.get(precision_of(cdt))
+        k, n = X.shape
+        m = self.numCols if adjoint else self.numRows
+        Y = D.empty(k, m, cdt)
+        need = ctypes.c_int64()
+        _lib.call("fm_conv_workspace_size", h, k, ctypes.byref(need))
+        ws = D.workspace(need.value)
+        _lib.call("fm_conv_apply", h, int(adjoint), _lib.ptr(X), n, _lib.ptr(Y), m, k,
+                  None if ws is None else _lib.ptr(ws), need.value, D.stream())
+        return Y
+
+    def _fwd(self, X):
+        return self._run(X, False)
+
+    def _bwd(self, X):
+        return self._run(X, True)
+
+    def generator_grad(self, X, G):
+        """Gradient of a real loss w.r.t. the generator (c, or [vecC, vecR]),
+        given the forward input block X (k, cols) and dLoss/dY block G
+        (k, rows), PyTorch convention.  Returns a device tensor."""
+        cdt = self.compute_dtype(X.dtype)
+        X = X.to(D.torch_dtype(cdt)).contiguous()
+        G = G.to(D.torch_dtype(cdt)).contiguous()
+        h = self._plans.get(precision_of(cdt))
+        k = X.shape[0]
+        need = ctypes.c_int64()
+        _lib.call("fm_conv_generator_grad_workspace_size", h, k, ctypes.byref(need))
+        ws = D.workspace(need.value)
+        out = D.empty(1, self._gen128.size, cdt)[0]
+        _lib.call("fm_conv_generator_grad", h, _lib.ptr(X), X.shape[1], _lib.ptr(G), G.shape[1],
+                  k, _lib.ptr(out), _lib.ptr(ws), need.value, D.stream())
+        return out
+
+    def _strided_ok(self, dtype):
+        dt = D.np_dtype(dtype)
+        n = self.numRows
+        limit = 8192 if dt == np.complex64 else 4096
+        return (dt in (np.dtype(np.complex64), np.dtype(np.complex128)) and
+                self.numRows == self.numCols and (n & (n - 1)) == 0 and n <= limit)
+
+    def _apply_strided(self, X, adjoint, outer, inner):
+        h = self._plans.get(precision_of(X.dtype))
+        Y = X.new_empty(X.shape)
+        _lib.call("fm_conv_apply_strided", h, int(adjoint), _lib.ptr(X), _lib.ptr(Y), outer,
+                  inner, D.stream())
+        return Y
+
+
+def _next_pow2(n):
+    return 1 if n <= 1 else 1 << (int(n) - 1).bit_length()
+
+
+class Circulant(_ConvBase):
+    """C[i, j] = c[(i - j) mod N] (1-D generator)."""
+
+    def __init__(self, tenC, **options):
+        tenC = np.asarray(tenC)
+        c = D.squeeze1d(tenC)
+        if tenC.ndim < 1 or c.ndim < 1:
+            raise ValueError("Column-definition tensor must be at least 1D.")
+        if c.ndim > 1:
+            raise NotImplementedError("multi-level Circulant is not supported")
+        self._optimize = options.pop("optimize", True)
+        self._maxStage = options.pop("maxStage", 4)
+        dtype = self._setup(c, options.pop("dtype", None), options)
+        self._tenC = c
+        n = c.size
+        self._content = ()
+        self._initProperties(n, n, dtype, **options)
+
+    @property
+    def tenC(self):
+        return self._tenC
+
+    @property
+    def fftLength(self):
+        n = self._tenC.size
+        return n if (n & (n - 1)) == 0 else _next_pow2(2 * n - 1)
+
+    def _make_plan(self, prec):
+        gen = D.as_tensor(self._gen128, np.complex128)
+        h = ctypes.c_void_p()
+        _lib.call("fm_circulant_plan_create", prec, self._gen128.size, _lib.ptr(gen), D.stream(),
+                  ctypes.byref(h))
+        return h
+
+    @property
+    def content(self):
+        """The equivalent Partial(Product(F.H, Diag(fft(c')/L), F)) chain
+        (Circulant.pyx:127-137, 217-221), GPU-backed."""
+        if self._content_cache is None:
+            from .diag import Diag
+            from .fourier import Fourier
+            from .partial import Partial
+            from .product import Product
+            n, L = self._tenC.size, self.fftLength
+            c = self._gen128
+            t = c if L == n else np.concatenate([c, np.zeros(L - 2 * n + 1), c[1:]])
+            F = Fourier(L, dtype=self.dtype)
+            P = Product(F.H, Diag((np.fft.fft(t) / L).astype(self.dtype)), F)
+            sel = np.arange(n) if L != n else None
+            self._content_cache = (Partial(P, rows=sel, cols=sel),)
+        return self._content_cache
+
+    def _reference(self):
+        c = self._tenC.astype(self.dtype)
+        n = c.size
+        i = np.arange(n)
+        return c[np.mod(i[:, None] - i[None, :], n)]
+
+
+class Toeplitz(_ConvBase):
+    """T[i, j] = tenT[(i - j) mod (m + n - 1)], tenT = [vecC (m), vecR (n-1)]."""
+
+    def __init__(self, *args, **options):
+        split = options.pop("split", None)
+        arrSplit = np.array([] if split is None else split)
+        if len(args) == 1:
+            tenT = np.asarray(args[0])
+        elif len(args) == 2:
+            if not all(isinstance(a, np.ndarray) for a in args):
+                raise ValueError(
+                    "You must specify two 1D-ndarrays containing the " +
+                    "column- and row-definition vectors or one ndarray tensor")
+            if arrSplit.size != 0:
+                raise ValueError(
+                    "You must not define split points when supplying " +
+                    "column- and row-definition vectors.")
+            dt = np.promote_types(args[0].dtype, args[1].dtype)
+            vecC = D.squeeze1d(args[0].astype(dt))
+            vecR = D.squeeze1d(args[1].astype(dt)) if args[1].size else args[1].reshape(0)
+            if vecC.ndim != 1 or vecR.ndim != 1:
+                raise ValueError("Column- and row-definition vectors must be 1D.")
+            arrSplit = np.array(vecC.size)
+            tenT = np.hstack((vecC, vecR))
+        else:
+            raise ValueError(
+                "Invalid number of arguments to Toeplitz: Expecting exactly " +
+                "one or two fixed arguments")
+        if tenT.ndim != 1:
+            raise NotImplementedError("multi-level Toeplitz is not supported")
+        arrDim = np.array(tenT.shape)
+        arrSplit = np.atleast_1d(arrSplit)
+        if arrSplit.size == 0:
+            if not all((ll + 1) % 2 == 0 for ll in arrDim):
+                raise ValueError(
+                    "Defining a tensor with non-square levels requires " +
+                    "explicit split points.")
+            arrSplit = (arrDim + 1) // 2
+        if arrSplit.size != tenT.ndim:
+            raise ValueError("The split point vector must have one entry for each " +
+                             "dimension of the defining tensor")
+        if any(ll < 1 or ll > arrDim[ii] for ii, ll in enumerate(arrSplit)):
+            raise ValueError("Entry in split vector outside of defining tensor bounds")
+        self._optimize = options.pop("optimize", True)
+        self._maxStage = options.pop("maxStage", 4)
+        m = int(arrSplit[0])
+        n = int(arrDim[0]) - m + 1
+        self._tenT = tenT
+        self._arrDimRows = np.array([m])
+        self._arrDimCols = np.array([n])
+        dtype = self._setup(tenT, options.pop("dtype", None), options)
+        self._content = ()
+        self._initProperties(m, n, dtype, **options)
+
+    @property
+    def tenT(self):
+        return self._tenT
+
+    @property
+    def vecC(self):
+        return self._tenT[:self.numRows]
+
+    @property
+    def vecR(self):
+        return self._tenT[self.numRows:]
+
+    @property
+    def fftLength(self):
+        return _next_pow2(self.numRows + self.numCols - 1)
+
+    def _make_plan(self, prec):
+        gen = D.as_tensor(self._gen128, np.complex128)
+        h = ctypes.c_void_p()
+        _lib.call("fm_toeplitz_plan_create", prec, self.numRows, self.numCols, _lib.ptr(gen),
+                  D.stream(), ctypes.byref(h))
+        return h
+
+    def _reference(self):
+        t = self._tenT.astype(self.dtype)
+        m, n = self.numRows, self.numCols
+        i = np.arange(m)[:, None]
+        j = np.arange(n)[None, :]
+        return t[np.mod(i - j, m + n - 1)]

@@ -1,0 +1,835 @@
+// Fourier, Circulant and Toeplitz operators on the FFT tile engine.
+//
+// Reference behaviour replaced (fastmat 0.2.2):
+//   Fourier.__init__/_forward/_backward   Fourier.pyx:63-238
+//   Circulant 1-D construction             Circulant.pyx:104-137
+//   Toeplitz 1-D construction              Toeplitz.pyx:146-316
+//   Partial(Product(F.H, Diag, F)) apply   Partial.pyx:268-294, Product.pyx:205-240
+//
+// Design: see fm_fft_core.hpp.  Twiddles, Bluestein chirps and generator
+// spectra are produced on the device in FP64 (sincospi on exactly reduced
+// arguments) and rounded once to the plan precision.
+#include <cstdarg>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "fm_fft_core.hpp"
+
+namespace fm {
+
+// ------------------------------------------------------------ error state
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+// ------------------------------------------------------- table generation
+// W_M^e = exp(-2 pi i e / M) for e < count, M = 2^logm; FP64 sincospi of the
+// exact argument -2e/M.
+template <typename T>
+__global__ void gen_twiddle_kernel(cpx<T> *out, int logm, int64_t count) {
+    const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (e >= count) return;
+    double s, c;
+    sincospi(-2.0 * double(e) / double(int64_t(1) << logm), &s, &c);
+    out[e] = mk(T(c), T(s));
+}
+
+// Bluestein chirp p[n] = exp(-i pi n^2 / N) with n^2 reduced modulo 2N in
+// int64 before forming the angle (the angle stays in [0, 2 pi)).
+template <typename T>
+__global__ void gen_chirp_kernel(cpx<T> *out, int64_t N, double sign) {
+    const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (n >= N) return;
+    const int64_t r = (n * n) % (2 * N);  // exact: N < 2^31
+    double s, c;
+    sincospi(sign * double(r) / double(N), &s, &c);
+    out[n] = mk(T(c), T(s));
+}
+
+// Wrapped Bluestein kernel g[n] = exp(+i pi n^2 / N), g[L-n] = g[n]
+// (Fourier.pyx:137-153), in complex128.
+__global__ void gen_bluestein_kernel(cpx<double> *g, int64_t N, int64_t L) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= L) return;
+    int64_t n = -1;
+    if (i < N) n = i;
+    else if (i > L - N) n = L - i;
+    cpx<double> v = mk(0.0, 0.0);
+    if (n >= 0) {
+        const int64_t r = (n * n) % (2 * N);  // exact: N < 2^31
+        double s, c;
+        sincospi(double(r) / double(N), &s, &c);
+        v = mk(c, s);
+    }
+    g[i] = v;
+}
+
+// Circulant / Toeplitz generator embedding into a length-L circulant:
+// t'[i] = gen[i] for i < head, t'[L - tail + q] = gen[head + q] for q < tail.
+__global__ void embed_kernel(cpx<double> *t, const cpx<double> *gen, int64_t L,
+                             int64_t head, int64_t tail) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= L) return;
+    cpx<double> v = mk(0.0, 0.0);
+    if (i < head) v = gen[i];
+    else if (i >= L - tail) v = gen[head + (i - (L - tail))];
+    t[i] = v;
+}
+
+template <typename T>
+__global__ void scale_cast_kernel(cpx<T> *out, const cpx<double> *in, int64_t n, double s) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    out[i] = mk(T(in[i].x * s), T(in[i].y * s));
+}
+
+static inline unsigned grid1d(int64_t n, int bs = 256) { return unsigned(ceil_div(n, bs)); }
+
+// ---------------------------------------------------------- twiddle cache
+namespace {
+std::mutex g_tw_mutex;
+std::map<std::tuple<int, int, int, int64_t>, void *> g_tw_tables;
+}  // namespace
+
+// Table of W_{2^logm}^e for e < count, cached per device for the process.
+static const void *table(int prec, int logm, int64_t count, hipStream_t stream,
+                         fm_status *st) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        *st = FM_ERR_HIP;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    auto key = std::make_tuple(dev, prec, logm, count);
+    auto it = g_tw_tables.find(key);
+    if (it != g_tw_tables.end()) return it->second;
+    const size_t esz = prec == FM_COMPLEX64 ? 8 : 16;
+    void *p = nullptr;
+    if (hipMalloc(&p, esz * size_t(count)) != hipSuccess) {
+        set_error("hipMalloc of a %lld-entry twiddle table failed", (long long)count);
+        *st = FM_ERR_ALLOC;
+        return nullptr;
+    }
+    if (prec == FM_COMPLEX64)
+        hipLaunchKernelGGL(gen_twiddle_kernel<float>, grid1d(count), 256, 0, stream,
+                           (cpx<float> *)p, logm, count);
+    else
+        hipLaunchKernelGGL(gen_twiddle_kernel<double>, grid1d(count), 256, 0, stream,
+                           (cpx<double> *)p, logm, count);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
+        set_error("twiddle generation failed");
+        *st = FM_ERR_HIP;
+        hipFree(p);
+        return nullptr;
+    }
+    g_tw_tables[key] = p;
+    return p;
+}
+
+const void *twiddle_table(int precision, int logn, hipStream_t stream, fm_status *st) {
+    return table(precision, logn, int64_t(1) << logn, stream, st);
+}
+
+// ------------------------------------------------------------ tile launch
+template <typename T, int NT, int MAXE, bool CONV>
+static fm_status launch_cfg(const TileArgs<T> &a, hipStream_t stream) {
+    const int N = 1 << a.logN;
+    const size_t lds = size_t(a.B) * (N + 1) * sizeof(cpx<T>) + 3 * size_t(a.B) * 8;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)fft_tile_kernel<T, NT, MAXE, CONV>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    FM_REQUIRE(lds <= 160 * 1024, FM_ERR_VALUE, "tile of %d x %d needs %zu bytes of LDS",
+               a.B, N, lds);
+    FM_REQUIRE(a.ntiles > 0 && a.ntiles < (int64_t(1) << 31), FM_ERR_VALUE,
+               "tile count %lld out of range", (long long)a.ntiles);
+    hipLaunchKernelGGL((fft_tile_kernel<T, NT, MAXE, CONV>), dim3(unsigned(a.ntiles)),
+                       dim3(NT), lds, stream, a);
+    FM_CHECK_LAUNCH();
+    return FM_SUCCESS;
+}
+
+template <typename T>
+fm_status launch_tile(const TileArgs<T> &a, bool conv, hipStream_t stream) {
+    const int64_t tot = int64_t(a.B) << a.logN;
+    FM_REQUIRE(a.B >= 1 && is_pow2(a.B), FM_ERR_VALUE, "tile batch must be a power of two");
+    FM_REQUIRE(tot <= max_tile_elems<T>(), FM_ERR_VALUE, "tile of %lld elements too large",
+               (long long)tot);
+    if (sizeof(T) == 4) {
+        if (tot <= 4096)
+            return conv ? launch_cfg<T, 256, 16, true>(a, stream) : launch_cfg<T, 256, 16, false>(a, stream);
+        if (tot <= 8192)
+            return conv ? launch_cfg<T, 256, 32, true>(a, stream) : launch_cfg<T, 256, 32, false>(a, stream);
+        return conv ? launch_cfg<T, 512, 32, true>(a, stream) : launch_cfg<T, 512, 32, false>(a, stream);
+    } else {
+        if (tot <= 4096)
+            return conv ? launch_cfg<T, 256, 16, true>(a, stream) : launch_cfg<T, 256, 16, false>(a, stream);
+        return conv ? launch_cfg<T, 512, 16, true>(a, stream) : launch_cfg<T, 512, 16, false>(a, stream);
+    }
+}
+
+template fm_status launch_tile<float>(const TileArgs<float> &, bool, hipStream_t);
+template fm_status launch_tile<double>(const TileArgs<double> &, bool, hipStream_t);
+
+template <typename T> static TileArgs<T> blank_args() {
+    TileArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    a.out_scale = T(1);
+    a.sign1 = -1;
+    a.twb_shift = 0;
+    a.n_valid = INT64_MAX;
+    a.m_valid = INT64_MAX;
+    return a;
+}
+
+static int pick_batch(int logN, int64_t nseq, int max_tile, int want) {
+    int B = 1;
+    while (B < want && (int64_t(B) * 2 << logN) <= max_tile && B * 2 <= nseq) B *= 2;
+    while ((int64_t(B) << logN) > max_tile && B > 1) B /= 2;
+    return B;
+}
+
+// ------------------------------------------------------------- transform
+// One length-L transform (L = 2^logL) over k columns, as a single tile pass
+// or as a four-step; optional Bluestein chirps and spectrum.
+struct Geometry {
+    int prec;
+    int logL;
+    int passes;  // 1 (single tile) or multi (2 plain / 3 conv)
+    int logN1, logN2;
+    const void *tw_full, *tw1, *tw2, *twb_hi, *twb_lo;
+    int twb_shift;
+};
+
+static fm_status make_geometry(int prec, int logL, bool conv, hipStream_t stream,
+                               Geometry *g) {
+    memset(g, 0, sizeof(*g));
+    g->prec = prec;
+    g->logL = logL;
+    const int max_tile = prec == FM_COMPLEX64 ? max_tile_elems<float>() : max_tile_elems<double>();
+    // single pass while one column fills at most 64 KB of LDS
+    const int single_max = prec == FM_COMPLEX64 ? 8192 : 4096;
+    fm_status st = FM_SUCCESS;
+    if ((int64_t(1) << logL) <= single_max) {
+        g->passes = 1;
+        g->tw_full = twiddle_table(prec, logL, stream, &st);
+        return st;
+    }
+    g->logN1 = logL / 2;
+    g->logN2 = logL - g->logN1;
+    FM_REQUIRE((int64_t(1) << g->logN1) <= max_tile && (int64_t(1) << g->logN2) <= max_tile,
+               FM_ERR_NOT_IMPLEMENTED, "FFT length 2^%d exceeds the two-level limit", logL);
+    g->passes = conv ? 3 : 2;
+    g->tw1 = twiddle_table(prec, g->logN1, stream, &st);
+    if (st) return st;
+    g->tw2 = twiddle_table(prec, g->logN2, stream, &st);
+    if (st) return st;
+    g->twb_shift = (logL + 1) / 2;
+    // hi: W_L^{a 2^s} = W_{L/2^s}^a (full table), lo: W_L^b for b < 2^s
+    g->twb_hi = table(prec, logL - g->twb_shift, int64_t(1) << (logL - g->twb_shift), stream, &st);
+    if (st) return st;
+    g->twb_lo = table(prec, logL, int64_t(1) << g->twb_shift, stream, &st);
+    return st;
+}
+
+struct PipeIO {
+    const void *x;
+    int64_t ldx;
+    void *y;
+    int64_t ldy;
+    int64_t k;
+    int64_t n_in;   // valid input length (zero-pad beyond)
+    int64_t m_out;  // stored output length (crop)
+    const void *pre, *post;  // chirps (Bluestein) or null
+    const void *D;           // spectrum (conv) or null
+    int conj_d;
+    int conj_io;             // conj on load and on store (adjoint DFT)
+    void *work;              // L*k elements for multi-pass
+};
+
+template <typename T>
+static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t stream) {
+    const bool conv = io.D != nullptr;
+    const int64_t L = int64_t(1) << g.logL;
+    const int max_tile = max_tile_elems<T>();
+    if (g.passes == 1) {
+        TileArgs<T> a = blank_args<T>();
+        a.logN = g.logL;
+        a.O = io.k;
+        a.J = 1;
+        a.B = pick_batch(g.logL, io.k, 4096, 64);
+        a.ntiles = ceil_div(io.k, a.B);
+        a.x = (const cpx<T> *)io.x;
+        a.in_os = io.ldx; a.in_js = 0; a.in_ns = 1;
+        a.in_gn = 1; a.in_gj = 0; a.n_valid = io.n_in;
+        a.pre = (const cpx<T> *)io.pre;
+        a.conj_in = io.conj_io;
+        a.tw = (const cpx<T> *)g.tw_full;
+        a.D = (const cpx<T> *)io.D;
+        a.d_gn = 1; a.d_gj = 0; a.conj_d = io.conj_d;
+        a.y = (cpx<T> *)io.y;
+        a.out_os = io.ldy; a.out_js = 0; a.out_ns = 1;
+        a.out_gn = 1; a.out_gj = 0; a.m_valid = io.m_out;
+        a.post = (const cpx<T> *)io.post;
+        a.conj_out = io.conj_io;
+        return launch_tile<T>(a, conv, stream);
+    }
+    FM_REQUIRE(io.work != nullptr, FM_ERR_VALUE, "multi-pass transform needs a workspace");
+    const int64_t N1 = int64_t(1) << g.logN1, N2 = int64_t(1) << g.logN2;
+    cpx<T> *tmp = (cpx<T> *)io.work;
+    // pass A: strided N1-point FFTs over n1 for every n2, twiddle W_L^{n2 k1}
+    {
+        TileArgs<T> a = blank_args<T>();
+        a.logN = g.logN1;
+        a.O = io.k; a.J = N2;
+        a.B = pick_batch(g.logN1, N2, max_tile, 16);
+        a.ntiles = ceil_div(io.k * N2, a.B);
+        a.x = (const cpx<T> *)io.x;
+        a.in_os = io.ldx; a.in_js = 1; a.in_ns = N2;
+        a.in_gn = N2; a.in_gj = 1; a.n_valid = io.n_in;
+        a.pre = (const cpx<T> *)io.pre;
+        a.conj_in = io.conj_io;
+        a.seqfast_in = a.B > 1;
+        a.tw = (const cpx<T> *)g.tw1;
+        a.twb_hi = (const cpx<T> *)g.twb_hi; a.twb_lo = (const cpx<T> *)g.twb_lo;
+        a.twb_shift = g.twb_shift; a.twb_sign = -1;
+        a.y = tmp;
+        a.out_os = L; a.out_js = 1; a.out_ns = N2;
+        a.seqfast_out = a.B > 1;
+        fm_status st = launch_tile<T>(a, false, stream);
+        if (st) return st;
+    }
+    if (!conv) {
+        // pass B: N2-point FFTs along rows k1, transposed store y[k1 + N1 k2]
+        TileArgs<T> a = blank_args<T>();
+        a.logN = g.logN2;
+        a.O = io.k; a.J = N1;
+        a.B = pick_batch(g.logN2, N1, max_tile, 16);
+        a.ntiles = ceil_div(io.k * N1, a.B);
+        a.x = tmp;
+        a.in_os = L; a.in_js = N2; a.in_ns = 1;
+        a.tw = (const cpx<T> *)g.tw2;
+        a.y = (cpx<T> *)io.y;
+        a.out_os = io.ldy; a.out_js = 1; a.out_ns = N1;
+        a.out_gn = N1; a.out_gj = 1; a.m_valid = io.m_out;
+        a.post = (const cpx<T> *)io.post;
+        a.conj_out = io.conj_io;
+        a.seqfast_out = a.B > 1;
+        return launch_tile<T>(a, false, stream);
+    }
+    // pass B (conv): rows k1: FFT, spectrum D[k1 + N1 k2], inverse FFT,
+    // twiddle W_L^{-n2 k1}; in place in the workspace
+    {
+        TileArgs<T> a = blank_args<T>();
+        a.logN = g.logN2;
+        a.O = io.k; a.J = N1;
+        a.B = pick_batch(g.logN2, N1, max_tile, 2);
+        a.ntiles = ceil_div(io.k * N1, a.B);
+        a.x = tmp;
+        a.in_os = L; a.in_js = N2; a.in_ns = 1;
+        a.tw = (const cpx<T> *)g.tw2;
+        a.D = (const cpx<T> *)io.D;
+        a.d_gn = N1; a.d_gj = 1; a.conj_d = io.conj_d;
+        a.twb_hi = (const cpx<T> *)g.twb_hi; a.twb_lo = (const cpx<T> *)g.twb_lo;
+        a.twb_shift = g.twb_shift; a.twb_sign = +1;
+        a.y = tmp;
+        a.out_os = L; a.out_js = N2; a.out_ns = 1;
+        fm_status st = launch_tile<T>(a, true, stream);
+        if (st) return st;
+    }
+    // pass C: strided inverse N1-point FFTs over k1 -> y[n2 + N2 n1], crop
+    {
+        TileArgs<T> a = blank_args<T>();
+        a.logN = g.logN1;
+        a.O = io.k; a.J = N2;
+        a.B = pick_batch(g.logN1, N2, max_tile, 16);
+        a.ntiles = ceil_div(io.k * N2, a.B);
+        a.x = tmp;
+        a.in_os = L; a.in_js = 1; a.in_ns = N2;
+        a.seqfast_in = a.B > 1;
+        a.sign1 = +1;
+        a.tw = (const cpx<T> *)g.tw1;
+        a.y = (cpx<T> *)io.y;
+        a.out_os = io.ldy; a.out_js = 1; a.out_ns = N2;
+        a.out_gn = N2; a.out_gj = 1; a.m_valid = io.m_out;
+        a.post = (const cpx<T> *)io.post;
+        a.conj_out = io.conj_io;
+        a.seqfast_out = a.B > 1;
+        return launch_tile<T>(a, false, stream);
+    }
+}
+
+static fm_status run_pipeline_any(const Geometry &g, const PipeIO &io, hipStream_t stream) {
+    return g.prec == FM_COMPLEX64 ? run_pipeline<float>(g, io, stream)
+                                  : run_pipeline<double>(g, io, stream);
+}
+
+// Strided single-pass application along a mode axis (Kron): vectors
+// v[i] = x[(o*N + i)*inner + j].
+template <typename T>
+static fm_status run_strided(const Geometry &g, const void *x, void *y, int64_t outer,
+                             int64_t inner, int conj_io, const void *D, int conj_d,
+                             hipStream_t stream) {
+    FM_REQUIRE(g.passes == 1, FM_ERR_NOT_IMPLEMENTED,
+               "strided mode application needs a single-pass transform");
+    const int64_t N = int64_t(1) << g.logL;
+    TileArgs<T> a = blank_args<T>();
+    a.logN = g.logL;
+    a.O = outer; a.J = inner;
+    int want = inner >= 16 ? 16 : 1;
+    a.B = pick_batch(g.logL, outer * inner, max_tile_elems<T>() / 2, inner > 1 ? want : 64);
+    a.ntiles = ceil_div(outer * inner, a.B);
+    a.x = (const cpx<T> *)x;
+    a.in_os = N * inner; a.in_js = 1; a.in_ns = inner;
+    a.conj_in = conj_io;
+    a.seqfast_in = (inner > 1 && a.B > 1);
+    a.tw = (const cpx<T> *)g.tw_full;
+    a.D = (const cpx<T> *)D;
+    a.d_gn = 1; a.d_gj = 0; a.conj_d = conj_d;
+    a.y = (cpx<T> *)y;
+    a.out_os = N * inner; a.out_js = 1; a.out_ns = inner;
+    a.conj_out = conj_io;
+    a.seqfast_out = a.seqfast_in;
+    return launch_tile<T>(a, D != nullptr, stream);
+}
+
+// FP64 in-place forward FFT of one length-L complex128 vector (plan set-up
+// only; synchronises the stream before freeing its workspace).
+static fm_status fft64_inplace(cpx<double> *v, int logL, hipStream_t stream) {
+    Geometry g;
+    fm_status st = make_geometry(FM_COMPLEX128, logL, false, stream, &g);
+    if (st) return st;
+    const int64_t L = int64_t(1) << logL;
+    void *work = nullptr;
+    if (g.passes > 1) FM_CHECK_HIP(hipMalloc(&work, size_t(L) * 16));
+    void *out = nullptr;
+    if (g.passes > 1) {
+        // pass A reads v and pass B writes y: keep them distinct
+        FM_CHECK_HIP(hipMalloc(&out, size_t(L) * 16));
+    }
+    PipeIO io;
+    memset(&io, 0, sizeof(io));
+    io.x = v; io.ldx = L; io.y = out ? out : v; io.ldy = L; io.k = 1;
+    io.n_in = L; io.m_out = L; io.work = work;
+    st = run_pipeline<double>(g, io, stream);
+    if (!st && out)
+        if (hipMemcpyAsync(v, out, size_t(L) * 16, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+            st = FM_ERR_HIP;
+    hipError_t e = hipStreamSynchronize(stream);
+    if (work) hipFree(work);
+    if (out) hipFree(out);
+    if (!st && e != hipSuccess) {
+        set_error("FP64 set-up FFT failed: %s", hipGetErrorString(e));
+        st = FM_ERR_HIP;
+    }
+    return st;
+}
+
+// Spectrum D = fft(t')/L of a complex128 generator vector, stored in `prec`.
+static fm_status spectrum_from(cpx<double> *t, int logL, int prec, void *D, hipStream_t stream) {
+    const int64_t L = int64_t(1) << logL;
+    fm_status st = fft64_inplace(t, logL, stream);
+    if (st) return st;
+    if (prec == FM_COMPLEX64)
+        hipLaunchKernelGGL(scale_cast_kernel<float>, grid1d(L), 256, 0, stream,
+                           (cpx<float> *)D, t, L, 1.0 / double(L));
+    else
+        hipLaunchKernelGGL(scale_cast_kernel<double>, grid1d(L), 256, 0, stream,
+                           (cpx<double> *)D, t, L, 1.0 / double(L));
+    FM_CHECK_LAUNCH();
+    FM_CHECK_HIP(hipStreamSynchronize(stream));
+    return FM_SUCCESS;
+}
+
+static inline size_t esize(int prec) { return prec == FM_COMPLEX64 ? 8 : 16; }
+
+}  // namespace fm
+
+using namespace fm;
+
+// ======================================================================
+//                                 Fourier
+// ======================================================================
+struct fm_fourier_plan {
+    int device;
+    int prec;
+    int64_t N;
+    int bluestein;
+    int64_t L;
+    Geometry geo;
+    void *chirp;  // p[n] = exp(-i pi n^2 / N), N entries (Bluestein)
+    void *D;      // fft(g)/L, L entries (Bluestein)
+};
+
+extern "C" int fm_abi_version(void) { return FM_ABI_VERSION; }
+extern "C" const char *fm_get_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" fm_status fm_fourier_plan_create(int64_t order, int precision, void *stream,
+                                            fm_fourier_plan **plan) {
+    FM_REQUIRE(plan != nullptr, FM_ERR_VALUE, "plan pointer is NULL");
+    *plan = nullptr;
+    FM_REQUIRE(order >= 1, FM_ERR_VALUE, "Fourier order cannot be smaller than 1.");
+    FM_REQUIRE(order < (int64_t(1) << 30), FM_ERR_NOT_IMPLEMENTED,
+               "Fourier order %lld exceeds the supported maximum 2^30", (long long)order);
+    FM_REQUIRE(precision == FM_COMPLEX64 || precision == FM_COMPLEX128, FM_ERR_TYPE,
+               "Fourier precision must be complex64 or complex128");
+    hipStream_t s = as_stream(stream);
+    auto *p = new fm_fourier_plan();
+    memset(p, 0, sizeof(*p));
+    hipGetDevice(&p->device);
+    p->prec = precision;
+    p->N = order;
+    fm_status st = FM_SUCCESS;
+    if (is_pow2(order)) {
+        p->L = order;
+        st = make_geometry(precision, ilog2(order), false, s, &p->geo);
+    } else {
+        p->bluestein = 1;
+        int logL = ilog2(2 * order - 1);
+        p->L = int64_t(1) << logL;
+        st = make_geometry(precision, logL, true, s, &p->geo);
+        if (!st && hipMalloc(&p->chirp, esize(precision) * size_t(order)) != hipSuccess) st = FM_ERR_ALLOC;
+        if (!st && hipMalloc(&p->D, esize(precision) * size_t(p->L)) != hipSuccess) st = FM_ERR_ALLOC;
+        cpx<double> *g = nullptr;
+        if (!st && hipMalloc(&g, 16 * size_t(p->L)) != hipSuccess) st = FM_ERR_ALLOC;
+        if (st == FM_ERR_ALLOC) set_error("device allocation for Bluestein plan failed");
+        if (!st) {
+            if (precision == FM_COMPLEX64)
+                hipLaunchKernelGGL(gen_chirp_kernel<float>, grid1d(order), 256, 0, s,
+                                   (cpx<float> *)p->chirp, order, -1.0);
+            else
+                hipLaunchKernelGGL(gen_chirp_kernel<double>, grid1d(order), 256, 0, s,
+                                   (cpx<double> *)p->chirp, order, -1.0);
+            hipLaunchKernelGGL(gen_bluestein_kernel, grid1d(p->L), 256, 0, s, g, order, p->L);
+            if (hipGetLastError() != hipSuccess) st = FM_ERR_HIP;
+        }
+        if (!st) st = spectrum_from(g, logL, precision, p->D, s);
+        if (g) hipFree(g);
+    }
+    if (st) {
+        fm_fourier_plan_destroy(p);
+        return st;
+    }
+    *plan = p;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_fourier_plan_info(const fm_fourier_plan *p, int64_t *fft_length,
+                                          int *bluestein, int *passes) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    if (fft_length) *fft_length = p->L;
+    if (bluestein) *bluestein = p->bluestein;
+    if (passes) *passes = p->geo.passes;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_fourier_workspace_size(const fm_fourier_plan *p, int64_t k,
+                                               int64_t *bytes) {
+    FM_REQUIRE(p != nullptr && bytes != nullptr, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
+    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L * k : 0;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_fourier_apply(const fm_fourier_plan *p, int adjoint, const void *x,
+                                      int64_t ldx, void *y, int64_t ldy, int64_t k,
+                                      void *workspace, int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
+    if (k == 0) return FM_SUCCESS;
+    FM_REQUIRE(x && y, FM_ERR_VALUE, "NULL data pointer");
+    FM_REQUIRE(ldx >= p->N && ldy >= p->N, FM_ERR_VALUE,
+               "leading dimensions (%lld, %lld) below the order %lld", (long long)ldx,
+               (long long)ldy, (long long)p->N);
+    int64_t need = 0;
+    fm_fourier_workspace_size(p, k, &need);
+    FM_REQUIRE(workspace_bytes >= need, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)need);
+    PipeIO io;
+    memset(&io, 0, sizeof(io));
+    io.x = x; io.ldx = ldx; io.y = y; io.ldy = ldy; io.k = k;
+    io.n_in = p->N; io.m_out = p->N;
+    io.conj_io = adjoint ? 1 : 0;
+    io.work = workspace;
+    if (p->bluestein) {
+        io.pre = p->chirp;
+        io.post = p->chirp;
+        io.D = p->D;
+    }
+    return run_pipeline_any(p->geo, io, as_stream(stream));
+}
+
+extern "C" fm_status fm_fourier_apply_strided(const fm_fourier_plan *p, int adjoint,
+                                              const void *x, void *y, int64_t outer,
+                                              int64_t inner, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(outer >= 0 && inner >= 1, FM_ERR_VALUE, "bad strided geometry");
+    FM_REQUIRE(!p->bluestein, FM_ERR_NOT_IMPLEMENTED,
+               "strided application needs a power-of-two order");
+    if (outer == 0) return FM_SUCCESS;
+    if (p->prec == FM_COMPLEX64)
+        return run_strided<float>(p->geo, x, y, outer, inner, adjoint, nullptr, 0, as_stream(stream));
+    return run_strided<double>(p->geo, x, y, outer, inner, adjoint, nullptr, 0, as_stream(stream));
+}
+
+extern "C" fm_status fm_fourier_plan_destroy(fm_fourier_plan *p) {
+    if (!p) return FM_SUCCESS;
+    if (p->chirp) hipFree(p->chirp);
+    if (p->D) hipFree(p->D);
+    delete p;
+    return FM_SUCCESS;
+}
+
+// ======================================================================
+//                          Circulant / Toeplitz
+// ======================================================================
+struct fm_conv_plan {
+    int device;
+    int prec;
+    int kind;          // 0 circulant, 1 toeplitz
+    int64_t rows, cols, gen_len, L;
+    int64_t head, tail;  // t' = [gen[:head], 0..., gen[head:head+tail]]
+    Geometry geo;      // conv pipeline (1 or 3 passes)
+    Geometry geo_fft;  // plain length-L FFT (1 or 2 passes), generator gradient
+    void *D;           // fft(t')/L in prec, L entries
+};
+
+static fm_status conv_plan_create(int prec, int kind, int64_t m, int64_t n, int64_t L,
+                                  int64_t head, int64_t tail, const void *gen,
+                                  int64_t gen_len, hipStream_t s, fm_conv_plan **out) {
+    auto *p = new fm_conv_plan();
+    memset(p, 0, sizeof(*p));
+    hipGetDevice(&p->device);
+    p->prec = prec; p->kind = kind; p->rows = m; p->cols = n; p->L = L;
+    p->gen_len = gen_len;
+    p->head = head;
+    p->tail = tail;
+    const int logL = ilog2(L);
+    fm_status st = make_geometry(prec, logL, true, s, &p->geo);
+    if (!st) st = make_geometry(prec, logL, false, s, &p->geo_fft);
+    cpx<double> *t = nullptr;
+    if (!st && hipMalloc(&p->D, esize(prec) * size_t(L)) != hipSuccess) st = FM_ERR_ALLOC;
+    if (!st && hipMalloc(&t, 16 * size_t(L)) != hipSuccess) st = FM_ERR_ALLOC;
+    if (st == FM_ERR_ALLOC) set_error("device allocation for convolution plan failed");
+    if (!st) {
+        hipLaunchKernelGGL(embed_kernel, grid1d(L), 256, 0, s, t, (const cpx<double> *)gen, L,
+                           head, tail);
+        if (hipGetLastError() != hipSuccess) st = FM_ERR_HIP;
+    }
+    if (!st) st = spectrum_from(t, logL, prec, p->D, s);
+    if (t) hipFree(t);
+    if (st) {
+        fm_conv_plan_destroy(p);
+        return st;
+    }
+    *out = p;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_circulant_plan_create(int precision, int64_t n, const void *generator,
+                                              void *stream, fm_conv_plan **plan) {
+    FM_REQUIRE(plan != nullptr, FM_ERR_VALUE, "plan pointer is NULL");
+    *plan = nullptr;
+    FM_REQUIRE(n >= 1, FM_ERR_VALUE, "Column-definition tensor must be at least 1D.");
+    FM_REQUIRE(generator != nullptr, FM_ERR_VALUE, "generator is NULL");
+    FM_REQUIRE(precision == FM_COMPLEX64 || precision == FM_COMPLEX128, FM_ERR_TYPE,
+               "Circulant precision must be complex64 or complex128");
+    int64_t L = is_pow2(n) ? n : (int64_t(1) << ilog2(2 * n - 1));
+    // t' = c (L == n) or [c, 0, ..., 0, c[1:]] (L >= 2n - 1)
+    int64_t head = n, tail = L == n ? 0 : n - 1;
+    // the tail copies c[1:], i.e. generator entries head-... : use an offset view
+    // by embedding from gen = c with head = n and tail entries c[1..n-1]
+    fm_status st;
+    if (tail == 0) {
+        st = conv_plan_create(precision, 0, n, n, L, head, 0, generator, n, as_stream(stream), plan);
+    } else {
+        // build [c, c[1:]] so the generic embed (head, tail) applies
+        cpx<double> *gen2 = nullptr;
+        FM_CHECK_HIP(hipMalloc(&gen2, 16 * size_t(2 * n - 1)));
+        hipStream_t s = as_stream(stream);
+        hipMemcpyAsync(gen2, generator, 16 * size_t(n), hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(gen2 + n, (const cpx<double> *)generator + 1, 16 * size_t(n - 1),
+                       hipMemcpyDeviceToDevice, s);
+        st = conv_plan_create(precision, 0, n, n, L, head, tail, gen2, n, s, plan);
+        hipStreamSynchronize(s);
+        hipFree(gen2);
+    }
+    return st;
+}
+
+extern "C" fm_status fm_toeplitz_plan_create(int precision, int64_t m, int64_t n,
+                                             const void *generator, void *stream,
+                                             fm_conv_plan **plan) {
+    FM_REQUIRE(plan != nullptr, FM_ERR_VALUE, "plan pointer is NULL");
+    *plan = nullptr;
+    FM_REQUIRE(m >= 1 && n >= 1, FM_ERR_VALUE, "Toeplitz dimensions must be positive");
+    FM_REQUIRE(generator != nullptr, FM_ERR_VALUE, "generator is NULL");
+    FM_REQUIRE(precision == FM_COMPLEX64 || precision == FM_COMPLEX128, FM_ERR_TYPE,
+               "Toeplitz precision must be complex64 or complex128");
+    const int64_t d = m + n - 1;
+    const int64_t L = int64_t(1) << ilog2(d);
+    return conv_plan_create(precision, 1, m, n, L, m, n - 1, generator, d, as_stream(stream), plan);
+}
+
+extern "C" fm_status fm_conv_plan_info(const fm_conv_plan *p, int64_t *rows, int64_t *cols,
+                                       int64_t *fft_length, int *passes) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    if (rows) *rows = p->rows;
+    if (cols) *cols = p->cols;
+    if (fft_length) *fft_length = p->L;
+    if (passes) *passes = p->geo.passes;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_conv_workspace_size(const fm_conv_plan *p, int64_t k, int64_t *bytes) {
+    FM_REQUIRE(p != nullptr && bytes != nullptr, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
+    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L * k : 0;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_conv_apply(const fm_conv_plan *p, int adjoint, const void *x,
+                                   int64_t ldx, void *y, int64_t ldy, int64_t k, void *workspace,
+                                   int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
+    if (k == 0) return FM_SUCCESS;
+    FM_REQUIRE(x && y, FM_ERR_VALUE, "NULL data pointer");
+    const int64_t nin = adjoint ? p->rows : p->cols;
+    const int64_t mout = adjoint ? p->cols : p->rows;
+    FM_REQUIRE(ldx >= nin && ldy >= mout, FM_ERR_VALUE, "leading dimension too small");
+    int64_t need = 0;
+    fm_conv_workspace_size(p, k, &need);
+    FM_REQUIRE(workspace_bytes >= need, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)need);
+    PipeIO io;
+    memset(&io, 0, sizeof(io));
+    io.x = x; io.ldx = ldx; io.y = y; io.ldy = ldy; io.k = k;
+    io.n_in = nin; io.m_out = mout;
+    io.D = p->D; io.conj_d = adjoint ? 1 : 0;
+    io.work = workspace;
+    return run_pipeline_any(p->geo, io, as_stream(stream));
+}
+
+extern "C" fm_status fm_conv_apply_strided(const fm_conv_plan *p, int adjoint, const void *x,
+                                           void *y, int64_t outer, int64_t inner, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->rows == p->cols && p->L == p->rows, FM_ERR_NOT_IMPLEMENTED,
+               "strided application needs a square plan without padding");
+    FM_REQUIRE(outer >= 0 && inner >= 1, FM_ERR_VALUE, "bad strided geometry");
+    if (outer == 0) return FM_SUCCESS;
+    if (p->prec == FM_COMPLEX64)
+        return run_strided<float>(p->geo, x, y, outer, inner, 0, p->D, adjoint, as_stream(stream));
+    return run_strided<double>(p->geo, x, y, outer, inner, 0, p->D, adjoint, as_stream(stream));
+}
+
+namespace fm {
+// acc[q] = sum_c G^[c, q] * conj(X^[c, q])
+template <typename T>
+__global__ void corr_reduce_kernel(const cpx<T> *__restrict__ gh, const cpx<T> *__restrict__ xh,
+                                   cpx<T> *__restrict__ acc, int64_t L, int64_t k) {
+    const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (q >= L) return;
+    cpx<T> s = mk(T(0), T(0));
+    for (int64_t c = 0; c < k; ++c) s = s + cmulc(gh[c * L + q], xh[c * L + q]);
+    acc[q] = s;
+}
+
+// Fold the gradient over t' (length L, unnormalised inverse transform r)
+// back onto the generator: Circulant c[j] <- r[j] (+ r[L - tail + j - 1] for
+// j >= 1 when padded); Toeplitz tenT[i] <- r[i] (i < head) or
+// r[L - tail + i - head].
+template <typename T>
+__global__ void fold_grad_kernel(const cpx<T> *__restrict__ r, cpx<T> *__restrict__ out,
+                                 int kind, int64_t L, int64_t head, int64_t tail,
+                                 int64_t gen_len, T s) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= gen_len) return;
+    cpx<T> v;
+    if (kind == 0) {
+        v = r[i];
+        if (tail > 0 && i >= 1) v = v + r[L - tail + i - 1];
+    } else {
+        v = i < head ? r[i] : r[L - tail + (i - head)];
+    }
+    out[i] = scale(v, s);
+}
+
+template <typename T>
+static fm_status conv_grad(const fm_conv_plan *p, const void *x, int64_t ldx, const void *g,
+                           int64_t ldg, int64_t k, void *grad, void *ws, hipStream_t s) {
+    const int64_t L = p->L;
+    cpx<T> *xh = (cpx<T> *)ws;
+    cpx<T> *gh = xh + L * k;
+    cpx<T> *tmp = gh + L * k;
+    cpx<T> *acc = tmp + L * k;
+    cpx<T> *r = acc + L;
+    PipeIO io;
+    memset(&io, 0, sizeof(io));
+    io.x = x; io.ldx = ldx; io.y = xh; io.ldy = L; io.k = k;
+    io.n_in = p->cols; io.m_out = L; io.work = tmp;
+    fm_status st = run_pipeline<T>(p->geo_fft, io, s);
+    if (st) return st;
+    io.x = g; io.ldx = ldg; io.y = gh; io.n_in = p->rows;
+    st = run_pipeline<T>(p->geo_fft, io, s);
+    if (st) return st;
+    hipLaunchKernelGGL(corr_reduce_kernel<T>, grid1d(L), 256, 0, s, gh, xh, acc, L, k);
+    FM_CHECK_LAUNCH();
+    // unnormalised inverse transform: conj(FFT(conj(acc)))
+    io.x = acc; io.ldx = L; io.y = r; io.ldy = L; io.k = 1; io.n_in = L; io.m_out = L;
+    io.conj_io = 1; io.work = tmp;
+    st = run_pipeline<T>(p->geo_fft, io, s);
+    if (st) return st;
+    hipLaunchKernelGGL(fold_grad_kernel<T>, grid1d(p->gen_len), 256, 0, s, r, (cpx<T> *)grad,
+                       p->kind, L, p->head, p->tail, p->gen_len, T(1.0 / double(L)));
+    FM_CHECK_LAUNCH();
+    return FM_SUCCESS;
+}
+}  // namespace fm
+
+extern "C" fm_status fm_conv_generator_grad_workspace_size(const fm_conv_plan *p, int64_t k,
+                                                           int64_t *bytes) {
+    FM_REQUIRE(p != nullptr && bytes != nullptr, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
+    const int64_t kk = k > 1 ? k : 1;
+    *bytes = int64_t(esize(p->prec)) * p->L * (3 * kk + 2);
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_conv_generator_grad(const fm_conv_plan *p, const void *x, int64_t ldx,
+                                            const void *g, int64_t ldg, int64_t k, void *grad,
+                                            void *workspace, int64_t workspace_bytes,
+                                            void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(k >= 1, FM_ERR_VALUE, "generator gradient needs at least one column");
+    FM_REQUIRE(x && g && grad && workspace, FM_ERR_VALUE, "NULL data pointer");
+    FM_REQUIRE(ldx >= p->cols && ldg >= p->rows, FM_ERR_VALUE, "leading dimension too small");
+    int64_t need = 0;
+    fm_conv_generator_grad_workspace_size(p, k, &need);
+    FM_REQUIRE(workspace_bytes >= need, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)need);
+    if (p->prec == FM_COMPLEX64)
+        return conv_grad<float>(p, x, ldx, g, ldg, k, grad, workspace, as_stream(stream));
+    return conv_grad<double>(p, x, ldx, g, ldg, k, grad, workspace, as_stream(stream));
+}
+
+extern "C" fm_status fm_conv_plan_destroy(fm_conv_plan *p) {
+    if (!p) return FM_SUCCESS;
+    if (p->D) hipFree(p->D);
+    delete p;
+    return FM_SUCCESS;
+}

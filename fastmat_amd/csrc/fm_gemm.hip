@@ -1,0 +1,212 @@
+// Dense factors on FP32 MFMA (v_mfma_f32_16x16x4f32, FP32 accumulation).
+//
+// Replaces the dense Matrix transform (fastmat/Matrix.pyx:1831-1842,
+// 1924-1935: ndarray.dot) for the dense factors of a Kron product
+// (Kron.pyx:267-341) and for plain dense matrices.
+//
+// One kernel computes Y[i, c] = sum_l A_eff[i, l] * Bm[l, c] with strided
+// addressing:
+//   A_eff[i, l] = A[i*a_rs + l*a_cs]                  (transpose by strides)
+//   Bm[l, c]    = X[s*(o*bos + l*brs + j*bjs) + p]    c -> (o, j, p)
+//   Y[i, c]     = Y[s*(o*yos + i*yrs + j*yjs) + p]
+// where s = 2 and p = c & 1 for complex data (real and imaginary parts become
+// neighbouring real columns, so a real A multiplies complex data without
+// widening) and s = 1, p = 0 otherwise.  A complex A is handled by the host
+// as the real 2m x 2n matrix [[Re, -Im], [Im, Re]] interleaved to match the
+// (re, im) storage of X and Y.
+//
+// Tiling: 256 threads (4 wave64) own a 64 x 64 output tile; each wave owns a
+// 32 x 32 quarter as 2 x 2 MFMA 16x16 blocks; K advances in slabs of 16
+// staged through LDS.
+#include "fm_common.hpp"
+
+namespace fm {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct GemmArgs {
+    int64_t M, K, NC;
+    const float *A;
+    int64_t a_rs, a_cs;
+    const float *X;
+    int64_t J, bos, brs, bjs;
+    float *Y;
+    int64_t yos, yrs, yjs;
+    int cplx;
+};
+
+constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
+
+__device__ __forceinline__ int64_t col_base(const GemmArgs &g, int64_t c, int64_t os, int64_t js,
+                                            int *part) {
+    int64_t cc = c;
+    *part = 0;
+    if (g.cplx) {
+        *part = int(c & 1);
+        cc = c >> 1;
+    }
+    const int64_t o = cc / g.J, j = cc - o * g.J;
+    return o * os + j * js;
+}
+
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(const GemmArgs g) {
+    __shared__ float As[GT_K][GT_M + 4];
+    __shared__ float Bs[GT_K][GT_N + 4];
+    __shared__ int64_t bbase[GT_N];
+    __shared__ int bpart[GT_N];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t tiles_n = ceil_div(g.NC, GT_N);
+    const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t tm = tile / tiles_n, tn = tile - tm * tiles_n;
+    const int64_t row0 = tm * GT_M, col0 = tn * GT_N;
+    const int s = g.cplx ? 2 : 1;
+
+    if (tid < GT_N) {
+        const int64_t c = col0 + tid;
+        int part = 0;
+        bbase[tid] = c < g.NC ? col_base(g, c, g.bos, g.bjs, &part) : -1;
+        bpart[tid] = part;
+    }
+    __syncthreads();
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const bool a_kfast = g.a_cs == 1;
+    const bool b_kfast = g.brs == 1;
+
+    for (int64_t k0 = 0; k0 < g.K; k0 += GT_K) {
+        // stage A (64 x 16) and B (16 x 64): 4 elements per thread each
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = tid + e * 256;
+            int kk, i;
+            if (a_kfast) { kk = idx & 15; i = idx >> 4; }
+            else { i = idx & 63; kk = idx >> 6; }
+            const int64_t gi = row0 + i, gk = k0 + kk;
+            As[kk][i] = (gi < g.M && gk < g.K) ? g.A[gi * g.a_rs + gk * g.a_cs] : 0.f;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = tid + e * 256;
+            int kk, c;
+            if (b_kfast) { kk = idx & 15; c = idx >> 4; }
+            else { c = idx & 63; kk = idx >> 6; }
+            const int64_t gk = k0 + kk;
+            const int64_t base = bbase[c];
+            Bs[kk][c] = (base >= 0 && gk < g.K) ? g.X[s * (base + gk * g.brs) + bpart[c]] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GT_K; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            float av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = As[kr][wr + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = Bs[kr][wc + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // D layout of 16x16x4 f32: lane holds column (lane & 15), rows 4*(lane>>4) + v
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int c = wc + b * 16 + (lane & 15);
+        const int64_t gc = col0 + c;
+        if (gc >= g.NC) continue;
+        int part = 0;
+        const int64_t ybase = col_base(g, gc, g.yos, g.yjs, &part);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gi = row0 + wr + a * 16 + 4 * (lane >> 4) + v;
+                if (gi < g.M) g.Y[s * (ybase + gi * g.yrs) + part] = acc[a][b][v];
+            }
+    }
+}
+
+static fm_status gemm_launch(const GemmArgs &g, hipStream_t stream) {
+    const int64_t tiles = ceil_div(g.M, GT_M) * ceil_div(g.NC, GT_N);
+    FM_REQUIRE(tiles < (int64_t(1) << 31), FM_ERR_VALUE, "GEMM too large");
+    if (tiles == 0) return FM_SUCCESS;
+    hipLaunchKernelGGL(gemm_mfma_kernel, dim3(unsigned(tiles)), dim3(256), 0, stream, g);
+    FM_CHECK_LAUNCH();
+    return FM_SUCCESS;
+}
+
+}  // namespace fm
+
+using namespace fm;
+
+// a_dtype FLOAT32 with x FLOAT32 or COMPLEX64 run directly; COMPLEX64 A is
+// passed by the host as its real 2m x 2n interleaved form with a_dtype
+// FLOAT32 and x COMPLEX64 viewed as real (x_dtype FLOAT32, n and m doubled).
+extern "C" fm_status fm_dense_apply(int a_dtype, int x_dtype, int adjoint, int64_t m, int64_t n,
+                                    const void *A, int64_t lda, const void *x, int64_t ldx,
+                                    void *y, int64_t ldy, int64_t k, void *stream) {
+    FM_REQUIRE(a_dtype == FM_FLOAT32, FM_ERR_TYPE,
+               "dense apply supports float32 matrices (complex ones via their real form)");
+    FM_REQUIRE(x_dtype == FM_FLOAT32 || x_dtype == FM_COMPLEX64, FM_ERR_TYPE,
+               "dense apply supports float32 or complex64 data");
+    FM_REQUIRE(m >= 0 && n >= 0 && k >= 0, FM_ERR_VALUE, "negative size");
+    if (m == 0 || k == 0) return FM_SUCCESS;
+    FM_REQUIRE(A && x && y, FM_ERR_VALUE, "NULL data pointer");
+    const int64_t out_rows = adjoint ? n : m, in_rows = adjoint ? m : n;
+    FM_REQUIRE(ldx >= in_rows && ldy >= out_rows && lda >= n, FM_ERR_VALUE,
+               "leading dimension too small");
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = out_rows;
+    g.K = in_rows;
+    g.A = (const float *)A;
+    g.a_rs = adjoint ? 1 : lda;
+    g.a_cs = adjoint ? lda : 1;
+    g.X = (const float *)x;
+    g.cplx = x_dtype == FM_COMPLEX64;
+    g.NC = g.cplx ? 2 * k : k;
+    g.J = k;
+    g.bos = 0; g.brs = 1; g.bjs = ldx;
+    g.Y = (float *)y;
+    g.yos = 0; g.yrs = 1; g.yjs = ldy;
+    return gemm_launch(g, as_stream(stream));
+}
+
+extern "C" fm_status fm_dense_apply_strided(int a_dtype, int x_dtype, int adjoint, int64_t n,
+                                            const void *A, int64_t lda, const void *x, void *y,
+                                            int64_t outer, int64_t inner, void *stream) {
+    FM_REQUIRE(a_dtype == FM_FLOAT32, FM_ERR_TYPE, "dense apply supports float32 matrices");
+    FM_REQUIRE(x_dtype == FM_FLOAT32 || x_dtype == FM_COMPLEX64, FM_ERR_TYPE,
+               "dense apply supports float32 or complex64 data");
+    FM_REQUIRE(n >= 1 && outer >= 0 && inner >= 1, FM_ERR_VALUE, "bad strided geometry");
+    if (outer == 0) return FM_SUCCESS;
+    FM_REQUIRE(A && x && y && x != y, FM_ERR_VALUE, "bad data pointers (in-place not allowed)");
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.M = n;
+    g.K = n;
+    g.A = (const float *)A;
+    g.a_rs = adjoint ? 1 : lda;
+    g.a_cs = adjoint ? lda : 1;
+    g.X = (const float *)x;
+    g.cplx = x_dtype == FM_COMPLEX64;
+    g.J = inner;
+    g.NC = (g.cplx ? 2 : 1) * outer * inner;
+    g.bos = n * inner; g.brs = inner; g.bjs = 1;
+    g.Y = (float *)y;
+    g.yos = n * inner; g.yrs = inner; g.yjs = 1;
+    return gemm_launch(g, as_stream(stream));
+}

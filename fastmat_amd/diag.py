@@ -1,0 +1,55 @@
+"""Diag: diagonal matrix (fastmat/Diag.pyx), GPU-backed element-wise kernel.
+Backward multiplies by conj(d) (Diag.pyx:159-167)."""
+import numpy as np
+
+from . import _device as D
+from . import _lib
+from .matrix import Matrix
+
+
+class Diag(Matrix):
+    def __init__(self, vecD, **options):
+        vecD = np.asarray(vecD)
+        d = D.squeeze1d(vecD)
+        if vecD.ndim != 1 or d.ndim != 1:
+            raise ValueError("Diag: Definition vector must have exactly one dimension.")
+        if d.dtype not in D.SUPPORTED:
+            raise TypeError("Unsupported data type %s" % d.dtype)
+        self._vecD = d
+        self._dev = {}
+        self._content = ()
+        self._initProperties(d.size, d.size, d.dtype, **options)
+        self._forceContiguousInput = True
+
+    @property
+    def vecD(self):
+        return self._vecD
+
+    def _d(self, dt):
+        t = self._dev.get(dt)
+        if t is None:
+            t = self._dev[dt] = D.as_tensor(self._vecD, dt)
+        return t
+
+    def _run(self, X, conj_d):
+        dt = D.promote(X.dtype, self.dtype)
+        if D.np_dtype(X.dtype) != dt:
+            X = X.to(D.torch_dtype(dt))
+        k, n = X.shape
+        Y = D.empty(k, n, dt)
+        d = self._d(dt)
+        _lib.call("fm_diag_apply", D.fm_type(dt), int(conj_d), n, _lib.ptr(d), _lib.ptr(X), n,
+                  _lib.ptr(Y), n, k, D.stream())
+        return Y
+
+    def _fwd(self, X):
+        return self._run(X, False)
+
+    def _bwd(self, X):
+        return self._run(X, True)
+
+    def _getT(self):
+        return self
+
+    def _reference(self):
+        return np.diag(self._vecD)

@@ -1,0 +1,61 @@
+"""Hadamard: unnormalised Sylvester Hadamard matrix (fastmat/Hadamard.pyx),
+GPU-backed fast Walsh-Hadamard transform.
+
+dtype int8 like the reference (Hadamard.pyx:130), so results have the input's
+promoted dtype; integer results wrap like the reference's C loop and float
+results equal the reference's bit for bit (same stage order).  Backward ==
+forward (Hadamard.pyx:232-239).
+"""
+import numpy as np
+
+from . import _device as D
+from . import _lib
+from .matrix import Matrix
+
+
+class Hadamard(Matrix):
+    def __init__(self, order, **options):
+        order = int(order)
+        if order < 1:
+            raise ValueError("Hadamard: Order must be larger than 0.")
+        maxOrder = 8 * 8 - 2
+        if order > maxOrder:
+            raise ValueError(
+                "Hadamard: Order exceeds maximum for this platform: %d" % maxOrder)
+        self._order = order
+        self._content = ()
+        self._initProperties(2 ** order, 2 ** order, np.int8, **options)
+        self._forceContiguousInput = True
+
+    @property
+    def order(self):
+        return self._order
+
+    def _fwd(self, X):
+        dt = D.promote(X.dtype, self.dtype)
+        if D.np_dtype(X.dtype) != dt:
+            X = X.to(D.torch_dtype(dt))
+        k, n = X.shape
+        Y = D.empty(k, n, dt)
+        _lib.call("fm_hadamard_apply", D.fm_type(dt), self._order, _lib.ptr(X), n, _lib.ptr(Y),
+                  n, k, D.stream())
+        return Y
+
+    _bwd = _fwd
+
+    def _apply_strided(self, X, adjoint, outer, inner):
+        Y = X.new_empty(X.shape)
+        _lib.call("fm_hadamard_apply_strided", D.fm_type(X.dtype), self._order, _lib.ptr(X),
+                  _lib.ptr(Y), outer, inner, D.stream())
+        return Y
+
+    def _strided_ok(self, dtype):
+        return True
+
+    def _reference(self):
+        n = self.numRows
+        i = np.arange(n, dtype=np.int64)
+        a = np.bitwise_and(i[:, None], i[None, :])
+        for s in (32, 16, 8, 4, 2, 1):
+            a ^= a >> s
+        return (1 - 2 * (a & 1)).astype(self.dtype)

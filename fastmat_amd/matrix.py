@@ -1,0 +1,476 @@
+"""Matrix base class and its transposition wrappers (GPU-backed).
+
+Mirrors fastmat.Matrix (fastmat/Matrix.pyx):
+  forward / backward           Matrix.pyx:1844-1910, 1937-2007
+  input preparation, dtypes    Matrix.pyx:1737-1817
+  .H / .T / .conj              Matrix.pyx:1209-1269, Hermitian 2224-2300,
+                               Conjugate 2326-2412, Transpose 2417-2482
+  operator overloads           Matrix.pyx:1663-1734
+  reference()                  Matrix.pyx:2010-2057
+
+Operators act on internal device blocks (k, n) (see _device).  A subclass
+implements ``_fwd(X)`` / ``_bwd(X)`` on blocks whose dtype has already been
+promoted by ``_apply`` following the reference's rules:
+  fInternal = promote(input, minType)  ; fOutput = promote(fInternal, dtype)
+  fInternal = fOutput if widenInputDatatype.
+"""
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+
+
+class FastmatFlags:
+    """Global flags (Matrix.pyx:43-48).  The dense-bypass strategy needs
+    calibration data and is off in the reference by default
+    (Matrix.pyx:1406-1409); it is not used here."""
+
+    def __init__(self):
+        self.bypassAllow = False
+        self.bypassAutoArray = False
+
+
+flags = FastmatFlags()
+
+
+class Matrix:
+    """Dense matrix; base of all structured operators.
+
+    ``Matrix(arr)`` stores a 2-D array on the device and applies it with the
+    library's FP32-MFMA GEMM for float32 matrices (float32 / complex64 data),
+    with the real 2m x 2n form for complex64 matrices, and with the generic
+    typed GEMM kernel otherwise.
+    """
+
+    __array_priority__ = 20.
+
+    def __init__(self, arrMatrix=None, **options):
+        if arrMatrix is None:
+            raise ValueError("Matrix: data array required")
+        arr = np.array(arrMatrix)
+        if arr.ndim != 2:
+            raise NotImplementedError("Matrix data array must be 2D")
+        if arr.dtype not in D.SUPPORTED:
+            raise TypeError("Unsupported data type %s" % arr.dtype)
+        self._array = arr
+        self._content = ()
+        self._dev = {}
+        self._initProperties(arr.shape[0], arr.shape[1], arr.dtype, **options)
+
+    # ------------------------------------------------------------ set-up
+    def _initProperties(self, numRows, numCols, dataType, **options):
+        """Matrix.pyx:1572-1610."""
+        self.numRows = int(numRows)
+        self.numCols = int(numCols)
+        self.dtype = D.np_dtype(dataType)
+        self._forceContiguousInput = options.get("forceContiguousInput", False)
+        self._widenInputDatatype = options.get("widenInputDatatype", False)
+        self._fortranStyle = options.get("fortranStyle", True)
+        self._minType = D.np_dtype(options.get("minType", np.int8))
+        self.bypassAllow = options.get("bypassAllow", flags.bypassAutoArray)
+        self.bypassAutoArray = options.get("bypassAutoArray", False)
+        self._H = self._T = self._conj = None
+        self._arrayCache = None
+        if not hasattr(self, "_content"):
+            self._content = ()
+
+    def _getProperties(self):
+        return {
+            "forceContiguousInput": self._forceContiguousInput,
+            "widenInputDatatype": self._widenInputDatatype,
+            "fortranStyle": self._fortranStyle,
+            "minType": self._minType,
+            "bypassAllow": self.bypassAllow,
+            "bypassAutoArray": self.bypassAutoArray,
+        }
+
+    # ------------------------------------------------------------ basics
+    @property
+    def shape(self):
+        return (self.numRows, self.numCols)
+
+    @property
+    def content(self):
+        return self._content
+
+    def __len__(self):
+        return len(self._content)
+
+    def __iter__(self):
+        return iter(self._content)
+
+    def __repr__(self):
+        return "<%s[%dx%d]:0x%12x>" % (self.__class__.__name__, self.numRows,
+                                       self.numCols, id(self))
+
+    def __copy__(self):
+        return self  # matrices are immutable (Matrix.pyx:341-358)
+
+    def __deepcopy__(self, memo):
+        return self
+
+    # --------------------------------------------------------- transforms
+    def forward(self, arrX):
+        """A @ x for 1-D (n,) or 2-D (n, k) input (Matrix.forward)."""
+        X, b = D.to_block(arrX, self.numCols)
+        return b.finish(self._apply(X, False))
+
+    def backward(self, arrX):
+        """A^H @ x (Matrix.backward)."""
+        X, b = D.to_block(arrX, self.numRows)
+        return b.finish(self._apply(X, True))
+
+    def _out_dtype(self, in_dtype):
+        fin = D.promote(in_dtype, self._minType)
+        return D.promote(fin, self.dtype)
+
+    def _apply(self, X, adjoint):
+        """Type promotion (Matrix._prepareInputArray) then the transform."""
+        fin = D.promote(X.dtype, self._minType)
+        fout = D.promote(fin, self.dtype)
+        if self._widenInputDatatype:
+            fin = fout
+        if D.np_dtype(X.dtype) != fin:
+            X = X.to(D.torch_dtype(fin))
+        return self._bwd(X) if adjoint else self._fwd(X)
+
+    # dense implementation (subclasses override)
+    def _fwd(self, X):
+        return _dense_apply(self, X, False)
+
+    def _bwd(self, X):
+        return _dense_apply(self, X, True)
+
+    def _strided_ok(self, dtype):
+        """True when the mode-wise (Kron) kernel can apply this dense matrix
+        to data of `dtype` directly: float32 A on float32 / complex64 data."""
+        return (type(self) is Matrix and self._array.dtype == np.float32 and
+                D.np_dtype(dtype) in (np.dtype(np.float32), np.dtype(np.complex64)))
+
+    def _apply_strided(self, X, adjoint, outer, inner):
+        A_d = self._dev.get("f32")
+        if A_d is None:
+            A_d = self._dev["f32"] = D.as_tensor(self._array, np.float32)
+        Y = X.new_empty(X.shape)
+        n = self.numRows
+        _lib.call("fm_dense_apply_strided", D.fm_type(np.float32), D.fm_type(X.dtype),
+                  int(adjoint), n, _lib.ptr(A_d), n, _lib.ptr(X), _lib.ptr(Y), outer, inner,
+                  D.stream())
+        return Y
+
+    def __mul__(self, factor):
+        if isinstance(factor, (np.ndarray, torch.Tensor)):
+            return self.forward(factor)
+        from .product import Product
+        return Product(self, factor, typeExpansion=np.int8)
+
+    def __rmul__(self, factor):
+        from .product import Product
+        if np.isscalar(factor) or isinstance(factor, Matrix):
+            return Product(factor, self, typeExpansion=np.int8)
+        raise TypeError("Invalid product term for fastmat Matrix.")
+
+    def __add__(self, element):
+        from .sum import Sum
+        if isinstance(element, Matrix):
+            return Sum(self, element)
+        raise TypeError("Not an addition of fastmat matrices.")
+
+    def __radd__(self, element):
+        from .sum import Sum
+        if isinstance(element, Matrix):
+            return Sum(self, element)
+        raise TypeError("Not an addition of fastmat matrices.")
+
+    def __sub__(self, element):
+        from .product import Product
+        from .sum import Sum
+        if isinstance(element, Matrix):
+            return Sum(self, Product(element, np.int8(-1), typeExpansion=np.int8))
+        raise TypeError("Not a subtraction of fastmat matrices.")
+
+    def __rsub__(self, element):
+        from .product import Product
+        from .sum import Sum
+        if isinstance(element, Matrix):
+            return Sum(element, Product(self, np.int8(-1), typeExpansion=np.int8))
+        raise TypeError("Not a subtraction of fastmat matrices.")
+
+    def __truediv__(self, divisor):
+        from .product import Product
+        if np.isscalar(divisor):
+            if divisor != 0:
+                return Product(self, 1. / divisor)
+            raise ZeroDivisionError("Division by Zero.")
+        raise TypeError("Only scalars allowed as divisors.")
+
+    def __floordiv__(self, divisor):
+        raise NotImplementedError("Matrix Floor division not allowed.")
+
+    # ------------------------------------------------------ transposition
+    @property
+    def H(self):
+        if self._H is None:
+            self._H = self._getH()
+        return self._H
+
+    @property
+    def T(self):
+        if self._T is None:
+            self._T = self._getT()
+        return self._T
+
+    @property
+    def conj(self):
+        if self._conj is None:
+            self._conj = self._getConj()
+        return self._conj
+
+    def getH(self):
+        return self.H
+
+    def getT(self):
+        return self.T
+
+    def getConj(self):
+        return self.conj
+
+    def _getH(self):
+        return Hermitian(self)
+
+    def _getT(self):
+        return Transpose(self)
+
+    def _getConj(self):
+        return getConjugate(self)
+
+    # --------------------------------------------------------- dense views
+    @property
+    def array(self):
+        return self.getArray()
+
+    def getArray(self):
+        if self._arrayCache is None:
+            self._arrayCache = self._getArray()
+        return self._arrayCache
+
+    def _getArray(self):
+        if getattr(self, "_array", None) is not None:
+            return self._array
+        return self.forward(np.eye(self.numCols, dtype=self.dtype))
+
+    def getCol(self, idx):
+        e = np.zeros(self.numCols, dtype=self.dtype)
+        e[idx] = 1
+        return self.forward(e)
+
+    def getRow(self, idx):
+        e = np.zeros(self.numRows, dtype=self.dtype)
+        e[idx] = 1
+        return np.conj(self.backward(e))
+
+    def __getitem__(self, key):
+        return self.getArray()[key]
+
+    # ----------------------------------------------------------- reference
+    def reference(self):
+        """Dense matrix built without the fast algorithms (Matrix.pyx:2010-2057)."""
+        return self._reference()
+
+    def _reference(self):
+        return self._array
+
+
+def _realified(A):
+    """Real 2m x 2n form of a complex m x n matrix, interleaved to act on
+    (re, im) storage: [[Re, -Im], [Im, Re]] per element."""
+    m, n = A.shape
+    R = np.empty((2 * m, 2 * n), dtype=np.float32)
+    R[0::2, 0::2] = A.real
+    R[0::2, 1::2] = -A.imag
+    R[1::2, 0::2] = A.imag
+    R[1::2, 1::2] = A.real
+    return R
+
+
+def _dense_apply(M, X, adjoint):
+    """Y = A X or A^H X for a dense Matrix on the device."""
+    A = M._array
+    m, n = A.shape
+    fout = D.promote(X.dtype, M.dtype) if not M._widenInputDatatype else D.np_dtype(X.dtype)
+    k = X.shape[0]
+    rows = n if adjoint else m
+    adt = A.dtype
+    xdt = D.np_dtype(X.dtype)
+    if adt == np.float32 and xdt in (np.float32, np.complex64):
+        A_d = M._dev.get("f32")
+        if A_d is None:
+            A_d = M._dev["f32"] = D.as_tensor(A, np.float32)
+        Y = D.empty(k, rows, xdt)
+        _lib.call("fm_dense_apply", D.fm_type(np.float32), D.fm_type(xdt), int(adjoint), m, n,
+                  _lib.ptr(A_d), n, _lib.ptr(X), X.shape[1], _lib.ptr(Y), rows, k, D.stream())
+        return Y
+    if adt == np.complex64 and xdt == np.complex64:
+        key = "re_h" if adjoint else "re"
+        A_d = M._dev.get(key)
+        if A_d is None:
+            A_d = M._dev[key] = D.as_tensor(_realified(A.conj().T if adjoint else A), np.float32)
+        Y = D.empty(k, rows, np.complex64)
+        mm, nn = (n, m) if adjoint else (m, n)
+        _lib.call("fm_dense_apply", D.fm_type(np.float32), D.fm_type(np.float32), 0, 2 * mm,
+                  2 * nn, _lib.ptr(A_d), 2 * nn, _lib.ptr(X), 2 * X.shape[1], _lib.ptr(Y),
+                  2 * rows, k, D.stream())
+        return Y
+    return _generic_dense(M, X, adjoint, fout)
+
+
+def _generic_dense(M, X, adjoint, fout):
+    """Typed GEMM for the remaining dtype combinations (plain library GEMM
+    of the dense matrix on the device, like ndarray.dot)."""
+    key = ("t", fout)
+    A_d = M._dev.get(key)
+    if A_d is None:
+        A_d = M._dev[key] = D.as_tensor(M._array, fout)
+    Xc = X.to(D.torch_dtype(fout))
+    if np.issubdtype(fout, np.integer):
+        # exact integer products: float64 is exact while |values| < 2^53
+        Af = A_d.to(torch.float64)
+        Y = (Af.conj().t() if adjoint else Af).matmul(Xc.to(torch.float64).t()).t()
+        return Y.round().to(D.torch_dtype(fout)).contiguous()
+    A_use = A_d.conj().t() if adjoint else A_d
+    return A_use.matmul(Xc.t()).t().contiguous()
+
+
+# ------------------------------------------------------------------ wrappers
+class Hermitian(Matrix):
+    """A^H (Matrix.pyx:2224-2300): forward = nested backward."""
+
+    def __init__(self, matrix):
+        if not isinstance(matrix, Matrix):
+            raise TypeError("Hermitian: Not a fastmat Matrix")
+        self._nested = matrix
+        self._content = (matrix,)
+        self._initProperties(matrix.numCols, matrix.numRows, matrix.dtype,
+                             **matrix._getProperties())
+
+    def __repr__(self):
+        return "<%s.H>" % (self._nested.__repr__(),)
+
+    def _apply(self, X, adjoint):
+        return self._nested._apply(X, not adjoint)
+
+    def _getH(self):
+        return self._nested
+
+    def _getT(self):
+        return getConjugate(self._nested)
+
+    def _getConj(self):
+        return Transpose(self._nested)
+
+    def _getArray(self):
+        return self._nested.getArray().T.conj()
+
+    def _reference(self):
+        return self._nested.reference().T.conj()
+
+
+def getConjugate(matrix):
+    """Conjugate(matrix) for complex matrices, else the matrix itself."""
+    return Conjugate(matrix) if np.iscomplexobj(np.empty(0, matrix.dtype)) else matrix
+
+
+class Conjugate(Matrix):
+    """conj(A) (Matrix.pyx:2326-2412): conj(A conj(x))."""
+
+    def __init__(self, matrix):
+        if not isinstance(matrix, Matrix):
+            raise TypeError("Conjugate: Not a fastmat Matrix")
+        self._nested = matrix
+        self._content = (matrix,)
+        self._initProperties(matrix.numRows, matrix.numCols, matrix.dtype,
+                             **matrix._getProperties())
+
+    def __repr__(self):
+        return "<conj(%s)>" % (self._nested.__repr__(),)
+
+    def _apply(self, X, adjoint):
+        Xc = _conj_block(X)
+        return _conj_block(self._nested._apply(Xc, adjoint))
+
+    def _getH(self):
+        return Transpose(self._nested)
+
+    def _getT(self):
+        return Hermitian(self._nested)
+
+    def _getConj(self):
+        return self._nested
+
+    def _getArray(self):
+        return self._nested.getArray().conj()
+
+    def _reference(self):
+        return self._nested.reference().conj()
+
+
+class Transpose(Hermitian):
+    """A^T = Hermitian(Conjugate(A)) (Matrix.pyx:2417-2482)."""
+
+    def __init__(self, matrix):
+        if not isinstance(matrix, Matrix):
+            raise TypeError("Transpose: Not a fastmat Matrix")
+        self._nestedConj = matrix
+        nested = getConjugate(matrix)
+        self._nested = nested
+        self._content = (nested,)
+        self._initProperties(nested.numCols, nested.numRows, nested.dtype,
+                             **nested._getProperties())
+
+    def __repr__(self):
+        return "<%s.T>" % (self._nestedConj.__repr__(),)
+
+    def _getT(self):
+        return self._nestedConj
+
+    def _getH(self):
+        return getConjugate(self._nestedConj)
+
+    def _getConj(self):
+        return Hermitian(self._nestedConj)
+
+    def _getArray(self):
+        return self._nestedConj.getArray().T
+
+    def _reference(self):
+        return self._nestedConj.reference().T
+
+
+def _conj_block(X):
+    """Conjugated copy of a complex block through the library (axpby with
+    conj); real blocks are returned as they are."""
+    if not X.is_complex():
+        return X
+    Y = torch.empty_like(X)
+    k, n = X.shape
+    _lib.call("fm_axpby", D.fm_type(X.dtype), n, k, _lib.dvec(1.0), _lib.ptr(X), n, 1,
+              _lib.dvec(0.0), _lib.ptr(Y), n, D.stream())
+    return Y
+
+
+def scale_block(X, s, conj=False, out=None):
+    """out = s * op(X) through the library."""
+    Y = torch.empty_like(X) if out is None else out
+    k, n = X.shape
+    _lib.call("fm_axpby", D.fm_type(X.dtype), n, k, _lib.dvec(s), _lib.ptr(X), n, int(conj),
+              _lib.dvec(0.0), _lib.ptr(Y), n, D.stream())
+    return Y
+
+
+def accumulate_block(Y, X):
+    """Y += X through the library."""
+    k, n = X.shape
+    _lib.call("fm_axpby", D.fm_type(X.dtype), n, k, _lib.dvec(1.0), _lib.ptr(X), n, 0,
+              _lib.dvec(1.0), _lib.ptr(Y), n, D.stream())
+    return Y

@@ -1,0 +1,203 @@
+/*
+ * fastmat_amd — C ABI of the MI355X (gfx950) structured-matvec library.
+ *
+ * Drop-in boundary for fastmat's C-style transform hook
+ * `_forwardC(arrX, arrRes, typeX, typeRes)` / `_backwardC(...)`
+ * (fastmat/Matrix.pyx:1819-1829, 1912-1922): every apply entry point takes a
+ * read-only input block X and writes a caller-allocated output block Y, both
+ * column-major with k right-hand sides (column j of X starts at x + j*ldx
+ * elements), exactly the F-contiguous (n, k) arrays fastmat passes.  All
+ * pointers are device pointers; `stream` is a hipStream_t (NULL = default
+ * stream).  Calls are stream-ordered and never synchronise with the host.
+ *
+ * Plans (Fourier, Circulant/Toeplitz) are immutable after creation and may be
+ * shared between threads, like fastmat matrices (Matrix.pyx:341-358).
+ * Multi-pass entry points take a caller-owned workspace whose size is given by
+ * the matching *_workspace_size query; apply calls allocate nothing.
+ *
+ * Element types use the reference's type ids (fastmat/core/types.pxd:45-55).
+ * Errors are integer status codes; fm_get_last_error() returns the message of
+ * the calling thread's last failure.  The Python layer maps FM_ERR_VALUE to
+ * ValueError, FM_ERR_TYPE to TypeError, FM_ERR_NOT_IMPLEMENTED to
+ * NotImplementedError, and the others to RuntimeError / MemoryError.
+ */
+#ifndef FASTMAT_AMD_H
+#define FASTMAT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FM_ABI_VERSION 1
+
+typedef int32_t fm_status;
+enum {
+    FM_SUCCESS = 0,
+    FM_ERR_VALUE = 1,           /* bad size/shape/argument   -> ValueError */
+    FM_ERR_TYPE = 2,            /* unsupported element type  -> TypeError */
+    FM_ERR_HIP = 3,             /* HIP runtime failure       -> RuntimeError */
+    FM_ERR_ALLOC = 4,           /* device allocation failed  -> MemoryError */
+    FM_ERR_NOT_IMPLEMENTED = 5  /* valid but unsupported     -> NotImplementedError */
+};
+
+/* Element type ids: same order as the reference's TYPES enum
+ * (core/types.pxd:45-55). */
+enum {
+    FM_INT8 = 0,
+    FM_INT16 = 1,
+    FM_INT32 = 2,
+    FM_INT64 = 3,
+    FM_FLOAT32 = 4,
+    FM_FLOAT64 = 5,
+    FM_COMPLEX64 = 6,
+    FM_COMPLEX128 = 7
+};
+
+/* ------------------------------------------------------------ library */
+int fm_abi_version(void);
+const char *fm_get_last_error(void);
+
+/* ----------------------------------------------------------- Hadamard
+ * Y = H_n X, H_n the unnormalised Sylvester Hadamard matrix of size 2^order.
+ * Replaces Hadamard._forwardC / _backwardC (Hadamard.pyx:164-239; the
+ * transform is symmetric, so backward == forward).  Any of the 8 element
+ * types; integer types wrap modulo 2^bits like the reference's C loop
+ * (bit-exact); float types apply the radix-2 stages in the reference's order
+ * (distance 1, 2, 4, ...), so float results equal the reference's bit for bit.
+ * In-place (x == y, ldx == ldy) is allowed. */
+fm_status fm_hadamard_apply(int dtype, int order, const void *x, int64_t ldx,
+                            void *y, int64_t ldy, int64_t k, void *stream);
+
+/* Mode-wise Hadamard along a strided axis: for every (o, j) in
+ * [0, outer) x [0, inner) the vector v[i] = x[(o*2^order + i)*inner + j] is
+ * transformed into y at the same positions (Kron mode application,
+ * Kron.pyx:267-303). */
+fm_status fm_hadamard_apply_strided(int dtype, int order, const void *x,
+                                    void *y, int64_t outer, int64_t inner,
+                                    void *stream);
+
+/* ------------------------------------------------------------- Fourier
+ * Unnormalised DFT of order N, F[j,l] = exp(-2 pi i j l / N); adjoint = F^H.
+ * Replaces Fourier.__init__ / _forward / _backward (Fourier.pyx:63-238).
+ * precision: FM_COMPLEX64 (FP32 butterflies, FP64-generated twiddles and
+ * chirps) or FM_COMPLEX128.  Power-of-two N uses radix kernels (one LDS pass
+ * up to the tile size, else a two-pass four-step); any other N uses Bluestein
+ * with a power-of-two length L >= 2N-1.  X and Y are complex of `precision`. */
+typedef struct fm_fourier_plan fm_fourier_plan;
+fm_status fm_fourier_plan_create(int64_t order, int precision, void *stream,
+                                 fm_fourier_plan **plan);
+fm_status fm_fourier_plan_info(const fm_fourier_plan *plan, int64_t *fft_length,
+                               int *bluestein, int *passes);
+fm_status fm_fourier_workspace_size(const fm_fourier_plan *plan, int64_t k,
+                                    int64_t *bytes);
+fm_status fm_fourier_apply(const fm_fourier_plan *plan, int adjoint,
+                           const void *x, int64_t ldx, void *y, int64_t ldy,
+                           int64_t k, void *workspace, int64_t workspace_bytes,
+                           void *stream);
+/* Mode-wise DFT along a strided axis (Kron mode application,
+ * Kron.pyx:267-303): vectors v[i] = x[(o*N + i)*inner + j]; requires a
+ * single-pass plan (N up to the LDS tile size). */
+fm_status fm_fourier_apply_strided(const fm_fourier_plan *plan, int adjoint,
+                                   const void *x, void *y, int64_t outer,
+                                   int64_t inner, void *stream);
+fm_status fm_fourier_plan_destroy(fm_fourier_plan *plan);
+
+/* ------------------------------------------------- Circulant / Toeplitz
+ * Y = A X with A the top-left rows x cols block of an L x L circulant whose
+ * generator t' embeds the structure; the spectrum fft(t')/L is computed at
+ * plan creation in FP64 on the device and stored in `precision`.
+ *   Circulant(c), c of length n: A[i,j] = c[(i-j) mod n]
+ *     (Circulant.pyx:104-137, 414-425); L = n for power-of-two n, otherwise
+ *     the smallest power of two >= 2n-1 with t' = [c, 0..., c[1:]].
+ *   Toeplitz(vecC, vecR), m x n: tenT = [vecC (m), vecR (n-1)],
+ *     A[i,j] = tenT[(i-j) mod (m+n-1)] (Toeplitz.pyx:146-176, 719-732);
+ *     L = smallest power of two >= m+n-1, zeros inserted at the split point
+ *     (Toeplitz.pyx:357-366).
+ * `generator` is a device pointer to complex128 values (c, or tenT).
+ * Replaces Partial(Product(F.H, Diag(fft(t')/L), F)) forward/backward
+ * (Partial.pyx:268-294, Product.pyx:205-240, Diag.pyx:149-167).
+ * Adjoint applies A^H. */
+typedef struct fm_conv_plan fm_conv_plan;
+fm_status fm_circulant_plan_create(int precision, int64_t n,
+                                   const void *generator, void *stream,
+                                   fm_conv_plan **plan);
+fm_status fm_toeplitz_plan_create(int precision, int64_t m, int64_t n,
+                                  const void *generator, void *stream,
+                                  fm_conv_plan **plan);
+fm_status fm_conv_plan_info(const fm_conv_plan *plan, int64_t *rows,
+                            int64_t *cols, int64_t *fft_length, int *passes);
+fm_status fm_conv_workspace_size(const fm_conv_plan *plan, int64_t k,
+                                 int64_t *bytes);
+fm_status fm_conv_apply(const fm_conv_plan *plan, int adjoint, const void *x,
+                        int64_t ldx, void *y, int64_t ldy, int64_t k,
+                        void *workspace, int64_t workspace_bytes, void *stream);
+/* Mode-wise application along a strided axis (square plans, single pass). */
+fm_status fm_conv_apply_strided(const fm_conv_plan *plan, int adjoint,
+                                const void *x, void *y, int64_t outer,
+                                int64_t inner, void *stream);
+/* Gradient of a real loss with respect to the generator, given the forward
+ * input X (cols x k) and the upstream gradient G = dLoss/dY (rows x k), in
+ * PyTorch's convention for complex tensors (grad = dL/dRe + i dL/dIm):
+ *   grad_t'[d] = sum_cols sum_i G[i] conj(X'[(i-d) mod L])   (a correlation)
+ * folded back onto the generator (c for Circulant: t'[0], t'[j] and
+ * t'[L-n+j] -> c[j]; [vecC, vecR] for Toeplitz).  `grad` receives complex
+ * values of `precision` (n entries for Circulant, m+n-1 for Toeplitz). */
+fm_status fm_conv_generator_grad_workspace_size(const fm_conv_plan *plan,
+                                                int64_t k, int64_t *bytes);
+fm_status fm_conv_generator_grad(const fm_conv_plan *plan, const void *x,
+                                 int64_t ldx, const void *g, int64_t ldg,
+                                 int64_t k, void *grad, void *workspace,
+                                 int64_t workspace_bytes, void *stream);
+fm_status fm_conv_plan_destroy(fm_conv_plan *plan);
+
+/* --------------------------------------------------------------- dense
+ * Y = A X (adjoint: Y = A^H X) for a dense m x n matrix A stored row-major
+ * (A[i*lda + j]); replaces Matrix._forward / _backward (Matrix.pyx:1831-1842,
+ * 1924-1935, ndarray.dot).  Supported (a_dtype, x_dtype) pairs:
+ * (FLOAT32, FLOAT32), (FLOAT32, COMPLEX64), (COMPLEX64, COMPLEX64); Y has the
+ * promoted type.  Real-A paths run on FP32 MFMA with FP32 accumulation. */
+fm_status fm_dense_apply(int a_dtype, int x_dtype, int adjoint, int64_t m,
+                         int64_t n, const void *A, int64_t lda, const void *x,
+                         int64_t ldx, void *y, int64_t ldy, int64_t k,
+                         void *stream);
+/* Mode-wise dense factor along a strided axis (Kron, Kron.pyx:267-303):
+ * square n x n A applied to v[i] = x[(o*n + i)*inner + j]. */
+fm_status fm_dense_apply_strided(int a_dtype, int x_dtype, int adjoint,
+                                 int64_t n, const void *A, int64_t lda,
+                                 const void *x, void *y, int64_t outer,
+                                 int64_t inner, void *stream);
+
+/* ---------------------------------------------------------- elementwise
+ * All element types; operands share `dtype`. */
+/* y[:,j] = d (.) x[:,j]  (conj_d: conj(d)); Diag._forwardC/_backwardC
+ * (Diag.pyx:149-167).  Mode-wise when inner > 1 (Kron). */
+fm_status fm_diag_apply(int dtype, int conj_d, int64_t n, const void *d,
+                        const void *x, int64_t ldx, void *y, int64_t ldy,
+                        int64_t k, void *stream);
+/* y[i,:] = x[idx[i],:] for i < n_out (row gather; Permutation._forward /
+ * _backward, Permutation.pyx:121-125, and Partial's row selection,
+ * Partial.pyx:268-294).  idx: device int64. */
+fm_status fm_gather_rows(int dtype, int64_t n_out, const int64_t *idx,
+                         const void *x, int64_t ldx, void *y, int64_t ldy,
+                         int64_t k, void *stream);
+/* y = 0 (n_out rows); y[idx[i],:] = x[i,:] for i < n_in (Partial's
+ * zero-padded scatter, Partial.pyx:270-276). */
+fm_status fm_scatter_rows(int dtype, int64_t n_in, const int64_t *idx,
+                          const void *x, int64_t ldx, void *y, int64_t ldy,
+                          int64_t n_out, int64_t k, void *stream);
+/* y = alpha * op(x) + beta * y over an n x k block, op = conj if conj_x
+ * (Sum accumulation Sum.pyx:151-175, Product's scalar Product.pyx:213-238).
+ * alpha/beta are given as two doubles each (re, im); for real and integer
+ * types only the real parts are used. */
+fm_status fm_axpby(int dtype, int64_t n, int64_t k, const double *alpha,
+                   const void *x, int64_t ldx, int conj_x, const double *beta,
+                   void *y, int64_t ldy, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTMAT_AMD_H */
