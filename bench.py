@@ -1,0 +1,338 @@
+#!/usr/bin/env python
+"""Benchmark of the structured-matvec hot path on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on):
+Fourier in complex64 at N = 2^20 (radix path) and N = 1,000,003 (prime,
+Bluestein path), each with X of 64 columns iid complex N(0, 1).  One step is
+the forward and the adjoint transform of both operators on resident device
+data.  ``value`` is whole-job throughput in GB/s over algorithmic bytes
+(each transform reads X once and writes Y once: 2 * N * k * 8 bytes).
+
+Other workloads (``--config``): hadamard (order 20, 64 columns, int32 and
+float32), circulant (N=4096, 32 columns, complex128), kron (Fourier(64) x
+Circulant(128) x dense 128x128, 16 columns, complex64).
+
+The JSON line also carries
+  roofline      dominant kernel: algorithmic bytes per launch / average launch
+                duration, both from HIP events recorded by the library around
+                its own launches during the timed region;
+  cpu_baseline  the restated CPU path of the reference (oracle.cpu_path, numpy
+                pocketfft, single core) timed on a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--config", default="fourier",
+                   choices=["fourier", "hadamard", "circulant", "kron"])
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cols", type=int, default=None, help="override the column count")
+    return p.parse_args()
+
+
+def crandn(rng, shape, dtype=np.complex64):
+    return (rng.standard_normal(shape, dtype=np.float32) +
+            1j * rng.standard_normal(shape, dtype=np.float32)).astype(dtype)
+
+
+def device_block(a, dtype):
+    """(n, k) host array -> internal (k, n) device block."""
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda").to(dtype)
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if recorded for this
+    kernel in profiles/*traffic*.txt as `<kernel label> | bytes_per_launch`."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if "traffic" not in f or not f.endswith((".txt", ".md")):
+            continue
+        for line in open(os.path.join(d, f)):
+            parts = [p.strip() for p in line.split("|")]
+            if len(parts) >= 2 and parts[0] == kernel:
+                try:
+                    return float(parts[1])
+                except ValueError:
+                    pass
+    return None
+
+
+# --------------------------------------------------------------- workloads
+def workload_fourier(args, rng):
+    import torch
+    import fastmat_amd as fm
+    k = args.cols or 64
+    sizes = [1 << 20, 1000003]
+    ops, xs = [], []
+    for n in sizes:
+        ops.append(fm.Fourier(n))
+        xs.append(device_block(crandn(rng, (n, k)), torch.complex64))
+    bytes_step = sum(2 * (2 * n * k * 8) for n in sizes)
+
+    def step():
+        outs = []
+        for F, X in zip(ops, xs):
+            outs.append(F._apply(X, False))
+            outs.append(F._apply(X, True))
+        return outs
+
+    def check():
+        from oracle import accept, fast64
+        errs = {}
+        for n, F, X in zip(sizes, ops, xs):
+            x = X[:2].cpu().numpy().T
+            y = F._apply(X[:2].contiguous(), False).cpu().numpy().T
+            errs[str(n)] = accept.max_rel_error(y, fast64.fourier(x))
+        return errs
+
+    def plain_gpu():
+        # the plain per-operator GPU path a user would write: torch.fft per op
+        def s():
+            for n, X in zip(sizes, xs):
+                torch.fft.fft(X, dim=1)
+                torch.fft.ifft(X, dim=1, norm="forward")
+        return s
+
+    def cpu():
+        from oracle import cpu_path
+        rs = np.random.default_rng(7)
+        legs = []
+        for n, kc in ((1 << 20, 2), (1000003, 1)):
+            F = cpu_path.FourierCPU(n)
+            x = crandn(rs, (n, kc))
+            legs.append((F, x, 2 * (2 * n * kc * 8)))
+        return legs, "Fourier 2^20 x 2 cols (radix) + 1,000,003 x 1 col (Bluestein), fwd+adj"
+
+    cfg = {"workload": "fourier_c64_fwd_adj", "sizes": sizes, "cols": k, "dtype": "complex64",
+           "paths": ["radix four-step 1024x1024", "bluestein L=2^21"]}
+    return dict(step=step, bytes=bytes_step, check=check, plain=plain_gpu(), cpu=cpu, cfg=cfg,
+                dtype="c64", metric="batched fwd+adjoint GB/s (% HBM peak)")
+
+
+def workload_hadamard(args, rng):
+    import torch
+    import fastmat_amd as fm
+    k = args.cols or 64
+    n = 1 << 20
+    H = fm.Hadamard(20)
+    xi = device_block(rng.integers(-8, 9, size=(n, k)).astype(np.int32), torch.int32)
+    xf = device_block(rng.standard_normal((n, k), dtype=np.float32), torch.float32)
+    bytes_step = 2 * 2 * (2 * n * k * 4)
+
+    def step():
+        return [H._apply(xi, False), H._apply(xi, True), H._apply(xf, False), H._apply(xf, True)]
+
+    def check():
+        from oracle import fwht_c
+        y = H._apply(xi[:2].contiguous(), False).cpu().numpy().T
+        ok = bool(np.array_equal(y, fwht_c.hadamard(xi[:2].cpu().numpy().T)))
+        yf = H._apply(xf[:2].contiguous(), False).cpu().numpy().T
+        okf = bool(np.array_equal(yf, fwht_c.hadamard(xf[:2].cpu().numpy().T)))
+        return {"int32_bitexact": ok, "float32_bitexact_vs_reference_loop": okf}
+
+    def cpu():
+        from oracle import fwht_c
+        rs = np.random.default_rng(7)
+        xc = rs.integers(-8, 9, size=(n, 2)).astype(np.int32)
+        xfc = rs.standard_normal((n, 2)).astype(np.float32)
+
+        class Leg:
+            def __init__(self, x):
+                self.x = x
+
+            def forward(self, x):
+                return fwht_c.hadamard(x)
+
+            backward = forward
+        return [(Leg(xc), xc, 2 * (2 * n * 2 * 4)), (Leg(xfc), xfc, 2 * (2 * n * 2 * 4))], \
+            "Hadamard order 20 x 2 cols int32 + 2 cols float32 (C restatement of the loop), fwd+adj"
+
+    cfg = {"workload": "hadamard_order20_int32_float32_fwd_adj", "order": 20, "cols": k}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=cpu, cfg=cfg,
+                dtype="int32+f32", metric="batched fwd+adjoint GB/s (% HBM peak)")
+
+
+def workload_circulant(args, rng):
+    import torch
+    import fastmat_amd as fm
+    k = args.cols or 32
+    n = 4096
+    c = crandn(rng, (n,), np.complex128)
+    C = fm.Circulant(c)
+    X = device_block(crandn(rng, (n, k), np.complex128), torch.complex128)
+    bytes_step = 2 * (2 * n * k * 16 + n * 16)
+
+    def step():
+        return [C._apply(X, False), C._apply(X, True)]
+
+    def check():
+        import scipy.linalg as sla
+        from oracle import accept
+        x = X.cpu().numpy().T
+        return {"fp64_dense": accept.max_rel_error(C._apply(X, False).cpu().numpy().T,
+                                                   sla.circulant(c) @ x)}
+
+    def cpu():
+        from oracle import cpu_path
+        op = cpu_path.circulant(c)
+        x = crandn(np.random.default_rng(7), (n, k), np.complex128)
+        return [(op, x, bytes_step)], "Circulant 4096 x 32 cols complex128, fwd+adj"
+
+    cfg = {"workload": "circulant_4096_c128_fwd_adj", "n": n, "cols": k}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=cpu, cfg=cfg,
+                dtype="c128", metric="batched fwd+adjoint GB/s (% HBM peak)")
+
+
+def workload_kron(args, rng):
+    import torch
+    import fastmat_amd as fm
+    k = args.cols or 16
+    A = (rng.standard_normal((128, 128)) / np.sqrt(128)).astype(np.float32)
+    c = rng.standard_normal(128).astype(np.complex64)
+    K = fm.Kron(fm.Fourier(64), fm.Circulant(c), fm.Matrix(A))
+    N = 1 << 20
+    X = device_block(crandn(rng, (N, k)), torch.complex64)
+    bytes_step = 2 * (2 * N * k * 8 + 128 * 128 * 4 + 128 * 8)
+
+    def step():
+        return [K._apply(X, False), K._apply(X, True)]
+
+    def check():
+        import scipy.linalg as sla
+        from oracle import accept, fast64
+        x = X[:1].cpu().numpy().T
+        ops = [lambda v: fast64.fourier(v), lambda v: sla.circulant(c.astype(np.complex128)) @ v,
+               lambda v: A.astype(np.float64) @ v]
+        ref = fast64.kron_apply(ops, (64, 128, 128), x)
+        return {"fwd": accept.max_rel_error(K._apply(X[:1].contiguous(), False).cpu().numpy().T,
+                                            ref)}
+
+    cfg = {"workload": "kron_F64_C128_M128_c64_fwd_adj", "N": N, "cols": k}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=None, cfg=cfg,
+                dtype="c64", metric="batched fwd+adjoint GB/s (% HBM peak)")
+
+
+WORKLOADS = {"fourier": workload_fourier, "hadamard": workload_hadamard,
+             "circulant": workload_circulant, "kron": workload_kron}
+
+
+def time_cpu(legs):
+    """Reference timing rule: repeat until >= 3 ms and >= 7 runs, take the
+    min (inspect/benchmark.py:208-255)."""
+    total_bytes, total_s = 0, 0.0
+    for op, x, nbytes in legs:
+        best = None
+        runs, t_all = 0, 0.0
+        while runs < 7 or t_all < 3e-3:
+            t0 = time.perf_counter()
+            op.forward(x)
+            op.backward(x)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+            runs += 1
+            t_all += dt
+        total_bytes += nbytes
+        total_s += best
+    return total_bytes / total_s / 1e9
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(columns / operators shard with no exchange; see the design notes).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fastmat_amd  # noqa: F401
+    from fastmat_amd import _lib
+    _lib.load()
+    rng = np.random.default_rng(20261202 + 1)
+    w = WORKLOADS[args.config](args, rng)
+
+    for _ in range(max(args.warmup, 1)):
+        w["step"]()
+    torch.cuda.synchronize()
+
+    with _lib.Profiler() as prof:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            w["step"]()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = w["bytes"] / (ms * 1e-3) / 1e9
+
+    # dominant kernel from the library's own events
+    name, rec = max(prof.results.items(), key=lambda kv: kv[1]["ms"])
+    avg_ms = rec["ms"] / rec["launches"]
+    bytes_per_launch = rec["bytes"] / rec["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(name), "kernel": name,
+                "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
+                "algorithmic_bytes_per_launch": bytes_per_launch}
+    kernels = {k: {"launches": v["launches"], "avg_us": round(v["ms"] / v["launches"] * 1e3, 2),
+                   "GBps": round(v["bytes"] / v["ms"] / 1e6, 1)}
+               for k, v in prof.results.items()}
+
+    extra = {}
+    try:
+        extra["max_rel_err"] = w["check"]()
+    except Exception as e:  # report, do not hide the measurement
+        extra["max_rel_err"] = "check failed: %s" % e
+    if w.get("plain") is not None:
+        s = w["plain"]
+        s()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            s()
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - t2) * 1e3 / args.steps
+        extra["plain_torch_fft_ms_per_step"] = round(pms, 3)
+        extra["speedup_vs_plain_torch_fft"] = round(pms / ms, 3)
+
+    cpu_baseline = None
+    if not args.no_cpu and w.get("cpu") is not None:
+        legs, sample = w["cpu"]()
+        cpu_baseline = {"value": round(time_cpu(legs), 4), "unit": "GB/s", "cores": 1,
+                        "kind": "port", "sample": sample,
+                        "host_cpus": os.cpu_count()}
+
+    out = {
+        "metric": w["metric"], "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": w["dtype"], "data": "synthetic (seeded iid N(0,1))",
+        "config": w["cfg"], "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
+        "roofline": roofline, "cpu_baseline": cpu_baseline, "kernels": kernels,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,13 @@ _PROTOTYPES = {
                                  _vp]),
     "fm_axpby": (_c_int, [_c_int, _c_i64, _c_i64, ctypes.POINTER(ctypes.c_double), _vp, _c_i64,
                           _c_int, ctypes.POINTER(ctypes.c_double), _vp, _c_i64, _vp]),
+    "fm_profiler_enable": (_c_int, [_c_int]),
+    "fm_profiler_reset": (_c_int, []),
+    "fm_profiler_count": (_c_int, [ctypes.POINTER(_c_int)]),
+    "fm_profiler_entry": (_c_int, [_c_int, ctypes.c_char_p, _c_int, ctypes.POINTER(_c_i64),
+                                   ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double)]),
 }
 
 EXPORTED = tuple(_PROTOTYPES)
@@ -138,3 +145,36 @@ def dvec(z):
     """Complex scalar as a ctypes double[2]."""
     z = complex(z)
     return (ctypes.c_double * 2)(z.real, z.imag)
+
+
+class Profiler:
+    """Context manager around the library's per-kernel event timing.
+
+    >>> with Profiler() as p:
+    ...     op.forward(x)
+    >>> p.results  # {kernel label: dict(launches, ms, bytes, flops)}
+    """
+
+    def __init__(self):
+        self.results = {}
+
+    def __enter__(self):
+        lib = load()
+        check(lib.fm_profiler_reset())
+        check(lib.fm_profiler_enable(1))
+        return self
+
+    def __exit__(self, *exc):
+        lib = load()
+        check(lib.fm_profiler_enable(0))
+        n = ctypes.c_int()
+        check(lib.fm_profiler_count(ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(128)
+        for i in range(n.value):
+            la, ms, by, fl = _c_i64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            check(lib.fm_profiler_entry(i, buf, 128, ctypes.byref(la), ctypes.byref(ms),
+                                        ctypes.byref(by), ctypes.byref(fl)))
+            self.results[buf.value.decode()] = dict(launches=la.value, ms=ms.value,
+                                                    bytes=by.value, flops=fl.value)
+        check(lib.fm_profiler_reset())
+        return False

@@ -108,8 +108,16 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nblocks) {
     return start + slot;
 }
 
+// ------------------------------------------------------------- profiler
+struct ProfToken {
+    bool active;
+    hipEvent_t a, b;
+};
+bool prof_on();
+void prof_begin(hipStream_t s, ProfToken *t);
+void prof_end(hipStream_t s, ProfToken *t, const char *name, double bytes, double flops);
+
 // Per-device cached tables (twiddles), keyed by (device, log2 size, precision).
-struct TwiddleCache;
 const void *twiddle_table(int precision, int logn, hipStream_t stream, fm_status *st);
 
 }  // namespace fm

@@ -140,43 +140,68 @@ const void *twiddle_table(int precision, int logn, hipStream_t stream, fm_status
 }
 
 // ------------------------------------------------------------ tile launch
-template <typename T, int NT, int MAXE, bool CONV>
-static fm_status launch_cfg(const TileArgs<T> &a, hipStream_t stream) {
-    const int N = 1 << a.logN;
-    const size_t lds = size_t(a.B) * (N + 1) * sizeof(cpx<T>) + 3 * size_t(a.B) * 8;
+template <typename T, int LOGN, bool CONV>
+static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
+    constexpr int NT = (sizeof(T) == 4 ? LOGN >= 14 : LOGN >= 13) ? 512 : 256;
+    constexpr int N = 1 << LOGN;
+    const size_t lds = size_t(a.B) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)fft_tile_kernel<T, NT, MAXE, CONV>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
-    FM_REQUIRE(lds <= 160 * 1024, FM_ERR_VALUE, "tile of %d x %d needs %zu bytes of LDS",
-               a.B, N, lds);
+    FM_REQUIRE(lds <= 160 * 1024, FM_ERR_VALUE, "tile of %d x %d needs %zu bytes of LDS", a.B, N,
+               lds);
     FM_REQUIRE(a.ntiles > 0 && a.ntiles < (int64_t(1) << 31), FM_ERR_VALUE,
                "tile count %lld out of range", (long long)a.ntiles);
-    hipLaunchKernelGGL((fft_tile_kernel<T, NT, MAXE, CONV>), dim3(unsigned(a.ntiles)),
-                       dim3(NT), lds, stream, a);
+    ProfToken tok;
+    prof_begin(stream, &tok);
+    hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV>), dim3(unsigned(a.ntiles)), dim3(NT),
+                       lds, stream, a);
     FM_CHECK_LAUNCH();
+    if (tok.active) {
+        const double span = double(a.J) * N;
+        const double esz = double(sizeof(cpx<T>));
+        const double in = double(a.O) * (double(a.n_valid) < span ? double(a.n_valid) : span);
+        const double out = double(a.O) * (double(a.m_valid) < span ? double(a.m_valid) : span);
+        const double spec = CONV ? (a.d_gj ? span : double(N)) : 0.0;
+        char name[96];
+        snprintf(name, sizeof(name), "fft_pass_kernel<%s,%d,%d,%s>",
+                 sizeof(T) == 4 ? "float" : "double", LOGN, NT, CONV ? "conv" : "fft");
+        const double nseq = double(a.O) * double(a.J);
+        const double flops = nseq * 5.0 * N * LOGN * (CONV ? 2.0 : 1.0);
+        prof_end(stream, &tok, name, (in + out + spec) * esz, flops);
+    }
     return FM_SUCCESS;
 }
 
 template <typename T>
-fm_status launch_tile(const TileArgs<T> &a, bool conv, hipStream_t stream) {
-    const int64_t tot = int64_t(a.B) << a.logN;
-    FM_REQUIRE(a.B >= 1 && is_pow2(a.B), FM_ERR_VALUE, "tile batch must be a power of two");
-    FM_REQUIRE(tot <= max_tile_elems<T>(), FM_ERR_VALUE, "tile of %lld elements too large",
-               (long long)tot);
-    if (sizeof(T) == 4) {
-        if (tot <= 4096)
-            return conv ? launch_cfg<T, 256, 16, true>(a, stream) : launch_cfg<T, 256, 16, false>(a, stream);
-        if (tot <= 8192)
-            return conv ? launch_cfg<T, 256, 32, true>(a, stream) : launch_cfg<T, 256, 32, false>(a, stream);
-        return conv ? launch_cfg<T, 512, 32, true>(a, stream) : launch_cfg<T, 512, 32, false>(a, stream);
-    } else {
-        if (tot <= 4096)
-            return conv ? launch_cfg<T, 256, 16, true>(a, stream) : launch_cfg<T, 256, 16, false>(a, stream);
-        return conv ? launch_cfg<T, 512, 16, true>(a, stream) : launch_cfg<T, 512, 16, false>(a, stream);
+fm_status launch_tile(const TileArgs<T> &a_in, bool conv, hipStream_t stream) {
+    TileArgs<T> a = a_in;
+    const int maxlog = sizeof(T) == 4 ? 14 : 13;
+    FM_REQUIRE(a.logN >= 0 && a.logN <= maxlog, FM_ERR_VALUE,
+               "sub-transform length 2^%d exceeds the tile limit", a.logN);
+    a.B = tile_batch<T>(a.logN);
+    a.ntiles = ceil_div(a.O * a.J, a.B);
+    a.logJ = is_pow2(a.J) ? ilog2(a.J) : -1;
+#define FM_PASS_CASE(L)                                                                  \
+    case L:                                                                              \
+        return conv ? launch_pass<T, L, true>(a, stream) : launch_pass<T, L, false>(a, stream);
+    switch (a.logN) {
+        FM_PASS_CASE(0) FM_PASS_CASE(1) FM_PASS_CASE(2) FM_PASS_CASE(3) FM_PASS_CASE(4)
+        FM_PASS_CASE(5) FM_PASS_CASE(6) FM_PASS_CASE(7) FM_PASS_CASE(8) FM_PASS_CASE(9)
+        FM_PASS_CASE(10) FM_PASS_CASE(11) FM_PASS_CASE(12) FM_PASS_CASE(13)
+    case 14:
+        if constexpr (sizeof(T) == 4)
+            return conv ? launch_pass<T, 14, true>(a, stream) : launch_pass<T, 14, false>(a, stream);
+        break;
+    default:
+        break;
     }
+#undef FM_PASS_CASE
+    set_error("unsupported sub-transform length 2^%d", a.logN);
+    return FM_ERR_VALUE;
 }
 
 template fm_status launch_tile<float>(const TileArgs<float> &, bool, hipStream_t);
@@ -191,6 +216,15 @@ template <typename T> static TileArgs<T> blank_args() {
     a.n_valid = INT64_MAX;
     a.m_valid = INT64_MAX;
     return a;
+}
+
+// Tile-shape knob: environment override (used by the tuning sweeps), else
+// the built-in default.
+static int knob(const char *name, int dflt) {
+    const char *v = getenv(name);
+    if (!v || !*v) return dflt;
+    int x = atoi(v);
+    return x > 0 ? x : dflt;
 }
 
 static int pick_batch(int logN, int64_t nseq, int max_tile, int want) {
@@ -293,7 +327,7 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN1;
         a.O = io.k; a.J = N2;
-        a.B = pick_batch(g.logN1, N2, max_tile, 16);
+        a.B = pick_batch(g.logN1, N2, max_tile, knob("FASTMAT_AMD_TILE_A", 16));
         a.ntiles = ceil_div(io.k * N2, a.B);
         a.x = (const cpx<T> *)io.x;
         a.in_os = io.ldx; a.in_js = 1; a.in_ns = N2;
@@ -315,7 +349,7 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN2;
         a.O = io.k; a.J = N1;
-        a.B = pick_batch(g.logN2, N1, max_tile, 16);
+        a.B = pick_batch(g.logN2, N1, max_tile, knob("FASTMAT_AMD_TILE_B", 16));
         a.ntiles = ceil_div(io.k * N1, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = N2; a.in_ns = 1;
@@ -334,7 +368,7 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN2;
         a.O = io.k; a.J = N1;
-        a.B = pick_batch(g.logN2, N1, max_tile, 2);
+        a.B = pick_batch(g.logN2, N1, max_tile, knob("FASTMAT_AMD_TILE_CONV", 2));
         a.ntiles = ceil_div(io.k * N1, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = N2; a.in_ns = 1;
@@ -353,7 +387,7 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN1;
         a.O = io.k; a.J = N2;
-        a.B = pick_batch(g.logN1, N2, max_tile, 16);
+        a.B = pick_batch(g.logN1, N2, max_tile, knob("FASTMAT_AMD_TILE_A", 16));
         a.ntiles = ceil_div(io.k * N2, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = 1; a.in_ns = N2;

@@ -66,10 +66,59 @@ template <int SIGN, typename T> __device__ __forceinline__ void dft8(cpx<T> *v) 
     }
 }
 
-template <int R, int SIGN, typename T> __device__ __forceinline__ void dft(cpx<T> *v) {
-    if constexpr (R == 2) dft2<SIGN>(v);
-    else if constexpr (R == 4) dft4<SIGN>(v);
-    else dft8<SIGN>(v);
+// W_32^k = exp(-2 pi i k / 32), k < 16 (cos, sin)
+__device__ __forceinline__ constexpr double w32c(int k) {
+    constexpr double c[16] = {1.0, 0.98078528040323043, 0.92387953251128674, 0.83146961230254524,
+                              0.70710678118654757, 0.55557023301960229, 0.38268343236508984,
+                              0.19509032201612833, 0.0, -0.19509032201612819, -0.38268343236508973,
+                              -0.55557023301960196, -0.70710678118654746, -0.83146961230254535,
+                              -0.92387953251128674, -0.98078528040323043};
+    return c[k];
+}
+__device__ __forceinline__ constexpr double w32s(int k) {
+    constexpr double s[16] = {0.0, 0.19509032201612825, 0.38268343236508978, 0.55557023301960218,
+                              0.70710678118654746, 0.83146961230254524, 0.92387953251128674,
+                              0.98078528040323043, 1.0, 0.98078528040323043, 0.92387953251128674,
+                              0.83146961230254546, 0.70710678118654757, 0.55557023301960218,
+                              0.38268343236508989, 0.19509032201612861};
+    return s[k];
+}
+
+// x * W_R^k (forward, SIGN < 0) or x * conj(W_R^k) (inverse), k < R/2, R <= 32;
+// k is a compile-time constant after unrolling, so the branches fold away.
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ cpx<T> wmul(cpx<T> x, int k) {
+    if (k == 0) return x;
+    if (4 * k == R) return mul_i<SIGN>(x);
+    const int q = k * (32 / R);
+    const T c = T(w32c(q)), s = T(w32s(q));
+    // forward W = c - i s ; inverse conj(W) = c + i s
+    return SIGN < 0 ? mk(x.x * c + x.y * s, x.y * c - x.x * s)
+                    : mk(x.x * c - x.y * s, x.y * c + x.x * s);
+}
+
+// In-register DFT of length 2^LOGR (LOGR <= 5), natural order in and out.
+template <int LOGR, int SIGN, typename T> __device__ __forceinline__ void dftr(cpx<T> *v) {
+    if constexpr (LOGR == 1) dft2<SIGN>(v);
+    else if constexpr (LOGR == 2) dft4<SIGN>(v);
+    else if constexpr (LOGR == 3) dft8<SIGN>(v);
+    else if constexpr (LOGR >= 4) {
+        constexpr int R = 1 << LOGR, H = R / 2;
+        cpx<T> e[H], o[H];
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        dftr<LOGR - 1, SIGN>(e);
+        dftr<LOGR - 1, SIGN>(o);
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            const cpx<T> t = wmul<R, SIGN>(o[k], k);
+            v[k] = e[k] + t;
+            v[k + H] = e[k] - t;
+        }
+    }
 }
 
 template <int SIGN, typename T>
@@ -78,63 +127,22 @@ __device__ __forceinline__ cpx<T> twiddle_mul(cpx<T> v, cpx<T> w) {
     return SIGN < 0 ? cmul(v, w) : cmulc(v, w);
 }
 
-// One in-place Stockham stage over B sequences of length N held in LDS
-// (sequence b at lds + b*S).  `tw` holds W_N^e = exp(-2 pi i e / N), e < N.
-template <typename T, int R, int SIGN, int NT, int MAXE>
-__device__ __forceinline__ void fft_stage(cpx<T> *lds, int S, int logN, int B,
-                                          int logp, const cpx<T> *__restrict__ tw) {
-    constexpr int LOGR = R == 2 ? 1 : (R == 4 ? 2 : 3);
-    constexpr int MAXB = MAXE / R;
-    const int logNR = logN - LOGR;
-    const int NR = 1 << logNR;
-    const int total = B << logNR;
-    const int p = 1 << logp;
-    cpx<T> v[MAXB * R];
-#pragma unroll
-    for (int e = 0; e < MAXB; ++e) {
-        const int g = threadIdx.x + e * NT;
-        if (g < total) {
-            const int b = g >> logNR, i = g & (NR - 1);
-            const cpx<T> *src = lds + b * S + i;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[e * R + r] = src[r * NR];
-        }
+// Stage plan of a length-2^LOGN transform for threads holding 2^LOGE
+// elements: S stages with radices 2^bits(s) <= 2^LOGE, bits as even as
+// possible, larger radices first.
+template <int LOGN, int LOGE> struct FftPlan {
+    static constexpr int S = LOGN == 0 ? 0 : (LOGN + LOGE - 1) / LOGE;
+    static constexpr int bits(int s) { return LOGN / S + (s < LOGN % S ? 1 : 0); }
+    static constexpr int logp(int s) {
+        int t = 0;
+        for (int q = 0; q < s; ++q) t += bits(q);
+        return t;
     }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < MAXB; ++e) {
-        const int g = threadIdx.x + e * NT;
-        if (g < total) {
-            const int b = g >> logNR, i = g & (NR - 1);
-            const int k = i & (p - 1);
-            if (logp > 0) {
-                // W_{pR}^{r k} = W_N^{r k N / (p R)}
-                const int step = k << (logN - logp - LOGR);
-#pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[e * R + r] = twiddle_mul<SIGN>(v[e * R + r], tw[r * step]);
-            }
-            dft<R, SIGN>(v + e * R);
-            cpx<T> *dst = lds + b * S + ((i - k) << LOGR) + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * p] = v[e * R + r];
-        }
-    }
-    __syncthreads();
-}
+};
 
-template <typename T, int SIGN, int NT, int MAXE>
-__device__ void lds_fft(cpx<T> *lds, int S, int logN, int B, const cpx<T> *__restrict__ tw) {
-    int logp = 0;
-    int rem = logN;
-    while (rem >= 3) {
-        fft_stage<T, 8, SIGN, NT, MAXE>(lds, S, logN, B, logp, tw);
-        logp += 3;
-        rem -= 3;
-    }
-    if (rem == 2) fft_stage<T, 4, SIGN, NT, MAXE>(lds, S, logN, B, logp, tw);
-    else if (rem == 1) fft_stage<T, 2, SIGN, NT, MAXE>(lds, S, logN, B, logp, tw);
-}
+// Padded LDS index: one spare slot every 32 elements keeps the strided
+// exchanges of the stages free of most bank conflicts.
+__device__ __forceinline__ int lpad(int n) { return n + (n >> 5); }
 
 // ------------------------------------------------------------ tile kernel
 template <typename T> struct TileArgs {
@@ -168,94 +176,266 @@ template <typename T> struct TileArgs {
     int conj_out;
     T out_scale;
     int seqfast_out;
+    int logJ;                       // log2(J) when J is a power of two, else -1
 };
 
-template <typename T, int NT, int MAXE, bool CONV>
-__global__ __launch_bounds__(NT) void fft_tile_kernel(const TileArgs<T> a) {
+// Sequence (o, j) of tile-local sequence b.
+struct SeqPos {
+    int64_t in_base, out_base, j;
+    bool ok;
+};
+
+template <typename T>
+__device__ __forceinline__ SeqPos seq_pos(const TileArgs<T> &a, int64_t s) {
+    SeqPos p;
+    p.ok = s < a.O * a.J;
+    int64_t o, j;
+    if (a.logJ >= 0) {
+        o = s >> a.logJ;
+        j = s & ((int64_t(1) << a.logJ) - 1);
+    } else {
+        o = s / a.J;
+        j = s - o * a.J;
+    }
+    p.in_base = o * a.in_os + j * a.in_js;
+    p.out_base = o * a.out_os + j * a.out_js;
+    p.j = j;
+    return p;
+}
+
+// Map butterfly g of a stage to (sequence b, butterfly index i): sequence-
+// fastest for strided global sides (neighbouring sequences are adjacent in
+// memory), butterfly-fastest otherwise.
+template <int LOGB, int LOGNR>
+__device__ __forceinline__ void bfly_pos(int g, bool seqfast, int &b, int &i) {
+    if (seqfast) {
+        b = g & ((1 << LOGB) - 1);
+        i = g >> LOGB;
+    } else {
+        i = g & ((1 << LOGNR) - 1);
+        b = g >> LOGNR;
+    }
+}
+
+// One radix-2^LOGR stage with sub-length 2^LOGP on registers v (E per thread,
+// NB = E / R butterflies).  SRC: 0 = v already holds this stage's inputs,
+// 1 = read them from LDS.  DST: 0 = write outputs to LDS, 1 = leave them in v.
+template <typename T, int LOGN, int LOGR, int LOGP, int SIGN, int NT, int E, int LOGB, int SRC,
+          int DST>
+__device__ __forceinline__ void fft_stage2(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw,
+                                           bool sf) {
+    // sf: mapping of the butterflies held in v when SRC == 0 (see bfly_pos);
+    // inputs read from LDS always use the butterfly-fastest mapping.
+    if constexpr (SRC == 1) sf = false;
+    constexpr int R = 1 << LOGR, NB = E / R;
+    constexpr int LOGNR = LOGN - LOGR, NR = 1 << LOGNR;
+    constexpr int P = 1 << LOGP;
+    constexpr int SB = (1 << LOGN) + ((1 << LOGN) >> 5) + 1;
+    if constexpr (SRC == 1) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            int b, i;
+            bfly_pos<LOGB, LOGNR>(threadIdx.x + e * NT, false, b, i);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[e * R + r] = lds[b * SB + lpad(i + r * NR)];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        int b, i;
+        bfly_pos<LOGB, LOGNR>(threadIdx.x + e * NT, sf, b, i);
+        if constexpr (LOGP > 0) {
+            const int k = i & (P - 1);
+            // W_{pR}^{r k} = W_N^{r k N / (p R)}
+            const int step = k << (LOGN - LOGP - LOGR);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[e * R + r] = twiddle_mul<SIGN>(v[e * R + r], tw[r * step]);
+        }
+        dftr<LOGR, SIGN>(v + e * R);
+    }
+    if constexpr (DST == 0) {
+        __syncthreads();  // every thread has read its inputs of this stage
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            int b, i;
+            bfly_pos<LOGB, LOGNR>(threadIdx.x + e * NT, sf, b, i);
+            const int k = i & (P - 1);
+            const int base = ((i - k) << LOGR) + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[b * SB + lpad(base + r * P)] = v[e * R + r];
+        }
+        __syncthreads();
+    }
+}
+
+// All stages of one transform.  FIRST_SRC: 0 = v holds stage-0 inputs
+// (loaded from global), 1 = they are in LDS in natural order.  LAST_DST:
+// 0 = leave natural-order outputs in LDS, 1 = leave the last stage's outputs
+// in v (butterfly-fastest mapping).
+template <typename T, int LOGN, int LOGE, int SIGN, int NT, int LOGB, int FIRST_SRC, int LAST_DST,
+          int s = 0>
+__device__ __forceinline__ void fft_all(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw,
+                                        bool sf0) {
+    using P = FftPlan<LOGN, LOGE>;
+    constexpr int E = 1 << LOGE;
+    if constexpr (s < P::S) {
+        constexpr int src = s == 0 ? FIRST_SRC : 1;
+        constexpr int dst = (s == P::S - 1) ? (LAST_DST == 1 ? 1 : 0) : 0;
+        fft_stage2<T, LOGN, P::bits(s), P::logp(s), SIGN, NT, E, LOGB, src, dst>(v, lds, tw, sf0);
+        fft_all<T, LOGN, LOGE, SIGN, NT, LOGB, FIRST_SRC, LAST_DST, s + 1>(v, lds, tw, false);
+    }
+}
+
+template <typename T, int LOGN, int NT, bool CONV>
+__global__ __launch_bounds__(NT) void fft_pass_kernel(const TileArgs<T> a) {
+    constexpr int LOGE = sizeof(T) == 4 ? 5 : 4;
+    constexpr int E = 1 << LOGE;
+    constexpr int N = 1 << LOGN;
+    using P = FftPlan<LOGN, LOGE>;
+    constexpr int LOGR0 = P::S > 0 ? P::bits(0) : 0;
+    constexpr int LOGRL = P::S > 0 ? P::bits(P::S - 1) : 0;
+    constexpr int R0 = 1 << LOGR0, RL = 1 << LOGRL;
+    // sequences per tile: NT * E / N (a multiple of NT*E ... >= 1 by construction)
+    constexpr int LOGNT = NT == 64 ? 6 : NT == 128 ? 7 : NT == 256 ? 8 : NT == 512 ? 9 : 10;
+    constexpr int LOGB = LOGNT + LOGE - LOGN;
+    constexpr int B = 1 << LOGB;
+    constexpr int SB = N + (N >> 5) + 1;
+    static_assert(LOGB >= 0, "tile must hold at least one sequence");
+
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int B = a.B;
-    const int N = 1 << a.logN;
-    const int S = N + 1;
     cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-    int64_t *seq_in = reinterpret_cast<int64_t *>(lds + B * S);
-    int64_t *seq_out = seq_in + B;
-    int64_t *seq_j = seq_out + B;
 
     const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t s0 = tile * B;
-    const int64_t nseq = a.O * a.J;
-    for (int b = threadIdx.x; b < B; b += NT) {
-        const int64_t s = s0 + b;
-        if (s < nseq) {
-            const int64_t o = s / a.J, j = s - o * a.J;
-            seq_in[b] = o * a.in_os + j * a.in_js;
-            seq_out[b] = o * a.out_os + j * a.out_js;
-            seq_j[b] = j;
-        } else {
-            seq_j[b] = -1;
-        }
-    }
-    __syncthreads();
+    const int64_t s0 = tile << LOGB;
 
-    // ---- load (zero-pad, conj, pre-multiply)
-    const int tot = B * N;
-    int logB = 0;
-    while ((1 << logB) < B) ++logB;
-    for (int idx = threadIdx.x; idx < tot; idx += NT) {
-        int b, n;
-        if (a.seqfast_in) { b = idx & (B - 1); n = idx >> logB; }
-        else { n = idx & (N - 1); b = idx >> a.logN; }
-        const int64_t j = seq_j[b];
-        cpx<T> v = mk(T(0), T(0));
-        if (j >= 0) {
-            const int64_t gidx = n * a.in_gn + j * a.in_gj;
-            if (gidx < a.n_valid) {
-                v = a.x[seq_in[b] + n * a.in_ns];
-                if (a.conj_in) v = conj(v);
-                if (a.pre) v = cmul(v, a.pre[gidx]);
+    cpx<T> v[E];
+    // ---- stage-0 inputs straight from global: zero-pad, conj, pre-multiply
+    {
+        constexpr int NB = E / R0;
+        constexpr int LOGNR = LOGN - LOGR0, NR = 1 << LOGNR;
+        cpx<T> pv[E];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            int b, i;
+            bfly_pos<LOGB, LOGNR>(threadIdx.x + e * NT, a.seqfast_in, b, i);
+            const SeqPos sp = seq_pos(a, s0 + b);
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int n = i + r * NR;
+                const int64_t gidx = n * a.in_gn + sp.j * a.in_gj;
+                const bool ok = sp.ok && gidx < a.n_valid;
+                v[e * R0 + r] = ok ? a.x[sp.in_base + n * a.in_ns] : mk(T(0), T(0));
+                if (a.pre) pv[e * R0 + r] = ok ? a.pre[gidx] : mk(T(0), T(0));
             }
         }
-        lds[b * S + n] = v;
+        if (a.conj_in) {
+#pragma unroll
+            for (int q = 0; q < E; ++q) v[q] = conj(v[q]);
+        }
+        if (a.pre) {
+#pragma unroll
+            for (int q = 0; q < E; ++q) v[q] = cmul(v[q], pv[q]);
+        }
     }
-    __syncthreads();
-
-    if (a.sign1 < 0) lds_fft<T, -1, NT, MAXE>(lds, S, a.logN, B, a.tw);
-    else lds_fft<T, +1, NT, MAXE>(lds, S, a.logN, B, a.tw);
-
-    if constexpr (CONV) {
-        for (int idx = threadIdx.x; idx < tot; idx += NT) {
-            const int n = idx & (N - 1), b = idx >> a.logN;
-            const int64_t j = seq_j[b];
-            if (j >= 0) {
-                const cpx<T> d = a.D[n * a.d_gn + j * a.d_gj];
-                lds[b * S + n] = a.conj_d ? cmulc(lds[b * S + n], d) : cmul(lds[b * S + n], d);
+    // mapping of the butterflies currently held in v (see bfly_pos)
+    bool cur_sf = a.seqfast_in && B > 1;
+    if constexpr (P::S == 0) {
+        // N == 1: identity transform (conv: multiply by the single spectrum value)
+        if constexpr (CONV) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                int b, i;
+                bfly_pos<LOGB, 0>(threadIdx.x + e * NT, cur_sf, b, i);
+                const SeqPos sp = seq_pos(a, s0 + b);
+                if (sp.ok) {
+                    const cpx<T> d = a.D[sp.j * a.d_gj];
+                    v[e] = a.conj_d ? cmulc(v[e], d) : cmul(v[e], d);
+                }
+            }
+        }
+    } else if constexpr (!CONV) {
+        if (a.sign1 < 0) fft_all<T, LOGN, LOGE, -1, NT, LOGB, 0, 1>(v, lds, a.tw, cur_sf);
+        else fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 1>(v, lds, a.tw, cur_sf);
+        if (P::S > 1) cur_sf = false;
+    } else {
+        if (a.sign1 < 0) fft_all<T, LOGN, LOGE, -1, NT, LOGB, 0, 0>(v, lds, a.tw, cur_sf);
+        else fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 0>(v, lds, a.tw, cur_sf);
+        // spectrum multiply in LDS (natural order), then the inverse transform
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int idx = threadIdx.x + q * NT;
+            const int b = idx >> LOGN, n = idx & (N - 1);
+            const SeqPos sp = seq_pos(a, s0 + b);
+            if (sp.ok) {
+                const cpx<T> d = a.D[n * a.d_gn + sp.j * a.d_gj];
+                const cpx<T> u = lds[b * SB + lpad(n)];
+                lds[b * SB + lpad(n)] = a.conj_d ? cmulc(u, d) : cmul(u, d);
             }
         }
         __syncthreads();
-        if (a.sign1 < 0) lds_fft<T, +1, NT, MAXE>(lds, S, a.logN, B, a.tw);
-        else lds_fft<T, -1, NT, MAXE>(lds, S, a.logN, B, a.tw);
+        if (a.sign1 < 0) fft_all<T, LOGN, LOGE, +1, NT, LOGB, 1, 1>(v, lds, a.tw, false);
+        else fft_all<T, LOGN, LOGE, -1, NT, LOGB, 1, 1>(v, lds, a.tw, false);
+        cur_sf = false;
     }
 
-    // ---- store (four-step twiddle, crop, post-multiply, conj, scale)
-    const int lo_mask = (1 << a.twb_shift) - 1;
-    for (int idx = threadIdx.x; idx < tot; idx += NT) {
-        int b, n;
-        if (a.seqfast_out) { b = idx & (B - 1); n = idx >> logB; }
-        else { n = idx & (N - 1); b = idx >> a.logN; }
-        const int64_t j = seq_j[b];
-        if (j < 0) continue;
-        const int64_t gidx = n * a.out_gn + j * a.out_gj;
-        if (gidx >= a.m_valid) continue;
-        cpx<T> v = lds[b * S + n];
-        if (a.twb_hi) {
-            const int64_t e = j * n;  // < L
-            const cpx<T> w = cmul(a.twb_hi[e >> a.twb_shift], a.twb_lo[e & lo_mask]);
-            v = a.twb_sign < 0 ? cmul(v, w) : cmulc(v, w);
+    // ---- last-stage outputs: v[e*RL + r] is element i + r*N/RL of sequence
+    // b in mapping cur_sf.  Re-stage through LDS when the store wants the
+    // other mapping (transposed stores write neighbouring sequences together).
+    constexpr int NBL = E / RL;
+    constexpr int LOGNRL = LOGN - LOGRL, NRL = 1 << LOGNRL;
+    const bool want_sf = a.seqfast_out && B > 1;
+    if (want_sf != cur_sf) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < NBL; ++e) {
+            int b, i;
+            bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, cur_sf, b, i);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) lds[b * SB + lpad(i + r * NRL)] = v[e * RL + r];
         }
-        if (a.post) v = cmul(v, a.post[gidx]);
-        if (a.conj_out) v = conj(v);
-        v = scale(v, a.out_scale);
-        a.y[seq_out[b] + n * a.out_ns] = v;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < NBL; ++e) {
+            int b, i;
+            bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, want_sf, b, i);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) v[e * RL + r] = lds[b * SB + lpad(i + r * NRL)];
+        }
+        cur_sf = want_sf;
+    }
+    const int lo_mask = (1 << a.twb_shift) - 1;
+#pragma unroll
+    for (int e = 0; e < NBL; ++e) {
+        int b, i;
+        bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, cur_sf, b, i);
+        const SeqPos sp = seq_pos(a, s0 + b);
+        if (!sp.ok) continue;
+        cpx<T> w[RL];
+        if (a.twb_hi || a.post) {
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const int n = i + r * NRL;
+                cpx<T> t = mk(T(1), T(0));
+                if (a.twb_hi) {
+                    const int64_t ex = sp.j * n;  // < L
+                    t = cmul(a.twb_hi[ex >> a.twb_shift], a.twb_lo[ex & lo_mask]);
+                    if (a.twb_sign > 0) t = conj(t);
+                }
+                const int64_t gidx = n * a.out_gn + sp.j * a.out_gj;
+                if (a.post && gidx < a.m_valid) t = cmul(t, a.post[gidx]);
+                w[r] = t;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            const int n = i + r * NRL;
+            const int64_t gidx = n * a.out_gn + sp.j * a.out_gj;
+            if (gidx >= a.m_valid) continue;
+            cpx<T> u = v[e * RL + r];
+            if (a.twb_hi || a.post) u = cmul(u, w[r]);
+            if (a.conj_out) u = conj(u);
+            a.y[sp.out_base + n * a.out_ns] = scale(u, a.out_scale);
+        }
     }
 }
 
@@ -266,6 +446,14 @@ fm_status launch_tile(const TileArgs<T> &a, bool conv, hipStream_t stream);
 // Largest tile (B*N elements) the engine supports for one precision.
 template <typename T> constexpr int max_tile_elems() {
     return sizeof(T) == 4 ? 16384 : 8192;
+}
+
+// Sequences per tile of the pass kernel for a length-2^logN transform.
+template <typename T> inline int tile_batch(int logN) {
+    const int loge = sizeof(T) == 4 ? 5 : 4;
+    const int lognt = (sizeof(T) == 4 ? logN >= 14 : logN >= 13) ? 9 : 8;
+    const int logb = lognt + loge - logN;
+    return logb > 0 ? (1 << logb) : 1;
 }
 
 }  // namespace fm

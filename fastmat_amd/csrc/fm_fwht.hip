@@ -149,9 +149,17 @@ static fm_status wht_launch(const WhtArgs &a, hipStream_t stream) {
     }
     FM_REQUIRE(a.ntiles > 0 && a.ntiles < (int64_t(1) << 31), FM_ERR_VALUE,
                "tile count out of range");
+    ProfToken tok;
+    prof_begin(stream, &tok);
     hipLaunchKernelGGL((wht_tile_kernel<S, 256>), dim3(unsigned(a.ntiles)), dim3(256), lds,
                        stream, a);
     FM_CHECK_LAUNCH();
+    if (tok.active) {
+        const double elems = double(a.O) * double(a.J) * double(N);
+        char name[64];
+        snprintf(name, sizeof(name), "wht_tile_kernel<%dB>", int(sizeof(S)));
+        prof_end(stream, &tok, name, 2.0 * elems * sizeof(S), elems * a.logN);
+    }
     return FM_SUCCESS;
 }
 

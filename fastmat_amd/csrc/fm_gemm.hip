@@ -143,8 +143,14 @@ static fm_status gemm_launch(const GemmArgs &g, hipStream_t stream) {
     const int64_t tiles = ceil_div(g.M, GT_M) * ceil_div(g.NC, GT_N);
     FM_REQUIRE(tiles < (int64_t(1) << 31), FM_ERR_VALUE, "GEMM too large");
     if (tiles == 0) return FM_SUCCESS;
+    ProfToken tok;
+    prof_begin(stream, &tok);
     hipLaunchKernelGGL(gemm_mfma_kernel, dim3(unsigned(tiles)), dim3(256), 0, stream, g);
     FM_CHECK_LAUNCH();
+    if (tok.active) {
+        const double bytes = 4.0 * (double(g.M) * g.K + double(g.K) * g.NC + double(g.M) * g.NC);
+        prof_end(stream, &tok, "gemm_mfma_kernel", bytes, 2.0 * double(g.M) * g.K * g.NC);
+    }
     return FM_SUCCESS;
 }
 

@@ -196,6 +196,18 @@ fm_status fm_axpby(int dtype, int64_t n, int64_t k, const double *alpha,
                    const void *x, int64_t ldx, int conj_x, const double *beta,
                    void *y, int64_t ldy, void *stream);
 
+/* ------------------------------------------------------------ profiler
+ * No counterpart in the reference.  For benchmarks: when enabled, each of the
+ * library's kernel launches is bracketed by HIP events on its own stream and
+ * tagged with its algorithmic bytes (compulsory reads + writes) and flops.
+ * fm_profiler_count() synchronises on the recorded events and aggregates per
+ * kernel label; fm_profiler_entry() reads one aggregate. */
+fm_status fm_profiler_enable(int on);
+fm_status fm_profiler_reset(void);
+fm_status fm_profiler_count(int *count);
+fm_status fm_profiler_entry(int index, char *name, int name_len, int64_t *launches,
+                            double *total_ms, double *total_bytes, double *total_flops);
+
 #ifdef __cplusplus
 }
 #endif
