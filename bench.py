@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="fourier",
-                   choices=["fourier", "hadamard", "circulant", "kron"])
+                   choices=["fourier", "hadamard", "circulant", "kron", "batch"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cols", type=int, default=None, help="override the column count")
     return p.parse_args()
@@ -232,8 +232,103 @@ def workload_kron(args, rng):
                 dtype="c64", metric="batched fwd+adjoint GB/s (% HBM peak)")
 
 
+def make_uneven_batch(seed=20261202 + 4, count=256, lo=256, hi=65536, kmax=32):
+    """The uneven batch: `count` operators, half Circulant(N), half
+    Toeplitz(m x n); N and m log-uniform in [lo, hi], n = round(m / r) with
+    r ~ U[0.5, 2] clipped to [lo, hi], k ~ U{1..kmax}; generators iid complex
+    N(0, 1) in complex64.  Returns [(kind, m, n, k, generator)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        k = int(rng.integers(1, kmax + 1))
+        if i % 2 == 0:
+            n = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+            gen = crandn(rng, (n,))
+            out.append(("circulant", n, n, k, gen))
+        else:
+            m = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+            r = rng.uniform(0.5, 2.0)
+            n = int(min(max(round(m / r), lo), hi))
+            gen = crandn(rng, (m + n - 1,))
+            out.append(("toeplitz", m, n, k, gen))
+    return out
+
+
+def build_batch_ops(spec):
+    import fastmat_amd as fm
+    ops, ks = [], []
+    for kind, m, n, k, gen in spec:
+        if kind == "circulant":
+            ops.append(fm.Circulant(gen))
+        else:
+            ops.append(fm.Toeplitz(gen[:m], gen[m:]))
+        ks.append(k)
+    return ops, ks
+
+
+def workload_batch(args, rng):
+    import torch
+    from fastmat_amd.batch import ConvBatch, plain_torch_path
+    spec = make_uneven_batch()
+    ops, ks = build_batch_ops(spec)
+    batch = ConvBatch(ops, ks)
+    xs = [device_block(crandn(rng, (op.numCols, k)), torch.complex64) for op, k in zip(ops, ks)]
+    gs = [device_block(crandn(rng, (op.numRows, k)), torch.complex64) for op, k in zip(ops, ks)]
+    ys = batch.new_outputs(False)
+    zs = batch.new_outputs(True)
+    grid_f = batch.tune(xs, ys, False)
+    grid_b = batch.tune(gs, zs, True)
+    gf = batch.graph(xs, ys, False)
+    gb = batch.graph(gs, zs, True)
+    bytes_step = batch.algorithmic_bytes(False) + batch.algorithmic_bytes(True)
+
+    def step():
+        gf.replay()
+        gb.replay()
+
+    def check():
+        from oracle import accept, fast64
+        errs = []
+        torch.cuda.synchronize()
+        for i in range(0, len(ops), 37):
+            kind, m, n, k, gen = spec[i]
+            x = xs[i].cpu().numpy().T
+            g = gs[i].cpu().numpy().T
+            if kind == "circulant":
+                rf = fast64.circulant(gen, x)
+                rb = fast64.circulant(gen, g, adjoint=True)
+            else:
+                rf = fast64.toeplitz(gen, m, x)
+                rb = fast64.toeplitz(gen, m, g, adjoint=True)
+            errs.append(accept.max_rel_error(ys[i].cpu().numpy().T, rf))
+            errs.append(accept.max_rel_error(zs[i].cpu().numpy().T, rb))
+        return {"sampled_ops_max": max(errs)}
+
+    def plain():
+        def s():
+            plain_torch_path(ops, xs, False)
+            plain_torch_path(ops, gs, True)
+        return s
+
+    def per_op():
+        def s():
+            for op, x in zip(ops, xs):
+                op._apply(x, False)
+            for op, g in zip(ops, gs):
+                op._apply(g, True)
+        return s
+
+    n_multi = sum(1 for op in ops if op.fftLength > 8192)
+    cfg = {"workload": "uneven_batch_256_circulant_toeplitz_c64_fwd_adj", "operators": len(ops),
+           "multi_pass_operators": n_multi, "total_columns": int(sum(ks)),
+           "grid_per_cu": [grid_f, grid_b], "hipgraph": True}
+    return dict(step=step, bytes=bytes_step, check=check, plain=plain(), per_op=per_op(),
+                cpu=None, cfg=cfg, dtype="c64",
+                metric="batched fwd+adjoint GB/s (% HBM peak) vs plain per-op GPU")
+
+
 WORKLOADS = {"fourier": workload_fourier, "hadamard": workload_hadamard,
-             "circulant": workload_circulant, "kron": workload_kron}
+             "circulant": workload_circulant, "kron": workload_kron, "batch": workload_batch}
 
 
 def time_cpu(legs):
@@ -314,6 +409,17 @@ def main():
         pms = (time.perf_counter() - t2) * 1e3 / args.steps
         extra["plain_torch_fft_ms_per_step"] = round(pms, 3)
         extra["speedup_vs_plain_torch_fft"] = round(pms / ms, 3)
+    if w.get("per_op") is not None:
+        s = w["per_op"]
+        s()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            s()
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - t2) * 1e3 / args.steps
+        extra["per_operator_library_ms_per_step"] = round(pms, 3)
+        extra["speedup_vs_per_operator_library"] = round(pms / ms, 3)
 
     cpu_baseline = None
     if not args.no_cpu and w.get("cpu") is not None:

@@ -66,6 +66,13 @@ _PROTOTYPES = {
                                  _vp]),
     "fm_axpby": (_c_int, [_c_int, _c_i64, _c_i64, ctypes.POINTER(ctypes.c_double), _vp, _c_i64,
                           _c_int, ctypes.POINTER(ctypes.c_double), _vp, _c_i64, _vp]),
+    "fm_batch_plan_create": (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _pp]),
+    "fm_batch_workspace_size": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
+    "fm_batch_set_grid": (_c_int, [_vp, _c_int]),
+    "fm_batch_bind": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                               ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _vp, _c_i64, _vp]),
+    "fm_batch_apply": (_c_int, [_vp, _c_int, _vp]),
+    "fm_batch_plan_destroy": (_c_int, [_vp]),
     "fm_profiler_enable": (_c_int, [_c_int]),
     "fm_profiler_reset": (_c_int, []),
     "fm_profiler_count": (_c_int, [ctypes.POINTER(_c_int)]),

@@ -1,0 +1,261 @@
+"""Uneven batches of Circulant / Toeplitz operators in a few persistent launches.
+
+A fastmat user applies many operators by calling ``forward``/``backward`` per
+operator (Matrix.pyx:1844-2007 once per matrix).  ``ConvBatch`` applies all
+of them together: the library groups every operator's passes by phase and
+transform length, and runs one persistent kernel per group over a device work
+table (see include/fastmat_amd.h, fm_batch_*).  Buffers are bound once per
+set of device pointers; the apply itself only launches kernels, so it can be
+captured in a HIP graph (``ConvBatch.graph``) and replayed.
+
+The launch shape (workgroups per CU of the persistent grid) is chosen by
+``Autotuner``, which times the candidates on the actual batch with HIP events
+and caches the winner per batch signature (in memory, and in a JSON file when
+FASTMAT_AMD_TUNE_CACHE names one).
+"""
+import ctypes
+import hashlib
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+from .circulant import _ConvBase
+from .fourier import precision_of
+
+
+class Autotuner:
+    """Times candidate launch configurations and caches the fastest."""
+
+    def __init__(self, path=None):
+        self.path = path if path is not None else os.environ.get("FASTMAT_AMD_TUNE_CACHE")
+        self.cache = {}
+        if self.path and os.path.exists(self.path):
+            try:
+                with open(self.path) as f:
+                    self.cache = json.load(f)
+            except (OSError, ValueError):
+                self.cache = {}
+
+    def best(self, key, candidates, run, reps=5):
+        """Return the candidate with the lowest mean time of run(candidate)."""
+        if key in self.cache and self.cache[key]["best"] in candidates:
+            return self.cache[key]["best"]
+        times = {}
+        for c in candidates:
+            run(c)  # warm-up / first-touch
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                run(c)
+            e1.record()
+            e1.synchronize()
+            times[c] = e0.elapsed_time(e1) / reps
+        best = min(times, key=times.get)
+        self.cache[key] = {"best": best, "ms": {str(k): v for k, v in times.items()}}
+        if self.path:
+            try:
+                os.makedirs(os.path.dirname(os.path.abspath(self.path)), exist_ok=True)
+                with open(self.path, "w") as f:
+                    json.dump(self.cache, f, indent=1)
+            except OSError:
+                pass
+        return best
+
+
+_default_tuner = None
+
+
+def default_tuner():
+    global _default_tuner
+    if _default_tuner is None:
+        _default_tuner = Autotuner()
+    return _default_tuner
+
+
+class ConvBatch:
+    """Apply a list of Circulant/Toeplitz operators with per-operator column
+    counts ``ks`` in one batched call.
+
+    ``dtype`` (complex64 default) is the compute and data type of every
+    operator in the batch.
+    """
+
+    def __init__(self, ops, ks, dtype=np.complex64):
+        if len(ops) != len(ks) or not ops:
+            raise ValueError("ConvBatch: need one column count per operator")
+        for op in ops:
+            if not isinstance(op, _ConvBase):
+                raise TypeError("ConvBatch: operators must be Circulant or Toeplitz")
+        self.ops = list(ops)
+        self.ks = [int(k) for k in ks]
+        self.dtype = np.dtype(dtype)
+        prec = precision_of(self.dtype)
+        handles = [op.plan_handle(self.dtype) for op in ops]
+        arr = (ctypes.c_void_p * len(ops))(*[h.value for h in handles])
+        karr = (ctypes.c_int64 * len(ops))(*self.ks)
+        self._h = ctypes.c_void_p()
+        _lib.call("fm_batch_plan_create", len(ops), arr, karr, ctypes.byref(self._h))
+        need = ctypes.c_int64()
+        _lib.call("fm_batch_workspace_size", self._h, ctypes.byref(need))
+        self._ws = D.workspace(need.value)
+        self._ws_bytes = need.value
+        self._bound = [None, None]
+        self.grid_per_cu = 2
+        self._prec = prec
+
+    def __del__(self):
+        try:
+            if self._h and _lib._lib is not None:
+                _lib._lib.fm_batch_plan_destroy(self._h)
+        except Exception:
+            pass
+
+    @property
+    def signature(self):
+        desc = ",".join("%s%dx%d:%d" % (type(op).__name__[0], op.numRows, op.numCols, k)
+                        for op, k in zip(self.ops, self.ks))
+        return "batch-%s-%s" % (self.dtype.name, hashlib.sha1(desc.encode()).hexdigest()[:16])
+
+    def algorithmic_bytes(self, adjoint=False):
+        """Bytes one apply must move: read every input, write every output,
+        read every spectrum once."""
+        esz = self.dtype.itemsize
+        tot = 0
+        for op, k in zip(self.ops, self.ks):
+            tot += (op.numRows + op.numCols) * k * esz + op.fftLength * esz
+        return tot
+
+    # ----------------------------------------------------------- buffers
+    def new_inputs(self, adjoint=False, fill=None):
+        """Allocate input blocks (k_i, n_i) for a direction."""
+        out = []
+        for op, k in zip(self.ops, self.ks):
+            n = op.numRows if adjoint else op.numCols
+            t = D.empty(k, n, self.dtype)
+            if fill is not None:
+                t.copy_(fill(k, n))
+            out.append(t)
+        return out
+
+    def new_outputs(self, adjoint=False):
+        return [D.empty(k, (op.numCols if adjoint else op.numRows), self.dtype)
+                for op, k in zip(self.ops, self.ks)]
+
+    def bind(self, ins, outs, adjoint=False):
+        key = tuple(t.data_ptr() for t in ins) + tuple(t.data_ptr() for t in outs)
+        if self._bound[int(adjoint)] == key:
+            return
+        for i, (op, k) in enumerate(zip(self.ops, self.ks)):
+            nin = op.numRows if adjoint else op.numCols
+            mout = op.numCols if adjoint else op.numRows
+            if tuple(ins[i].shape) != (k, nin) or tuple(outs[i].shape) != (k, mout):
+                raise ValueError("ConvBatch: buffer %d has the wrong shape" % i)
+            if not (ins[i].is_contiguous() and outs[i].is_contiguous()):
+                raise ValueError("ConvBatch: buffers must be contiguous blocks")
+            if D.np_dtype(ins[i].dtype) != self.dtype or D.np_dtype(outs[i].dtype) != self.dtype:
+                raise TypeError("ConvBatch: buffers must be %s" % self.dtype)
+        n = len(self.ops)
+        pin = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ins])
+        pout = (ctypes.c_void_p * n)(*[t.data_ptr() for t in outs])
+        ldin = (ctypes.c_int64 * n)(*[t.shape[1] for t in ins])
+        ldout = (ctypes.c_int64 * n)(*[t.shape[1] for t in outs])
+        ws = None if self._ws is None else _lib.ptr(self._ws)
+        _lib.call("fm_batch_bind", self._h, int(adjoint), pin, ldin, pout, ldout, ws,
+                  self._ws_bytes, D.stream())
+        self._bound[int(adjoint)] = key
+        self._keep = getattr(self, "_keep", {})
+        self._keep[int(adjoint)] = (list(ins), list(outs))
+
+    def set_grid(self, workgroups_per_cu):
+        _lib.call("fm_batch_set_grid", self._h, int(workgroups_per_cu))
+        self.grid_per_cu = int(workgroups_per_cu)
+
+    def launch(self, adjoint=False):
+        """Launch the bound direction (kernels only; graph-capturable)."""
+        _lib.call("fm_batch_apply", self._h, int(adjoint), D.stream())
+
+    def apply(self, ins, outs=None, adjoint=False):
+        if outs is None:
+            outs = self.new_outputs(adjoint)
+        self.bind(ins, outs, adjoint)
+        self.launch(adjoint)
+        return outs
+
+    def forward(self, ins, outs=None):
+        return self.apply(ins, outs, False)
+
+    def backward(self, ins, outs=None):
+        return self.apply(ins, outs, True)
+
+    # ---------------------------------------------------------- tuning
+    def tune(self, ins, outs, adjoint=False, tuner=None, candidates=(1, 2, 3, 4)):
+        tuner = tuner or default_tuner()
+        self.bind(ins, outs, adjoint)
+
+        def run(c):
+            self.set_grid(c)
+            self.launch(adjoint)
+
+        best = tuner.best(self.signature + ("-adj" if adjoint else "-fwd"), list(candidates), run)
+        self.set_grid(best)
+        return best
+
+    # ----------------------------------------------------------- graphs
+    def graph(self, ins, outs, adjoint=False):
+        """Capture the bound apply into a HIP graph; returns the graph (call
+        ``.replay()``)."""
+        self.bind(ins, outs, adjoint)
+        self.launch(adjoint)  # first launch outside capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.launch(adjoint)
+        torch.cuda.current_stream().wait_stream(s)
+        return g
+
+
+def plain_torch_path(ops, xs, adjoint=False):
+    """The plain per-operator GPU path a user would write with torch.fft:
+    for every operator, zero-pad, FFT, multiply by the spectrum, inverse FFT
+    and crop (the Product(F.H, Diag, F) structure, Circulant.pyx:127-137),
+    one operator after another.  Used as the comparison baseline."""
+    outs = []
+    for op, X in zip(ops, xs):
+        L = op.fftLength
+        spec = getattr(op, "_torch_spec", None)
+        if spec is None or spec.dtype != X.dtype:
+            t = _embedded_generator(op)
+            spec = torch.from_numpy((np.fft.fft(t) / L).astype(D.np_dtype(X.dtype))).to(X.device)
+            op._torch_spec = spec
+        if adjoint:
+            spec = spec.conj()
+        Y = torch.fft.ifft(torch.fft.fft(X, n=L, dim=1) * spec, dim=1, norm="forward")
+        outs.append(Y[:, :(op.numCols if adjoint else op.numRows)])
+    return outs
+
+
+def _embedded_generator(op):
+    from .circulant import Circulant
+    L = op.fftLength
+    if isinstance(op, Circulant):
+        c = op._gen128
+        n = c.size
+        if L == n:
+            return c
+        return np.concatenate([c, np.zeros(L - 2 * n + 1), c[1:]])
+    t = op._gen128
+    m = op.numRows
+    out = np.zeros(L, dtype=np.complex128)
+    out[:m] = t[:m]
+    if t.size > m:
+        out[L - (t.size - m):] = t[m:]
+    return out

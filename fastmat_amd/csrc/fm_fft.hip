@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <map>
 #include <tuple>
+#include <algorithm>
 #include <vector>
 
 #include "fm_fft_core.hpp"
@@ -176,15 +177,22 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     return FM_SUCCESS;
 }
 
-template <typename T>
-fm_status launch_tile(const TileArgs<T> &a_in, bool conv, hipStream_t stream) {
-    TileArgs<T> a = a_in;
+// Fill in the tile shape (batch, tile count, log2 J) of a pass.
+template <typename T> static fm_status normalize_pass(TileArgs<T> &a) {
     const int maxlog = sizeof(T) == 4 ? 14 : 13;
     FM_REQUIRE(a.logN >= 0 && a.logN <= maxlog, FM_ERR_VALUE,
                "sub-transform length 2^%d exceeds the tile limit", a.logN);
     a.B = tile_batch<T>(a.logN);
     a.ntiles = ceil_div(a.O * a.J, a.B);
     a.logJ = is_pow2(a.J) ? ilog2(a.J) : -1;
+    return FM_SUCCESS;
+}
+
+template <typename T>
+fm_status launch_tile(const TileArgs<T> &a_in, bool conv, hipStream_t stream) {
+    TileArgs<T> a = a_in;
+    fm_status nst = normalize_pass(a);
+    if (nst) return nst;
 #define FM_PASS_CASE(L)                                                                  \
     case L:                                                                              \
         return conv ? launch_pass<T, L, true>(a, stream) : launch_pass<T, L, false>(a, stream);
@@ -292,8 +300,14 @@ struct PipeIO {
     void *work;              // L*k elements for multi-pass
 };
 
+template <typename T> struct PassSpec {
+    TileArgs<T> a;
+    bool conv;
+};
+
 template <typename T>
-static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t stream) {
+static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
+                                std::vector<PassSpec<T>> *out) {
     const bool conv = io.D != nullptr;
     const int64_t L = int64_t(1) << g.logL;
     const int max_tile = max_tile_elems<T>();
@@ -317,7 +331,8 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         a.out_gn = 1; a.out_gj = 0; a.m_valid = io.m_out;
         a.post = (const cpx<T> *)io.post;
         a.conj_out = io.conj_io;
-        return launch_tile<T>(a, conv, stream);
+        out->push_back(PassSpec<T>{a, conv});
+        return FM_SUCCESS;
     }
     FM_REQUIRE(io.work != nullptr, FM_ERR_VALUE, "multi-pass transform needs a workspace");
     const int64_t N1 = int64_t(1) << g.logN1, N2 = int64_t(1) << g.logN2;
@@ -341,8 +356,7 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         a.y = tmp;
         a.out_os = L; a.out_js = 1; a.out_ns = N2;
         a.seqfast_out = a.B > 1;
-        fm_status st = launch_tile<T>(a, false, stream);
-        if (st) return st;
+        out->push_back(PassSpec<T>{a, false});
     }
     if (!conv) {
         // pass B: N2-point FFTs along rows k1, transposed store y[k1 + N1 k2]
@@ -360,7 +374,8 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         a.post = (const cpx<T> *)io.post;
         a.conj_out = io.conj_io;
         a.seqfast_out = a.B > 1;
-        return launch_tile<T>(a, false, stream);
+        out->push_back(PassSpec<T>{a, false});
+        return FM_SUCCESS;
     }
     // pass B (conv): rows k1: FFT, spectrum D[k1 + N1 k2], inverse FFT,
     // twiddle W_L^{-n2 k1}; in place in the workspace
@@ -379,8 +394,7 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         a.twb_shift = g.twb_shift; a.twb_sign = +1;
         a.y = tmp;
         a.out_os = L; a.out_js = N2; a.out_ns = 1;
-        fm_status st = launch_tile<T>(a, true, stream);
-        if (st) return st;
+        out->push_back(PassSpec<T>{a, true});
     }
     // pass C: strided inverse N1-point FFTs over k1 -> y[n2 + N2 n1], crop
     {
@@ -400,8 +414,17 @@ static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t s
         a.post = (const cpx<T> *)io.post;
         a.conj_out = io.conj_io;
         a.seqfast_out = a.B > 1;
-        return launch_tile<T>(a, false, stream);
+        out->push_back(PassSpec<T>{a, false});
+        return FM_SUCCESS;
     }
+}
+
+template <typename T>
+static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t stream) {
+    std::vector<PassSpec<T>> passes;
+    fm_status st = build_pipeline<T>(g, io, &passes);
+    for (size_t q = 0; !st && q < passes.size(); ++q) st = launch_tile<T>(passes[q].a, passes[q].conv, stream);
+    return st;
 }
 
 static fm_status run_pipeline_any(const Geometry &g, const PipeIO &io, hipStream_t stream) {
@@ -864,6 +887,290 @@ extern "C" fm_status fm_conv_generator_grad(const fm_conv_plan *p, const void *x
 extern "C" fm_status fm_conv_plan_destroy(fm_conv_plan *p) {
     if (!p) return FM_SUCCESS;
     if (p->D) hipFree(p->D);
+    delete p;
+    return FM_SUCCESS;
+}
+
+// ======================================================================
+//                    Uneven batch of Circulant / Toeplitz
+// ======================================================================
+// Every operator's transform is a list of passes (one fused pass, or the
+// three passes A, B, C of the four-step); pass q of all operators forms
+// phase q.  One persistent kernel per phase walks a device table of
+// (descriptor, tile) work items in decreasing-cost order, grid-strided, and
+// runs the compile-time FFT body the descriptor's length selects.  Items of
+// one phase are independent, so no workgroup ever waits for another.
+namespace fm {
+
+template <typename T> struct BatchDesc {
+    TileArgs<T> a;
+    int conv;
+};
+
+// One persistent kernel per (length, conv) class: grid-strided over the
+// class's (descriptor, tile) items, most expensive first.
+template <typename T, int LOGN, bool CONV>
+__global__ __launch_bounds__(256) void batch_pass_kernel(const BatchDesc<T> *__restrict__ descs,
+                                                         const int2 *__restrict__ items,
+                                                         int64_t nitems) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 w = items[it];
+        const TileArgs<T> a = descs[w.x].a;
+        fft_pass_body<T, LOGN, 256, CONV>(a, w.y, lds);
+        __syncthreads();  // the next item reuses the LDS tile
+    }
+}
+
+template <typename T> constexpr int batch_max_log() { return sizeof(T) == 4 ? 13 : 12; }
+
+// Work items of one (phase, length, conv) class, in launch order.
+struct BatchGroup {
+    int phase, logN, conv;
+    void *items;
+    int64_t nitems;
+    double bytes;
+};
+
+struct BatchDir {
+    void *descs = nullptr;
+    std::vector<BatchGroup> groups;
+    bool bound = false;
+};
+
+}  // namespace fm
+
+struct fm_batch_plan {
+    int device;
+    int prec;
+    int count;
+    std::vector<const fm_conv_plan *> plans;
+    std::vector<int64_t> ks;
+    std::vector<int64_t> ws_off;
+    int64_t ws_bytes;
+    int grid_per_cu;
+    int ncu;
+    BatchDir dir[2];
+};
+
+static void batch_free_dir(BatchDir &d) {
+    if (d.descs) (void)hipFree(d.descs);
+    for (auto &g : d.groups)
+        if (g.items) (void)hipFree(g.items);
+    d = BatchDir();
+}
+
+extern "C" fm_status fm_batch_plan_create(int count, const fm_conv_plan *const *plans,
+                                          const int64_t *ks, fm_batch_plan **out) {
+    FM_REQUIRE(out != nullptr, FM_ERR_VALUE, "plan pointer is NULL");
+    *out = nullptr;
+    FM_REQUIRE(count >= 1 && plans && ks, FM_ERR_VALUE, "empty batch");
+    const int prec = plans[0]->prec;
+    auto *p = new fm_batch_plan();
+    p->prec = prec;
+    p->count = count;
+    p->ws_bytes = 0;
+    p->grid_per_cu = 2;
+    (void)hipGetDevice(&p->device);
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+        ncu <= 0)
+        ncu = 256;
+    p->ncu = ncu;
+    for (int i = 0; i < count; ++i) {
+        if (!plans[i] || plans[i]->prec != prec || ks[i] < 1) {
+            delete p;
+            set_error("batch entry %d: missing plan, mixed precision or no columns", i);
+            return FM_ERR_VALUE;
+        }
+        const int maxlog = prec == FM_COMPLEX64 ? batch_max_log<float>() : batch_max_log<double>();
+        const Geometry &g = plans[i]->geo;
+        const int worst = g.passes == 1 ? g.logL : (g.logN1 > g.logN2 ? g.logN1 : g.logN2);
+        if (worst > maxlog) {
+            delete p;
+            set_error("batch entry %d: transform length exceeds the batched kernel limit", i);
+            return FM_ERR_NOT_IMPLEMENTED;
+        }
+        p->plans.push_back(plans[i]);
+        p->ks.push_back(ks[i]);
+        p->ws_off.push_back(p->ws_bytes);
+        if (g.passes > 1) {
+            int64_t b = int64_t(esize(prec)) * plans[i]->L * ks[i];
+            p->ws_bytes += (b + 255) / 256 * 256;
+        }
+    }
+    *out = p;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_workspace_size(const fm_batch_plan *p, int64_t *bytes) {
+    FM_REQUIRE(p && bytes, FM_ERR_VALUE, "NULL argument");
+    *bytes = p->ws_bytes;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_set_grid(fm_batch_plan *p, int workgroups_per_cu) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(workgroups_per_cu >= 1 && workgroups_per_cu <= 16, FM_ERR_VALUE,
+               "workgroups per CU must be in [1, 16]");
+    p->grid_per_cu = workgroups_per_cu;
+    return FM_SUCCESS;
+}
+
+template <typename T>
+static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *ins,
+                              const int64_t *ld_in, void *const *outs, const int64_t *ld_out,
+                              void *workspace, hipStream_t s) {
+    BatchDir &d = p->dir[adjoint ? 1 : 0];
+    batch_free_dir(d);
+    std::vector<BatchDesc<T>> descs;
+    std::vector<int> phase_of;
+    for (int i = 0; i < p->count; ++i) {
+        const fm_conv_plan *c = p->plans[i];
+        const int64_t nin = adjoint ? c->rows : c->cols;
+        const int64_t mout = adjoint ? c->cols : c->rows;
+        FM_REQUIRE(ins[i] && outs[i], FM_ERR_VALUE, "batch entry %d: NULL buffer", i);
+        FM_REQUIRE(ld_in[i] >= nin && ld_out[i] >= mout, FM_ERR_VALUE,
+                   "batch entry %d: leading dimension too small", i);
+        PipeIO io;
+        memset(&io, 0, sizeof(io));
+        io.x = ins[i]; io.ldx = ld_in[i]; io.y = outs[i]; io.ldy = ld_out[i]; io.k = p->ks[i];
+        io.n_in = nin; io.m_out = mout; io.D = c->D; io.conj_d = adjoint ? 1 : 0;
+        io.work = c->geo.passes > 1 ? (char *)workspace + p->ws_off[i] : nullptr;
+        FM_REQUIRE(c->geo.passes == 1 || workspace, FM_ERR_VALUE, "batch needs a workspace");
+        std::vector<PassSpec<T>> passes;
+        fm_status st = build_pipeline<T>(c->geo, io, &passes);
+        if (st) return st;
+        for (size_t q = 0; q < passes.size(); ++q) {
+            TileArgs<T> a = passes[q].a;
+            st = normalize_pass(a);
+            if (st) return st;
+            descs.push_back(BatchDesc<T>{a, passes[q].conv ? 1 : 0});
+            phase_of.push_back(int(q));
+        }
+    }
+    // work items grouped by (phase, length, conv); groups of a phase run
+    // largest first, items inside a group in descriptor order
+    std::map<std::tuple<int, int, int>, std::vector<int2>> groups;
+    std::map<std::tuple<int, int, int>, double> gbytes;
+    for (size_t di = 0; di < descs.size(); ++di) {
+        const TileArgs<T> &a = descs[di].a;
+        auto key = std::make_tuple(phase_of[di], -a.logN, descs[di].conv ? 0 : 1);
+        auto &v = groups[key];
+        for (int64_t t = 0; t < a.ntiles; ++t) {
+            int2 w;
+            w.x = int(di);
+            w.y = int(t);
+            v.push_back(w);
+        }
+        const double span = double(a.J) * double(int64_t(1) << a.logN);
+        const double in = double(a.O) * (double(a.n_valid) < span ? double(a.n_valid) : span);
+        const double outb = double(a.O) * (double(a.m_valid) < span ? double(a.m_valid) : span);
+        gbytes[key] += (in + outb) * sizeof(cpx<T>);
+    }
+    FM_CHECK_HIP(hipMalloc(&d.descs, sizeof(BatchDesc<T>) * descs.size()));
+    FM_CHECK_HIP(hipMemcpyAsync(d.descs, descs.data(), sizeof(BatchDesc<T>) * descs.size(),
+                                hipMemcpyHostToDevice, s));
+    for (auto &kv : groups) {
+        BatchGroup g;
+        g.phase = std::get<0>(kv.first);
+        g.logN = -std::get<1>(kv.first);
+        g.conv = std::get<2>(kv.first) == 0 ? 1 : 0;
+        g.nitems = int64_t(kv.second.size());
+        g.bytes = gbytes[kv.first];
+        g.items = nullptr;
+        FM_CHECK_HIP(hipMalloc(&g.items, sizeof(int2) * kv.second.size()));
+        FM_CHECK_HIP(hipMemcpyAsync(g.items, kv.second.data(), sizeof(int2) * kv.second.size(),
+                                    hipMemcpyHostToDevice, s));
+        d.groups.push_back(g);
+    }
+    FM_CHECK_HIP(hipStreamSynchronize(s));
+    d.bound = true;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_bind(fm_batch_plan *p, int adjoint, const void *const *ins,
+                                   const int64_t *ld_in, void *const *outs,
+                                   const int64_t *ld_out, void *workspace,
+                                   int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p && ins && ld_in && outs && ld_out, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(workspace_bytes >= p->ws_bytes, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)p->ws_bytes);
+    if (p->prec == FM_COMPLEX64)
+        return batch_bind_t<float>(p, adjoint, ins, ld_in, outs, ld_out, workspace,
+                                   as_stream(stream));
+    return batch_bind_t<double>(p, adjoint, ins, ld_in, outs, ld_out, workspace,
+                                as_stream(stream));
+}
+
+template <typename T, int LOGN, bool CONV>
+static fm_status batch_launch(const fm_batch_plan *p, const BatchDir &d, const BatchGroup &g,
+                              hipStream_t s) {
+    constexpr int N = 1 << LOGN;
+    const size_t lds = size_t(tile_batch<T>(LOGN)) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)batch_pass_kernel<T, LOGN, CONV>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    int64_t grid = int64_t(p->ncu) * p->grid_per_cu;
+    if (grid > g.nitems) grid = g.nitems;
+    ProfToken tok;
+    prof_begin(s, &tok);
+    hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV>), dim3(unsigned(grid)), dim3(256), lds, s,
+                       (const BatchDesc<T> *)d.descs, (const int2 *)g.items, g.nitems);
+    FM_CHECK_LAUNCH();
+    if (tok.active) {
+        char name[80];
+        snprintf(name, sizeof(name), "batch_pass_kernel<%s,%d,%s>",
+                 sizeof(T) == 4 ? "float" : "double", LOGN, CONV ? "conv" : "fft");
+        prof_end(s, &tok, name, g.bytes, 0.0);
+    }
+    return FM_SUCCESS;
+}
+
+template <typename T>
+static fm_status batch_apply_t(const fm_batch_plan *p, const BatchDir &d, hipStream_t s) {
+    for (const BatchGroup &g : d.groups) {
+        fm_status st = FM_ERR_VALUE;
+#define FM_BATCH_CASE(L)                                                                 \
+    case L:                                                                              \
+        st = g.conv ? batch_launch<T, L, true>(p, d, g, s) : batch_launch<T, L, false>(p, d, g, s); \
+        break;
+        switch (g.logN) {
+            FM_BATCH_CASE(0) FM_BATCH_CASE(1) FM_BATCH_CASE(2) FM_BATCH_CASE(3)
+            FM_BATCH_CASE(4) FM_BATCH_CASE(5) FM_BATCH_CASE(6) FM_BATCH_CASE(7)
+            FM_BATCH_CASE(8) FM_BATCH_CASE(9) FM_BATCH_CASE(10) FM_BATCH_CASE(11)
+            FM_BATCH_CASE(12)
+        case 13:
+            if constexpr (sizeof(T) == 4)
+                st = g.conv ? batch_launch<T, 13, true>(p, d, g, s)
+                            : batch_launch<T, 13, false>(p, d, g, s);
+            break;
+        default:
+            break;
+        }
+#undef FM_BATCH_CASE
+        if (st) return st;
+    }
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_apply(const fm_batch_plan *p, int adjoint, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    const BatchDir &d = p->dir[adjoint ? 1 : 0];
+    FM_REQUIRE(d.bound, FM_ERR_VALUE, "batch direction %d is not bound to buffers", adjoint);
+    if (p->prec == FM_COMPLEX64) return batch_apply_t<float>(p, d, as_stream(stream));
+    return batch_apply_t<double>(p, d, as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_plan_destroy(fm_batch_plan *p) {
+    if (!p) return FM_SUCCESS;
+    batch_free_dir(p->dir[0]);
+    batch_free_dir(p->dir[1]);
     delete p;
     return FM_SUCCESS;
 }

@@ -286,8 +286,10 @@ __device__ __forceinline__ void fft_all(cpx<T> *v, cpx<T> *lds, const cpx<T> *__
     }
 }
 
+// One tile of one pass: the body shared by the per-operator kernel and the
+// batched persistent kernel.  `lds` holds B * (N + N/32 + 1) elements.
 template <typename T, int LOGN, int NT, bool CONV>
-__global__ __launch_bounds__(NT) void fft_pass_kernel(const TileArgs<T> a) {
+__device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile, cpx<T> *lds) {
     constexpr int LOGE = sizeof(T) == 4 ? 5 : 4;
     constexpr int E = 1 << LOGE;
     constexpr int N = 1 << LOGN;
@@ -302,10 +304,6 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(const TileArgs<T> a) {
     constexpr int SB = N + (N >> 5) + 1;
     static_assert(LOGB >= 0, "tile must hold at least one sequence");
 
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
-
-    const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t s0 = tile << LOGB;
 
     cpx<T> v[E];
@@ -437,6 +435,13 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(const TileArgs<T> a) {
             a.y[sp.out_base + n * a.out_ns] = scale(u, a.out_scale);
         }
     }
+}
+
+template <typename T, int LOGN, int NT, bool CONV>
+__global__ __launch_bounds__(NT) void fft_pass_kernel(const TileArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    fft_pass_body<T, LOGN, NT, CONV>(a, xcd_remap(blockIdx.x, gridDim.x),
+                                     reinterpret_cast<cpx<T> *>(smem_raw));
 }
 
 // Host-side launcher for one tile pass (defined in fm_fft.hip).

@@ -153,6 +153,30 @@ fm_status fm_conv_generator_grad(const fm_conv_plan *plan, const void *x,
                                  int64_t workspace_bytes, void *stream);
 fm_status fm_conv_plan_destroy(fm_conv_plan *plan);
 
+/* ------------------------------------------------- uneven operator batch
+ * Many Circulant/Toeplitz plans of one precision, each with its own k_i
+ * right-hand sides, applied together.  No single reference call does this;
+ * it replaces the per-operator loop of forward()/backward() calls fastmat
+ * users write (Matrix.pyx:1844-2007 once per operator).  Every operator's
+ * passes are grouped by phase and by transform length; each group is one
+ * persistent launch over a device work table (at most a dozen or so launches
+ * per direction, none depending on another inside a phase), so the apply is
+ * graph-capturable.  fm_batch_bind() binds one direction to buffers: input i
+ * is cols_i x k_i (rows_i x k_i for the adjoint), output the other shape, and
+ * uploads the work tables (synchronising `stream`); fm_batch_apply() then
+ * only launches.  Plans must outlive the batch plan. */
+typedef struct fm_batch_plan fm_batch_plan;
+fm_status fm_batch_plan_create(int count, const fm_conv_plan *const *plans,
+                               const int64_t *ks, fm_batch_plan **plan);
+fm_status fm_batch_workspace_size(const fm_batch_plan *plan, int64_t *bytes);
+fm_status fm_batch_set_grid(fm_batch_plan *plan, int workgroups_per_cu);
+fm_status fm_batch_bind(fm_batch_plan *plan, int adjoint, const void *const *inputs,
+                        const int64_t *ld_in, void *const *outputs,
+                        const int64_t *ld_out, void *workspace,
+                        int64_t workspace_bytes, void *stream);
+fm_status fm_batch_apply(const fm_batch_plan *plan, int adjoint, void *stream);
+fm_status fm_batch_plan_destroy(fm_batch_plan *plan);
+
 /* --------------------------------------------------------------- dense
  * Y = A X (adjoint: Y = A^H X) for a dense m x n matrix A stored row-major
  * (A[i*lda + j]); replaces Matrix._forward / _backward (Matrix.pyx:1831-1842,
