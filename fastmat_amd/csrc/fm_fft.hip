@@ -85,11 +85,19 @@ __global__ void embed_kernel(cpx<double> *t, const cpx<double> *gen, int64_t L,
     t[i] = v;
 }
 
+// out[q] = s * in[perm(q)]; with logN2 > 0 the output is in the order the
+// four-step's row pass reads it: q = k1 * N2 + k2 holds bin k1 + N1 * k2.
 template <typename T>
-__global__ void scale_cast_kernel(cpx<T> *out, const cpx<double> *in, int64_t n, double s) {
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i >= n) return;
-    out[i] = mk(T(in[i].x * s), T(in[i].y * s));
+__global__ void scale_cast_kernel(cpx<T> *out, const cpx<double> *in, int64_t n, double s,
+                                  int logN1, int logN2) {
+    const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (q >= n) return;
+    int64_t src = q;
+    if (logN2 > 0) {
+        const int64_t k1 = q >> logN2, k2 = q & ((int64_t(1) << logN2) - 1);
+        src = k1 + (k2 << logN1);
+    }
+    out[q] = mk(T(in[src].x * s), T(in[src].y * s));
 }
 
 static inline unsigned grid1d(int64_t n, int bs = 256) { return unsigned(ceil_div(n, bs)); }
@@ -389,7 +397,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.in_os = L; a.in_js = N2; a.in_ns = 1;
         a.tw = (const cpx<T> *)g.tw2;
         a.D = (const cpx<T> *)io.D;
-        a.d_gn = N1; a.d_gj = 1; a.conj_d = io.conj_d;
+        a.d_gn = 1; a.d_gj = N2; a.conj_d = io.conj_d;  // spectrum in row-pass order
         a.twb_hi = (const cpx<T> *)g.twb_hi; a.twb_lo = (const cpx<T> *)g.twb_lo;
         a.twb_shift = g.twb_shift; a.twb_sign = +1;
         a.y = tmp;
@@ -494,16 +502,22 @@ static fm_status fft64_inplace(cpx<double> *v, int logL, hipStream_t stream) {
 }
 
 // Spectrum D = fft(t')/L of a complex128 generator vector, stored in `prec`.
-static fm_status spectrum_from(cpx<double> *t, int logL, int prec, void *D, hipStream_t stream) {
+// The spectrum is stored in natural order for single-pass geometries and in
+// the row pass's order (see scale_cast_kernel) for four-step geometries, so
+// that the row pass reads it contiguously.
+static fm_status spectrum_from(cpx<double> *t, const Geometry &g, int prec, void *D,
+                               hipStream_t stream) {
+    const int logL = g.logL;
     const int64_t L = int64_t(1) << logL;
     fm_status st = fft64_inplace(t, logL, stream);
     if (st) return st;
+    const int l1 = g.passes > 1 ? g.logN1 : 0, l2 = g.passes > 1 ? g.logN2 : 0;
     if (prec == FM_COMPLEX64)
         hipLaunchKernelGGL(scale_cast_kernel<float>, grid1d(L), 256, 0, stream,
-                           (cpx<float> *)D, t, L, 1.0 / double(L));
+                           (cpx<float> *)D, t, L, 1.0 / double(L), l1, l2);
     else
         hipLaunchKernelGGL(scale_cast_kernel<double>, grid1d(L), 256, 0, stream,
-                           (cpx<double> *)D, t, L, 1.0 / double(L));
+                           (cpx<double> *)D, t, L, 1.0 / double(L), l1, l2);
     FM_CHECK_LAUNCH();
     FM_CHECK_HIP(hipStreamSynchronize(stream));
     return FM_SUCCESS;
@@ -571,7 +585,7 @@ extern "C" fm_status fm_fourier_plan_create(int64_t order, int precision, void *
             hipLaunchKernelGGL(gen_bluestein_kernel, grid1d(p->L), 256, 0, s, g, order, p->L);
             if (hipGetLastError() != hipSuccess) st = FM_ERR_HIP;
         }
-        if (!st) st = spectrum_from(g, logL, precision, p->D, s);
+        if (!st) st = spectrum_from(g, p->geo, precision, p->D, s);
         if (g) hipFree(g);
     }
     if (st) {
@@ -685,7 +699,7 @@ static fm_status conv_plan_create(int prec, int kind, int64_t m, int64_t n, int6
                            head, tail);
         if (hipGetLastError() != hipSuccess) st = FM_ERR_HIP;
     }
-    if (!st) st = spectrum_from(t, logL, prec, p->D, s);
+    if (!st) st = spectrum_from(t, p->geo, prec, p->D, s);
     if (t) hipFree(t);
     if (st) {
         fm_conv_plan_destroy(p);

@@ -130,9 +130,13 @@ __device__ __forceinline__ cpx<T> twiddle_mul(cpx<T> v, cpx<T> w) {
 // Stage plan of a length-2^LOGN transform for threads holding 2^LOGE
 // elements: S stages with radices 2^bits(s) <= 2^LOGE, bits as even as
 // possible, larger radices first.
-template <int LOGN, int LOGE> struct FftPlan {
+template <int LOGN, int LOGE, bool REV = false> struct FftPlan {
     static constexpr int S = LOGN == 0 ? 0 : (LOGN + LOGE - 1) / LOGE;
-    static constexpr int bits(int s) { return LOGN / S + (s < LOGN % S ? 1 : 0); }
+    static constexpr int fbits(int s) { return LOGN / S + (s < LOGN % S ? 1 : 0); }
+    // REV runs the same radices last-to-first (any order is a valid Stockham
+    // factorisation); a transform whose first radix equals the previous
+    // transform's last radix can start from that transform's registers.
+    static constexpr int bits(int s) { return REV ? fbits(S - 1 - s) : fbits(s); }
     static constexpr int logp(int s) {
         int t = 0;
         for (int q = 0; q < s; ++q) t += bits(q);
@@ -273,16 +277,16 @@ __device__ __forceinline__ void fft_stage2(cpx<T> *v, cpx<T> *lds, const cpx<T> 
 // 0 = leave natural-order outputs in LDS, 1 = leave the last stage's outputs
 // in v (butterfly-fastest mapping).
 template <typename T, int LOGN, int LOGE, int SIGN, int NT, int LOGB, int FIRST_SRC, int LAST_DST,
-          int s = 0>
+          bool REV = false, int s = 0>
 __device__ __forceinline__ void fft_all(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw,
                                         bool sf0) {
-    using P = FftPlan<LOGN, LOGE>;
+    using P = FftPlan<LOGN, LOGE, REV>;
     constexpr int E = 1 << LOGE;
     if constexpr (s < P::S) {
         constexpr int src = s == 0 ? FIRST_SRC : 1;
         constexpr int dst = (s == P::S - 1) ? (LAST_DST == 1 ? 1 : 0) : 0;
         fft_stage2<T, LOGN, P::bits(s), P::logp(s), SIGN, NT, E, LOGB, src, dst>(v, lds, tw, sf0);
-        fft_all<T, LOGN, LOGE, SIGN, NT, LOGB, FIRST_SRC, LAST_DST, s + 1>(v, lds, tw, false);
+        fft_all<T, LOGN, LOGE, SIGN, NT, LOGB, FIRST_SRC, LAST_DST, REV, s + 1>(v, lds, tw, false);
     }
 }
 
@@ -295,7 +299,9 @@ __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile
     constexpr int N = 1 << LOGN;
     using P = FftPlan<LOGN, LOGE>;
     constexpr int LOGR0 = P::S > 0 ? P::bits(0) : 0;
-    constexpr int LOGRL = P::S > 0 ? P::bits(P::S - 1) : 0;
+    // radix of the stage that produces the stored outputs (the inverse of a
+    // conv runs the radices reversed, so it ends with bits(0))
+    constexpr int LOGRL = P::S > 0 ? (CONV ? P::bits(0) : P::bits(P::S - 1)) : 0;
     constexpr int R0 = 1 << LOGR0, RL = 1 << LOGRL;
     // sequences per tile: NT * E / N (a multiple of NT*E ... >= 1 by construction)
     constexpr int LOGNT = NT == 64 ? 6 : NT == 128 ? 7 : NT == 256 ? 8 : NT == 512 ? 9 : 10;
@@ -356,24 +362,33 @@ __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile
         else fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 1>(v, lds, a.tw, cur_sf);
         if (P::S > 1) cur_sf = false;
     } else {
-        if (a.sign1 < 0) fft_all<T, LOGN, LOGE, -1, NT, LOGB, 0, 0>(v, lds, a.tw, cur_sf);
-        else fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 0>(v, lds, a.tw, cur_sf);
-        // spectrum multiply in LDS (natural order), then the inverse transform
+        // forward transform ends in registers (last radix RL); multiply by the
+        // spectrum there and run the inverse with the radices reversed, so its
+        // first stage (radix RL, p = 1) consumes exactly those registers
+        if (a.sign1 < 0) fft_all<T, LOGN, LOGE, -1, NT, LOGB, 0, 1>(v, lds, a.tw, cur_sf);
+        else fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 1>(v, lds, a.tw, cur_sf);
+        if (P::S > 1) cur_sf = false;
+        {
+            constexpr int RF = 1 << P::bits(P::S - 1);
+            constexpr int NBF = E / RF;
+            constexpr int LOGNRF = LOGN - P::bits(P::S - 1), NRF = 1 << LOGNRF;
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int idx = threadIdx.x + q * NT;
-            const int b = idx >> LOGN, n = idx & (N - 1);
-            const SeqPos sp = seq_pos(a, s0 + b);
-            if (sp.ok) {
-                const cpx<T> d = a.D[n * a.d_gn + sp.j * a.d_gj];
-                const cpx<T> u = lds[b * SB + lpad(n)];
-                lds[b * SB + lpad(n)] = a.conj_d ? cmulc(u, d) : cmul(u, d);
+            for (int e = 0; e < NBF; ++e) {
+                int b, i;
+                bfly_pos<LOGB, LOGNRF>(threadIdx.x + e * NT, cur_sf, b, i);
+                const SeqPos sp = seq_pos(a, s0 + b);
+                if (!sp.ok) continue;
+#pragma unroll
+                for (int r = 0; r < RF; ++r) {
+                    const int n = i + r * NRF;
+                    const cpx<T> d = a.D[n * a.d_gn + sp.j * a.d_gj];
+                    v[e * RF + r] = a.conj_d ? cmulc(v[e * RF + r], d) : cmul(v[e * RF + r], d);
+                }
             }
         }
-        __syncthreads();
-        if (a.sign1 < 0) fft_all<T, LOGN, LOGE, +1, NT, LOGB, 1, 1>(v, lds, a.tw, false);
-        else fft_all<T, LOGN, LOGE, -1, NT, LOGB, 1, 1>(v, lds, a.tw, false);
-        cur_sf = false;
+        if (a.sign1 < 0) fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 1, true>(v, lds, a.tw, cur_sf);
+        else fft_all<T, LOGN, LOGE, -1, NT, LOGB, 0, 1, true>(v, lds, a.tw, cur_sf);
+        if (P::S > 1) cur_sf = false;
     }
 
     // ---- last-stage outputs: v[e*RL + r] is element i + r*N/RL of sequence
