@@ -102,6 +102,8 @@ __global__ void scale_cast_kernel(cpx<T> *out, const cpx<double> *in, int64_t n,
 
 static inline unsigned grid1d(int64_t n, int bs = 256) { return unsigned(ceil_div(n, bs)); }
 
+static inline size_t esize(int prec) { return prec == FM_COMPLEX64 ? 8 : 16; }
+
 // ---------------------------------------------------------- twiddle cache
 namespace {
 std::mutex g_tw_mutex;
@@ -193,6 +195,7 @@ template <typename T> static fm_status normalize_pass(TileArgs<T> &a) {
     a.B = tile_batch<T>(a.logN);
     a.ntiles = ceil_div(a.O * a.J, a.B);
     a.logJ = is_pow2(a.J) ? ilog2(a.J) : -1;
+    if (a.o_fastest && (a.J % a.B) != 0) a.o_fastest = 0;
     return FM_SUCCESS;
 }
 
@@ -398,6 +401,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.tw = (const cpx<T> *)g.tw2;
         a.D = (const cpx<T> *)io.D;
         a.d_gn = 1; a.d_gj = N2; a.conj_d = io.conj_d;  // spectrum in row-pass order
+        a.o_fastest = 1;  // all columns of a row block reuse its spectrum rows
         a.twb_hi = (const cpx<T> *)g.twb_hi; a.twb_lo = (const cpx<T> *)g.twb_lo;
         a.twb_shift = g.twb_shift; a.twb_sign = +1;
         a.y = tmp;
@@ -427,12 +431,33 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
     }
 }
 
+// Columns per chunk of a multi-pass transform: chunks are sized so that one
+// chunk's intermediate (L x columns) fits the memory-side cache (MALL), which
+// then serves the second and third pass instead of HBM.
+static int64_t chunk_columns(const Geometry &g, int64_t k, size_t esz) {
+    if (g.passes == 1) return k;
+    const int64_t budget = int64_t(knob("FASTMAT_AMD_CHUNK_MB", 96)) << 20;
+    const int64_t per_col = (int64_t(1) << g.logL) * int64_t(esz);
+    int64_t c = budget / per_col;
+    if (c < 1) c = 1;
+    return c < k ? c : k;
+}
+
 template <typename T>
 static fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t stream) {
-    std::vector<PassSpec<T>> passes;
-    fm_status st = build_pipeline<T>(g, io, &passes);
-    for (size_t q = 0; !st && q < passes.size(); ++q) st = launch_tile<T>(passes[q].a, passes[q].conv, stream);
-    return st;
+    const int64_t cc = chunk_columns(g, io.k, sizeof(cpx<T>));
+    for (int64_t c0 = 0; c0 < io.k; c0 += cc) {
+        PipeIO sub = io;
+        sub.k = io.k - c0 < cc ? io.k - c0 : cc;
+        sub.x = (const cpx<T> *)io.x + c0 * io.ldx;
+        sub.y = (cpx<T> *)io.y + c0 * io.ldy;
+        std::vector<PassSpec<T>> passes;
+        fm_status st = build_pipeline<T>(g, sub, &passes);
+        for (size_t q = 0; !st && q < passes.size(); ++q)
+            st = launch_tile<T>(passes[q].a, passes[q].conv, stream);
+        if (st) return st;
+    }
+    return FM_SUCCESS;
 }
 
 static fm_status run_pipeline_any(const Geometry &g, const PipeIO &io, hipStream_t stream) {
@@ -523,7 +548,7 @@ static fm_status spectrum_from(cpx<double> *t, const Geometry &g, int prec, void
     return FM_SUCCESS;
 }
 
-static inline size_t esize(int prec) { return prec == FM_COMPLEX64 ? 8 : 16; }
+
 
 }  // namespace fm
 
@@ -609,7 +634,9 @@ extern "C" fm_status fm_fourier_workspace_size(const fm_fourier_plan *p, int64_t
                                                int64_t *bytes) {
     FM_REQUIRE(p != nullptr && bytes != nullptr, FM_ERR_VALUE, "NULL argument");
     FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
-    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L * k : 0;
+    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L *
+                                     chunk_columns(p->geo, k, esize(p->prec))
+                               : 0;
     return FM_SUCCESS;
 }
 
@@ -767,7 +794,9 @@ extern "C" fm_status fm_conv_plan_info(const fm_conv_plan *p, int64_t *rows, int
 extern "C" fm_status fm_conv_workspace_size(const fm_conv_plan *p, int64_t k, int64_t *bytes) {
     FM_REQUIRE(p != nullptr && bytes != nullptr, FM_ERR_VALUE, "NULL argument");
     FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
-    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L * k : 0;
+    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L *
+                                     chunk_columns(p->geo, k, esize(p->prec))
+                               : 0;
     return FM_SUCCESS;
 }
 

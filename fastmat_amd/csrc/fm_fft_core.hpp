@@ -1,23 +1,30 @@
-// FFT engine: LDS-resident batched Stockham FFT tiles and the tile kernel
-// used by every Fourier / Circulant / Toeplitz / Bluestein pass.
+// FFT engine: the tile body used by every Fourier / Circulant / Toeplitz /
+// Bluestein pass (per-operator kernels and the batched persistent kernels).
 //
 // Method (written from the published Stockham autosort formulation): a
-// length-N transform (N = 2^logN) is done as radix-8 stages plus one radix-2
-// or radix-4 stage.  Stage with radix R and sub-length p (1, R, R^2, ...):
-// butterfly i in [0, N/R) reads X[i + r N/R], multiplies input r by
-// W_{pR}^{r k} with k = i mod p, does an R-point DFT in registers and writes
-// Y[(i - k) R + k + r p].  Output comes out in natural order; all stages run
-// in place on one LDS buffer (read to registers, barrier, write, barrier).
+// length-N transform (N = 2^LOGN, compile-time) is split into S stages of
+// radix 2^bits(s) <= E, E = 32 (complex64) or 16 (complex128) elements per
+// thread.  Stage with radix R and sub-length p: butterfly i in [0, N/R) takes
+// X[i + r N/R], multiplies input r by W_{pR}^{r k} (k = i mod p), runs an
+// R-point DFT in registers and produces Y[(i - k) R + k + r p]; outputs come
+// out in natural order.  Stage 0 loads its inputs straight from HBM, the last
+// stage stores straight to HBM, and the S-1 exchanges in between go through a
+// padded LDS tile (one spare slot per 32 elements).  1024 = 32 x 32 needs one
+// exchange.
 //
-// Tile kernel: a tile is B consecutive sequences s = o*J + j of a (O x J)
-// sequence space.  Element n of sequence (o, j) is read at
-//     x[o*in_os + j*in_js + n*in_ns]
-// and written at y[o*out_os + j*out_js + n*out_ns]; this one addressing form
-// covers contiguous columns, four-step passes (strided and transposed sides)
-// and Kron mode axes.  Per-element work is fused around the transform:
+// Tile: B = 256 * E / N consecutive sequences s = o*J + j of an (O x J)
+// sequence space (512 threads for the largest N).  Element n of sequence
+// (o, j) is read at  x[o*in_os + j*in_js + n*in_ns]  and written at
+// y[o*out_os + j*out_js + n*out_ns]; this one addressing form covers
+// contiguous columns, the strided and the transposed sides of the four-step
+// passes, and Kron mode axes.  Global accesses map neighbouring lanes to
+// neighbouring sequences where those are adjacent in memory (strided sides)
+// and to neighbouring elements otherwise.  Work fused around the transform:
 //     load:  zero-pad (global index n*in_gn + j*in_gj >= n_valid), conj,
 //            pre-multiply by a chirp, then FFT(sign1)
-//     conv:  multiply by the spectrum D[n*d_gn + j*d_gj] (or conj), FFT(sign2)
+//     conv:  multiply by the spectrum D[n*d_gn + j*d_gj] (or conj) in
+//            registers, then FFT(-sign1) with the radices reversed so it
+//            starts from the forward transform's registers
 //     store: twiddle W_L^{+-j n} of an enclosing four-step, crop
 //            (n*out_gn + j*out_gj >= m_valid), post-multiply, conj, scale.
 #pragma once
@@ -181,6 +188,7 @@ template <typename T> struct TileArgs {
     T out_scale;
     int seqfast_out;
     int logJ;                       // log2(J) when J is a power of two, else -1
+    int o_fastest;                  // tile t covers (o = t mod O, j-block t / O)
 };
 
 // Sequence (o, j) of tile-local sequence b.
@@ -310,7 +318,10 @@ __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile
     constexpr int SB = N + (N >> 5) + 1;
     static_assert(LOGB >= 0, "tile must hold at least one sequence");
 
-    const int64_t s0 = tile << LOGB;
+    // first sequence of the tile: consecutive tiles step through j-blocks of
+    // one o, or (o_fastest) through the o's of one j-block, so that tiles
+    // sharing per-j data (a spectrum row) run close together
+    const int64_t s0 = a.o_fastest ? (tile % a.O) * a.J + (tile / a.O) * B : (tile << LOGB);
 
     cpx<T> v[E];
     // ---- stage-0 inputs straight from global: zero-pad, conj, pre-multiply

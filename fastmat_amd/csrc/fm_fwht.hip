@@ -10,12 +10,15 @@
 //     hence equal the reference bit for bit as well.
 //
 // Structure: the index bits are split into passes of consecutive bit ranges
-// [lo, hi), lowest bits first.  A pass transforms vectors of length 2^(hi-lo)
-// with element stride 2^lo, staged through LDS in tiles of B vectors; inside a
-// tile the stages run in groups of up to three bits, eight elements per
-// thread in registers, in place (the WHT does not reorder).  Order <= 14
-// (4-byte types) is one pass; 2^20 is two passes of 10 bits.
-#include "fm_common.hpp"
+// [lo, hi), lowest bits first (one pass up to 2^14 for 4-byte types, two
+// passes of 10 bits for 2^20).  A pass kernel is compiled per pass length; a
+// thread holds E = 32 / 16 / 8 elements (4 / 8 / 16-byte compute type) and
+// runs the pass as stages of at most log2(E) bits in registers, in place
+// (the WHT does not reorder), exchanging through a padded LDS tile between
+// stages.  Strided passes read their first stage straight from HBM with
+// neighbouring lanes on neighbouring vectors; contiguous passes load the
+// tile coalesced into LDS first.  The last stage stores straight to HBM.
+#include "fm_fft_core.hpp"  // lpad, shared helpers
 
 namespace fm {
 
@@ -42,138 +45,260 @@ template <typename S, typename C> __device__ __forceinline__ S to_s(C v) {
     else return v;
 }
 
+// Per-thread element count (power of two) by compute-type size.
+template <typename C> constexpr int wht_loge() {
+    return sizeof(C) <= 4 ? 5 : (sizeof(C) == 8 ? 4 : 3);
+}
+
 struct WhtArgs {
     int logN;          // vector length 2^logN for this pass
-    int B;             // vectors per tile
     int64_t O, J;      // vector space: v = o*J + j; element n at o*os + j + n*ns
+    int logJ;          // log2(J) if J is a power of two, else -1
     int64_t ntiles;
-    int64_t in_os, in_ns, out_os, out_ns;
-    int seqfast;
+    int64_t os, ns;
+    int seqfast;       // neighbouring vectors are adjacent in memory (J > 1)
     const void *x;
     void *y;
 };
 
-// In-place WHT of B vectors of length N in LDS (vector b at lds + b*S),
-// stages d = 1, 2, 4, ... in groups of up to three bits.
-template <typename C, int NT>
-__device__ void lds_wht(C *lds, int S, int logN, int B) {
-    int q = 0;
-    while (q < logN) {
-        const int g = (logN - q) >= 3 ? 3 : (logN - q);
-        const int G = 1 << g;
-        const int ngroups = (B << logN) >> g;   // butterfly groups in the tile
-        const int per_seq_log = logN - g;
-        for (int u = threadIdx.x; u < ngroups; u += NT) {
-            const int b = u >> per_seq_log;
-            const int w = u & ((1 << per_seq_log) - 1);
-            const int low = w & ((1 << q) - 1);
-            const int high = w >> q;
-            C *base = lds + b * S + ((high << (q + g)) | low);
-            C v[8];
+// Stage plan: S stages of at most LOGE bits, lowest bits first (the
+// reference's order, distance 1, 2, 4, ...).
+template <int LOGN, int LOGE> struct WhtPlan {
+    static constexpr int S = LOGN == 0 ? 0 : (LOGN + LOGE - 1) / LOGE;
+    static constexpr int bits(int s) { return LOGN / S + (s < LOGN % S ? 1 : 0); }
+    static constexpr int q(int s) {
+        int t = 0;
+        for (int k = 0; k < s; ++k) t += bits(k);
+        return t;
+    }
+};
+
+// Position of element t of butterfly group u for a stage on bits [Q, Q+R).
+template <int Q, int R> __device__ __forceinline__ int wht_pos(int u, int t) {
+    const int low = u & ((1 << Q) - 1);
+    const int high = u >> Q;
+    return (high << (Q + R)) | (t << Q) | low;
+}
+
+// In-register WHT over the R = 2^LR elements of a group, stages h = 1, 2, ...
+template <typename C, int LR> __device__ __forceinline__ void wht_regs(C *v) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
-                if (t < G) v[t] = base[t << q];
-            // stages in increasing distance, pairs (t, t ^ h)
+    for (int h = 1; h < (1 << LR); h <<= 1) {
 #pragma unroll
-            for (int h = 1; h < 8; h <<= 1) {
-                if (h < G) {
-#pragma unroll
-                    for (int t = 0; t < 8; ++t)
-                        if (t < G && !(t & h)) {
-                            C a = v[t], c = v[t | h];
-                            v[t] = wadd(a, c);
-                            v[t | h] = wsub(a, c);
-                        }
-                }
+        for (int t = 0; t < (1 << LR); ++t) {
+            if (!(t & h)) {
+                const C a = v[t], c = v[t | h];
+                v[t] = wadd(a, c);
+                v[t | h] = wsub(a, c);
             }
+        }
+    }
+}
+
+__device__ __forceinline__ void wht_seq(const WhtArgs &a, int64_t s, bool &ok, int64_t &base) {
+    ok = s < a.O * a.J;
+    int64_t o, j;
+    if (a.logJ >= 0) {
+        o = s >> a.logJ;
+        j = s & ((int64_t(1) << a.logJ) - 1);
+    } else {
+        o = s / a.J;
+        j = s - o * a.J;
+    }
+    base = o * a.os + j;
+}
+
+template <int LOGB, int LOGU>
+__device__ __forceinline__ void wht_map(int g, bool seqfast, int &b, int &u) {
+    if (seqfast) {
+        b = g & ((1 << LOGB) - 1);
+        u = g >> LOGB;
+    } else {
+        u = g & ((1 << LOGU) - 1);
+        b = g >> LOGU;
+    }
+}
+
+// One stage s of the plan on registers v.  SRC: 0 = v holds the inputs
+// (loaded in mapping sf), 1 = read from LDS.  DST: 0 = write back to LDS,
+// 1 = keep in v.
+template <typename C, int LOGN, int LOGE, int NT, int LOGB, int s, int SRC, int DST>
+__device__ __forceinline__ void wht_stage(C *v, C *lds, bool sf) {
+    using P = WhtPlan<LOGN, LOGE>;
+    constexpr int LR = P::bits(s), R = 1 << LR, Q = P::q(s);
+    constexpr int E = 1 << LOGE, NB = E / R, LOGU = LOGN - LR;
+    constexpr int SB = (1 << LOGN) + ((1 << LOGN) >> 5) + 1;
+    if constexpr (SRC == 1) sf = false;
+    if constexpr (SRC == 1) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
-                if (t < G) base[t << q] = v[t];
+        for (int e = 0; e < NB; ++e) {
+            int b, u;
+            wht_map<LOGB, LOGU>(threadIdx.x + e * NT, sf, b, u);
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[e * R + t] = lds[b * SB + lpad(wht_pos<Q, LR>(u, t))];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) wht_regs<C, LR>(v + e * R);
+    if constexpr (DST == 0) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            int b, u;
+            wht_map<LOGB, LOGU>(threadIdx.x + e * NT, sf, b, u);
+#pragma unroll
+            for (int t = 0; t < R; ++t) lds[b * SB + lpad(wht_pos<Q, LR>(u, t))] = v[e * R + t];
         }
         __syncthreads();
-        q += g;
     }
 }
 
-template <typename S, int NT>
-__global__ __launch_bounds__(NT) void wht_tile_kernel(const WhtArgs a) {
+template <typename C, int LOGN, int LOGE, int NT, int LOGB, int FIRST_SRC, int s = 0>
+__device__ __forceinline__ void wht_all(C *v, C *lds, bool sf0) {
+    using P = WhtPlan<LOGN, LOGE>;
+    if constexpr (s < P::S) {
+        constexpr int src = s == 0 ? FIRST_SRC : 1;
+        constexpr int dst = s == P::S - 1 ? 1 : 0;
+        wht_stage<C, LOGN, LOGE, NT, LOGB, s, src, dst>(v, lds, sf0);
+        wht_all<C, LOGN, LOGE, NT, LOGB, FIRST_SRC, s + 1>(v, lds, false);
+    }
+}
+
+template <typename S, int LOGN, int NT>
+__global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
     using C = typename WhtTraits<S>::C;
+    constexpr int LOGE = wht_loge<C>(), E = 1 << LOGE;
+    constexpr int N = 1 << LOGN;
+    using P = WhtPlan<LOGN, LOGE>;
+    constexpr int LOGNT = NT == 256 ? 8 : 9;
+    constexpr int LOGB = LOGNT + LOGE - LOGN;
+    constexpr int B = 1 << LOGB;
+    constexpr int SB = N + (N >> 5) + 1;
+    static_assert(LOGB >= 0, "tile must hold a vector");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *lds = reinterpret_cast<C *>(smem_raw);
-    const int N = 1 << a.logN;
-    const int S_ = N + (sizeof(C) >= 8 ? 1 : 1);
-    const int B = a.B;
-    int logB = 0;
-    while ((1 << logB) < B) ++logB;
-    const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t s0 = tile * B;
-    const int64_t nseq = a.O * a.J;
-    const S *x = reinterpret_cast<const S *>(a.x);
-    S *y = reinterpret_cast<S *>(a.y);
-    const int tot = B << a.logN;
-    for (int idx = threadIdx.x; idx < tot; idx += NT) {
-        int b, n;
-        if (a.seqfast) { b = idx & (B - 1); n = idx >> logB; }
-        else { n = idx & (N - 1); b = idx >> a.logN; }
-        const int64_t s = s0 + b;
-        if (s < nseq) {
-            const int64_t o = s / a.J, j = s - o * a.J;
-            lds[b * S_ + n] = to_c<S, C>(x[o * a.in_os + j + n * a.in_ns]);
+    const S *__restrict__ x = reinterpret_cast<const S *>(a.x);
+    S *__restrict__ y = reinterpret_cast<S *>(a.y);
+    const int64_t s0 = xcd_remap(blockIdx.x, gridDim.x) << LOGB;
+    const bool seqfast = a.seqfast && B > 1;
+    C v[E];
+    if (seqfast) {
+        // strided vectors: stage 0 straight from HBM, neighbouring lanes on
+        // neighbouring vectors (contiguous in memory)
+        constexpr int LR = P::bits(0), R = 1 << LR, NB = E / R, LOGU = LOGN - LR;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            int b, u;
+            wht_map<LOGB, LOGU>(threadIdx.x + e * NT, true, b, u);
+            bool ok;
+            int64_t base;
+            wht_seq(a, s0 + b, ok, base);
+#pragma unroll
+            for (int t = 0; t < R; ++t)
+                v[e * R + t] = ok ? to_c<S, C>(x[base + int64_t(wht_pos<0, LR>(u, t)) * a.ns]) : C{};
         }
+        wht_all<C, LOGN, LOGE, NT, LOGB, 0>(v, lds, true);
+    } else {
+        // contiguous vectors: coalesced natural-order load into LDS first
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int idx = threadIdx.x + e * NT;
+            const int b = idx >> LOGN, n = idx & (N - 1);
+            bool ok;
+            int64_t base;
+            wht_seq(a, s0 + b, ok, base);
+            v[e] = ok ? to_c<S, C>(x[base + int64_t(n) * a.ns]) : C{};
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int idx = threadIdx.x + e * NT;
+            lds[(idx >> LOGN) * SB + lpad(idx & (N - 1))] = v[e];
+        }
+        __syncthreads();
+        wht_all<C, LOGN, LOGE, NT, LOGB, 1>(v, lds, false);
     }
-    __syncthreads();
-    lds_wht<C, NT>(lds, S_, a.logN, B);
-    for (int idx = threadIdx.x; idx < tot; idx += NT) {
-        int b, n;
-        if (a.seqfast) { b = idx & (B - 1); n = idx >> logB; }
-        else { n = idx & (N - 1); b = idx >> a.logN; }
-        const int64_t s = s0 + b;
-        if (s < nseq) {
-            const int64_t o = s / a.J, j = s - o * a.J;
-            y[o * a.out_os + j + n * a.out_ns] = to_s<S, C>(lds[b * S_ + n]);
-        }
+    // last stage (bits [q, LOGN)): element t of group u sits at (t << q) | u,
+    // so u-fastest lanes store contiguous runs; strided vectors store
+    // sequence-fastest when the last stage is also the first
+    constexpr int LRL = P::bits(P::S - 1), RL = 1 << LRL, QL = P::q(P::S - 1);
+    constexpr int NBL = E / RL, LOGUL = LOGN - LRL;
+    const bool sfl = P::S == 1 ? seqfast : false;
+#pragma unroll
+    for (int e = 0; e < NBL; ++e) {
+        int b, u;
+        wht_map<LOGB, LOGUL>(threadIdx.x + e * NT, sfl, b, u);
+        bool ok;
+        int64_t base;
+        wht_seq(a, s0 + b, ok, base);
+        if (!ok) continue;
+#pragma unroll
+        for (int t = 0; t < RL; ++t)
+            y[base + int64_t(wht_pos<QL, LRL>(u, t)) * a.ns] = to_s<S, C>(v[e * RL + t]);
     }
 }
 
-template <typename S>
-static fm_status wht_launch(const WhtArgs &a, hipStream_t stream) {
+template <typename S, int LOGN>
+static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
     using C = typename WhtTraits<S>::C;
-    const int N = 1 << a.logN;
-    const size_t lds = size_t(a.B) * (N + 1) * sizeof(C);
-    FM_REQUIRE(lds <= 160 * 1024, FM_ERR_VALUE, "Hadamard tile needs %zu bytes of LDS", lds);
+    constexpr int LOGE = wht_loge<C>();
+    constexpr int NT = LOGN > LOGE + 8 ? 512 : 256;
+    constexpr int B = (NT << LOGE) >> LOGN;
+    constexpr int N = 1 << LOGN;
+    const size_t lds = size_t(B) * (N + (N >> 5) + 1) * sizeof(C);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)wht_tile_kernel<S, 256>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)wht_pass_kernel<S, LOGN, NT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
+    a.ntiles = ceil_div(a.O * a.J, B);
+    a.logJ = is_pow2(a.J) ? ilog2(a.J) : -1;
     FM_REQUIRE(a.ntiles > 0 && a.ntiles < (int64_t(1) << 31), FM_ERR_VALUE,
                "tile count out of range");
     ProfToken tok;
     prof_begin(stream, &tok);
-    hipLaunchKernelGGL((wht_tile_kernel<S, 256>), dim3(unsigned(a.ntiles)), dim3(256), lds,
+    hipLaunchKernelGGL((wht_pass_kernel<S, LOGN, NT>), dim3(unsigned(a.ntiles)), dim3(NT), lds,
                        stream, a);
     FM_CHECK_LAUNCH();
     if (tok.active) {
         const double elems = double(a.O) * double(a.J) * double(N);
         char name[64];
-        snprintf(name, sizeof(name), "wht_tile_kernel<%dB>", int(sizeof(S)));
-        prof_end(stream, &tok, name, 2.0 * elems * sizeof(S), elems * a.logN);
+        snprintf(name, sizeof(name), "wht_pass_kernel<%dB,%d>", int(sizeof(S)), LOGN);
+        prof_end(stream, &tok, name, 2.0 * elems * sizeof(S), elems * LOGN);
     }
     return FM_SUCCESS;
 }
 
-// Transform along bits [0, order) of vectors v[i] = x[(o*2^order + i)*inner + j].
-// Passes: bit ranges from the lowest, each at most `maxbits` wide.
+template <typename S> constexpr int wht_maxlog() {
+    return wht_loge<typename WhtTraits<S>::C>() + 9;
+}
+
+template <typename S> static fm_status wht_launch(WhtArgs a, hipStream_t stream) {
+    switch (a.logN) {
+#define FM_W(L)                                                                  \
+    case L:                                                                      \
+        if constexpr (L <= wht_maxlog<S>()) return wht_launch_t<S, L>(a, stream); \
+        break;
+        FM_W(1) FM_W(2) FM_W(3) FM_W(4) FM_W(5) FM_W(6) FM_W(7) FM_W(8) FM_W(9) FM_W(10)
+        FM_W(11) FM_W(12) FM_W(13) FM_W(14)
+#undef FM_W
+    default:
+        break;
+    }
+    set_error("Hadamard pass of 2^%d elements unsupported", a.logN);
+    return FM_ERR_VALUE;
+}
+
+// Transform along bits [0, order) of vectors v[i] = x[(o*2^order + i)*inner + j]:
+// passes over consecutive bit ranges from the lowest, each at most
+// wht_maxlog bits, split as evenly as possible.
 template <typename S>
 static fm_status wht_run(int order, const void *x, int64_t ldx, void *y, int64_t ldy,
                          int64_t k, int64_t inner, hipStream_t stream) {
-    using C = typename WhtTraits<S>::C;
-    // 64 KB tiles of the compute type
-    const int tile_elems = int(65536 / sizeof(C));
-    int maxbits = 0;
-    while ((2 << maxbits) <= tile_elems) ++maxbits;
-    const int npass = order == 0 ? 1 : (order + maxbits - 1) / maxbits;
+    (void)ldx;
+    (void)ldy;
+    const int maxbits = wht_maxlog<S>();
+    const int npass = (order + maxbits - 1) / maxbits;
     int lo = 0;
     for (int p = 0; p < npass; ++p) {
         const int bits = (order - lo) / (npass - p);
@@ -181,27 +306,15 @@ static fm_status wht_run(int order, const void *x, int64_t ldx, void *y, int64_t
         WhtArgs a;
         memset(&a, 0, sizeof(a));
         a.logN = bits;
-        // vector space: (col, high bits) x (low bits [0, lo) and inner)
         const int64_t Jlow = (int64_t(1) << lo) * inner;
+        // vector (col, high bits) x (low bits, inner); element n at stride Jlow
         a.O = k << (order - hi);
         a.J = Jlow;
-        // element n of vector (o, j): col = o >> (order-hi), hb = o & mask
-        // address = col*ld + ((hb << hi) + (n << lo)) * inner + j
-        // expressed with a single os requires ld == 2^order * inner for k > 1
-        a.in_os = (int64_t(1) << hi) * inner;
-        a.out_os = a.in_os;
-        a.in_ns = Jlow;
-        a.out_ns = Jlow;
-        int B = 1;
-        const int want = Jlow > 1 ? 16 : 8;
-        while (B < want && (int64_t(B * 2) << bits) <= tile_elems && B * 2 <= a.O * a.J) B *= 2;
-        a.B = B;
-        a.seqfast = (Jlow > 1 && B > 1) ? 1 : 0;
-        a.ntiles = ceil_div(a.O * a.J, B);
+        a.os = (int64_t(1) << hi) * inner;
+        a.ns = Jlow;
+        a.seqfast = Jlow > 1 ? 1 : 0;
         a.x = p == 0 ? x : y;
         a.y = y;
-        (void)ldx;
-        (void)ldy;
         fm_status st = wht_launch<S>(a, stream);
         if (st) return st;
         lo = hi;

@@ -37,6 +37,8 @@ struct GemmArgs {
 
 constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
 
+static fm_status gemm_launch_tiled(const GemmArgs &g, hipStream_t stream);
+
 __device__ __forceinline__ int64_t col_base(const GemmArgs &g, int64_t c, int64_t os, int64_t js,
                                             int *part) {
     int64_t cc = c;
@@ -139,7 +141,146 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(const GemmArgs g) {
     }
 }
 
+// Small factors (M, K <= 128, the dense factors of a Kron product): every
+// workgroup (8 wave64) keeps A in registers in the MFMA operand layout for
+// its whole life and walks 64-column tiles of the data persistently.  Wave w
+// owns rows [16w, 16w + 16); per k-step of 4 it issues four
+// v_mfma_f32_16x16x4f32 (one per 16-column block).  B tiles are staged
+// through LDS (coalesced along the data's contiguous axis), results through
+// LDS again so that stores are as contiguous as the layout allows.  Two
+// workgroups share a CU, so one streams while the other multiplies.
+constexpr int SK_N = 64;
+constexpr int SK_K = 128;
+constexpr int SK_T = 512;
+
+__global__ __launch_bounds__(SK_T) void gemm_smallk_kernel(const GemmArgs g) {
+    __shared__ float Bs[SK_K][SK_N + 1];  // B tile, then the output tile
+    __shared__ int xcol[SK_N];
+    __shared__ int ycol[SK_N];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int s = g.cplx ? 2 : 1;
+    const bool b_kfast = g.brs == 1;
+    const bool y_rfast = g.yrs == 1;
+    const int64_t ntiles = ceil_div(g.NC, SK_N);
+    // element offsets fit in 32 bits (checked by the launcher)
+    const int xk = int(s * g.brs), yr = int(s * g.yrs);
+
+    // a[st] = A_eff[16*wave + (lane & 15)][4*st + (lane >> 4)]; zero beyond M, K
+    float a[SK_K / 4];
+    {
+        const int64_t i = 16 * wave + (lane & 15);
+#pragma unroll
+        for (int st = 0; st < SK_K / 4; ++st) {
+            const int64_t k = 4 * st + (lane >> 4);
+            a[st] = (i < g.M && k < g.K) ? g.A[i * g.a_rs + k * g.a_cs] : 0.f;
+        }
+    }
+    constexpr int PER = SK_K * SK_N / SK_T;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (tid < SK_N) {
+            const int64_t gc = t * SK_N + tid;
+            int xb = -1, yb = -1;
+            if (gc < g.NC) {
+                int64_t cc = gc;
+                int part = 0;
+                if (g.cplx) {
+                    part = int(gc & 1);
+                    cc = gc >> 1;
+                }
+                const int64_t o = cc / g.J, j = cc - o * g.J;
+                xb = int(s * (o * g.bos + j * g.bjs) + part);
+                yb = int(s * (o * g.yos + j * g.yjs) + part);
+            }
+            xcol[tid] = xb;
+            ycol[tid] = yb;
+        }
+        __syncthreads();
+        {
+            float pre[PER];
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int idx = tid + e * SK_T;
+                int kk, c;
+                if (b_kfast) { kk = idx & (SK_K - 1); c = idx >> 7; }
+                else { c = idx & (SK_N - 1); kk = idx >> 6; }
+                const int xb = xcol[c];
+                pre[e] = (xb >= 0 && kk < g.K) ? g.X[xb + kk * xk] : 0.f;
+            }
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int idx = tid + e * SK_T;
+                int kk, c;
+                if (b_kfast) { kk = idx & (SK_K - 1); c = idx >> 7; }
+                else { c = idx & (SK_N - 1); kk = idx >> 6; }
+                Bs[kk][c] = pre[e];
+            }
+        }
+        __syncthreads();
+        f32x4 acc[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < SK_K / 4; ++st) {  // rows k >= K of A and B are zero
+            const int kr = 4 * st + (lane >> 4);
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[st], Bs[kr][16 * cb + (lane & 15)],
+                                                               acc[cb], 0, 0, 0);
+        }
+        __syncthreads();  // all waves done reading Bs
+        // stage the result: block cb holds rows 16*wave + 4*(lane>>4) + v, column lane&15
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                Bs[16 * wave + 4 * (lane >> 4) + v][16 * cb + (lane & 15)] = acc[cb][v];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + e * SK_T;
+            int r, c;
+            if (y_rfast) { r = idx & (SK_K - 1); c = idx >> 7; }
+            else { c = idx & (SK_N - 1); r = idx >> 6; }
+            const int yb = ycol[c];
+            if (yb >= 0 && r < g.M) g.Y[yb + r * yr] = Bs[r][c];
+        }
+        __syncthreads();  // Bs and the column tables are rewritten next iteration
+    }
+}
+
+// Largest element offset a strided column of the data can reach.
+static int64_t max_offset(const GemmArgs &g, int64_t os, int64_t rs, int64_t js, int64_t rows) {
+    const int64_t cols = g.cplx ? g.NC / 2 : g.NC;
+    const int64_t o_max = (cols - 1) / g.J, j_max = g.J - 1 < cols - 1 ? g.J - 1 : cols - 1;
+    return (g.cplx ? 2 : 1) * (o_max * os + j_max * js + (rows - 1) * rs) + 1;
+}
+
 static fm_status gemm_launch(const GemmArgs &g, hipStream_t stream) {
+    const bool small = g.M <= SK_K && g.K <= SK_K && g.NC > 0 &&
+                       max_offset(g, g.bos, g.brs, g.bjs, g.K) < (int64_t(1) << 31) &&
+                       max_offset(g, g.yos, g.yrs, g.yjs, g.M) < (int64_t(1) << 31);
+    if (small) {
+        const int64_t tiles = ceil_div(g.NC, SK_N);
+        if (tiles == 0) return FM_SUCCESS;
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        int64_t grid = int64_t(ncu) * 2;
+        if (grid > tiles) grid = tiles;
+        ProfToken tok;
+        prof_begin(stream, &tok);
+        hipLaunchKernelGGL(gemm_smallk_kernel, dim3(unsigned(grid)), dim3(SK_T), 0, stream, g);
+        FM_CHECK_LAUNCH();
+        if (tok.active) {
+            const double bytes = 4.0 * (double(g.M) * g.K + double(g.K) * g.NC + double(g.M) * g.NC);
+            prof_end(stream, &tok, "gemm_smallk_kernel", bytes, 2.0 * double(g.M) * g.K * g.NC);
+        }
+        return FM_SUCCESS;
+    }
+    return gemm_launch_tiled(g, stream);
+}
+
+static fm_status gemm_launch_tiled(const GemmArgs &g, hipStream_t stream) {
     const int64_t tiles = ceil_div(g.M, GT_M) * ceil_div(g.NC, GT_N);
     FM_REQUIRE(tiles < (int64_t(1) << 31), FM_ERR_VALUE, "GEMM too large");
     if (tiles == 0) return FM_SUCCESS;

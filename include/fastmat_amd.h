@@ -197,7 +197,7 @@ fm_status fm_dense_apply_strided(int a_dtype, int x_dtype, int adjoint,
 /* ---------------------------------------------------------- elementwise
  * All element types; operands share `dtype`. */
 /* y[:,j] = d (.) x[:,j]  (conj_d: conj(d)); Diag._forwardC/_backwardC
- * (Diag.pyx:149-167).  Mode-wise when inner > 1 (Kron). */
+ * (Diag.pyx:149-167). */
 fm_status fm_diag_apply(int dtype, int conj_d, int64_t n, const void *d,
                         const void *x, int64_t ldx, void *y, int64_t ldy,
                         int64_t k, void *stream);

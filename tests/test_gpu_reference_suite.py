@@ -1,0 +1,198 @@
+"""The reference's own randomized transform tests, replayed on the GPU path.
+
+Mirrors inspect/test.py:318-368 (forward vs reference() @ x, backward vs
+reference().T.conj() @ x, input not modified) with each class's _getTest()
+parameters (Fourier.pyx:249-270, Circulant.pyx:428-467, Toeplitz.pyx:737-800,
+Hadamard.pyx:251-267, Kron.pyx:356-385, Product.pyx:279-317,
+Sum.pyx:189-219, Partial.pyx:306-362, Permutation.pyx:132-145): inputs drawn
+like arrTestDist (inspect/common.py:369-422), columns {1, 5}
+(inspect/test.py:969), all 8 dtypes (:970), F / C / strided layouts (:973),
+acceptance by compareResults (inspect/test.py:104-202: max-norm relative
+error within 5*2*eps*(2*sqrt(maxDim))**TOL_POWER, and the result dtype equal
+to promote(instance dtype, input dtype)).  The dense matrices come from the
+oracle (oracle/dense.py), not from the classes' own reference().
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import accept, dense
+
+pytestmark = pytest.mark.gpu
+
+ALLTYPES = [np.int8, np.int16, np.int32, np.int64, np.float32, np.float64, np.complex64,
+            np.complex128]
+FEWTYPES = [np.int32, np.float32, np.complex64]
+RNG = np.random.default_rng(20261202 + 99)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def test_dist(shape, dtype, rng=RNG):
+    """arrTestDist: ints from {-2,-1,1,2}, floats |x| in [1,2] with random
+    sign, the largest element bumped (inspect/common.py:369-422)."""
+    dtype = np.dtype(dtype)
+
+    def draw(s):
+        return rng.uniform(1.0, 2.0, size=s) * rng.choice([-1, 1], s)
+
+    if np.issubdtype(dtype, np.integer):
+        r = rng.choice([-2, -1, 1, 2], shape).astype(dtype)
+    elif np.issubdtype(dtype, np.complexfloating):
+        r = (draw(shape) + 1j * draw(shape)).astype(dtype)
+    else:
+        r = draw(shape).astype(dtype)
+    idx = np.unravel_index(np.abs(r).argmax(), r.shape)
+    if np.issubdtype(dtype, np.integer):
+        r[idx] += np.sign(r[idx])
+    else:
+        r[idx] *= 1.5
+    return r
+
+
+test_dist.__test__ = False  # not a pytest test
+
+
+def layouts(x):
+    yield "F", np.asfortranarray(x)
+    yield "C", np.ascontiguousarray(x)
+    big = np.zeros((x.shape[0], 3 * x.shape[1]), dtype=x.dtype)
+    big[:, ::3] = x
+    yield "strided", big[:, ::3]
+
+
+def check_transforms(fm_mat, ref, tol_power, min_eps=0.0, name=""):
+    """Forward and backward of fm_mat against the dense `ref` for every input
+    dtype, column count and layout; returns the number of queries run."""
+    from fastmat_amd import _device as D
+    n_rows, n_cols = ref.shape
+    ref64 = ref.astype(np.complex128)
+    queries = 0
+    for dt in ALLTYPES:
+        for k in (1, 5):
+            for adjoint in (False, True):
+                n_in = n_rows if adjoint else n_cols
+                x0 = test_dist((n_in, k), dt)
+                for lname, x in layouts(x0):
+                    if k == 1 and lname != "F":
+                        continue
+                    keep = x.copy()
+                    out = fm_mat.backward(x) if adjoint else fm_mat.forward(x)
+                    assert np.array_equal(x, keep), "input modified"
+                    expect = (ref64.conj().T if adjoint else ref64) @ x0.astype(np.complex128)
+                    exp_dtype = np.promote_types(
+                        np.promote_types(fm_mat.dtype, np.int8), x0.dtype)
+                    assert out.dtype == exp_dtype, (name, dt, out.dtype, exp_dtype)
+                    max_dim = max(ref.shape + fm_mat.shape)
+                    eps_types = [x0.dtype, np.promote_types(np.int8, fm_mat.dtype),
+                                 np.promote_types(np.int8, ref.dtype)]
+                    ok, err, tol = accept.passes(out, expect, max_dim, tol_power, eps_types,
+                                                 min_eps)
+                    assert ok, (name, np.dtype(dt).name, k, lname, adjoint, err, tol)
+                    queries += 1
+    return queries
+
+
+@pytest.fixture(scope="module")
+def fm():
+    import fastmat_amd
+    from fastmat_amd import _lib
+    _lib.load()
+    return fastmat_amd
+
+
+@pytest.mark.parametrize("n", [35, 127])
+@pytest.mark.parametrize("optimize", [False, True])
+def test_fourier(fm, n, optimize):
+    say("reference suite Fourier", n, optimize)
+    F = fm.Fourier(n, optimize=optimize)
+    check_transforms(F, dense.fourier(n), 3.0, name="Fourier")
+
+
+@pytest.mark.parametrize("n", [7, 41])
+@pytest.mark.parametrize("mtype", FEWTYPES)
+def test_circulant(fm, n, mtype):
+    say("reference suite Circulant", n, np.dtype(mtype).name)
+    c = test_dist((n,), mtype)
+    C = fm.Circulant(c)
+    check_transforms(C, dense.circulant(c), 2.0, float(np.finfo(np.float64).eps),
+                     name="Circulant")
+
+
+@pytest.mark.parametrize("ncols", [3, 5, 41])
+@pytest.mark.parametrize("th", FEWTYPES)
+@pytest.mark.parametrize("tv", FEWTYPES)
+def test_toeplitz(fm, ncols, th, tv):
+    say("reference suite Toeplitz", 4, ncols, np.dtype(th).name, np.dtype(tv).name)
+    vecH = test_dist((4,), th)
+    vecV = test_dist((ncols - 1,), tv)
+    T = fm.Toeplitz(vecH, vecV)
+    check_transforms(T, dense.toeplitz(vecH, vecV), 2.0, float(np.finfo(np.float64).eps),
+                     name="Toeplitz")
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_hadamard(fm, order):
+    say("reference suite Hadamard", order)
+    H = fm.Hadamard(order)
+    check_transforms(H, dense.hadamard(order, np.int8), 1.0, name="Hadamard")
+
+
+def test_permutation(fm):
+    say("reference suite Permutation")
+    sigma = RNG.permutation(35)
+    P = fm.Permutation(sigma)
+    check_transforms(P, dense.permutation(sigma), 1.0, name="Permutation")
+
+
+@pytest.mark.parametrize("t1", ALLTYPES)
+@pytest.mark.parametrize("t2", ALLTYPES)
+def test_kron_dense_factors(fm, t1, t2):
+    say("reference suite Kron", np.dtype(t1).name, np.dtype(t2).name)
+    a1, a2, a3 = test_dist((5, 5), t1), test_dist((4, 4), t2), test_dist((3, 3), t1)
+    K = fm.Kron(fm.Matrix(a1), fm.Matrix(a2), fm.Matrix(a3))
+    check_transforms(K, dense.kron(a1, a2, a3), 4.0, name="Kron")
+
+
+@pytest.mark.parametrize("ncols", [7, 5])
+@pytest.mark.parametrize("t1", ALLTYPES)
+@pytest.mark.parametrize("t2", FEWTYPES)
+def test_product(fm, ncols, t1, t2):
+    say("reference suite Product", ncols, np.dtype(t1).name, np.dtype(t2).name)
+    for st in FEWTYPES:
+        s = test_dist((1,), st)[0]
+        a1, a2, a3 = test_dist((5, ncols), t1), test_dist((ncols, 5), t2), \
+            test_dist((5, ncols), t1)
+        P = fm.Product(s, fm.Matrix(a1), fm.Matrix(a2), fm.Matrix(a3))
+        ref = dense.product([a1, a2, a3], scalar=s)
+        check_transforms(P, ref, 3.0, name="Product")
+
+
+@pytest.mark.parametrize("ncols", [17, 25])
+@pytest.mark.parametrize("t1", ALLTYPES)
+@pytest.mark.parametrize("t2", FEWTYPES)
+def test_sum(fm, ncols, t1, t2):
+    say("reference suite Sum", ncols, np.dtype(t1).name, np.dtype(t2).name)
+    a1, a2, a3 = (test_dist((25, ncols), t1), test_dist((25, ncols), t2),
+                  test_dist((25, ncols), t1))
+    S = fm.Sum(fm.Matrix(a1), fm.Matrix(a2), fm.Matrix(a3))
+    check_transforms(S, dense.sum_([a1, a2, a3]), 3.0, name="Sum")
+
+
+SUBCOLS = [np.array([1, 2, 3, 11, 12]), np.array([6]),
+           np.array([1, 1, 0, 0, 1, 0, 0, 1, 0, 1, 1, 0, 0, 1, 0, 1], dtype=bool)]
+SUBROWS = [np.array([7, 8, 9, 13]), np.array([10]),
+           np.array([0, 1, 1, 0, 0, 1, 0, 1, 1, 1, 0, 0, 1, 0, 0, 1], dtype=bool)]
+
+
+@pytest.mark.parametrize("ci", range(3))
+@pytest.mark.parametrize("ri", range(3))
+def test_partial(fm, ci, ri):
+    say("reference suite Partial", ci, ri)
+    P = fm.Partial(fm.Hadamard(4), rows=SUBROWS[ri], cols=SUBCOLS[ci])
+    ref = dense.partial(dense.hadamard(4, np.int8), SUBROWS[ri], SUBCOLS[ci])
+    check_transforms(P, ref, 1.0, name="Partial")
