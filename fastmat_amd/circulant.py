@@ -38,6 +38,8 @@ class _ConvBase(Matrix):
         self._gen128 = gen.astype(np.complex128)
         self._plans = _PlanCache(self._make_plan, "fm_conv_plan_destroy")
         self._content_cache = None
+        self._multi = None
+        self._multi_chains = {}
         return dtype
 
     def __del__(self):
@@ -63,6 +65,10 @@ class _ConvBase(Matrix):
 
     def _run(self, X, adjoint):
         cdt = self.compute_dtype(X.dtype)
+        if self._multi is not None:
+            if D.np_dtype(X.dtype) != cdt:
+                X = X.to(D.torch_dtype(cdt))
+            return self._multilevel_chain(cdt)._apply(X, adjoint)
         if D.np_dtype(X.dtype) != cdt:
             X = X.to(D.torch_dtype(cdt))
         h = self._plans.get(precision_of(cdt))
@@ -76,13 +82,18 @@ class _ConvBase(Matrix):
                   None if ws is None else _lib.ptr(ws), need.value, D.stream())
         return Y
 
+    def _getArray(self):
+        # the dense array is the closed form (Circulant.pyx:241-242,
+        # Toeplitz.pyx:325-326)
+        return self._reference()
+
     def _fwd(self, X):
         return self._run(X, False)
 
     def _bwd(self, X):
         return self._run(X, True)
 
-    def generator_grad(self, X, G):
+    def _generator_grad_1d(self, X, G):
         """Gradient of a real loss w.r.t. the generator (c, or [vecC, vecR]),
         given the forward input block X (k, cols) and dLoss/dY block G
         (k, rows), PyTorch convention.  Returns a device tensor."""
@@ -99,7 +110,37 @@ class _ConvBase(Matrix):
                   k, _lib.ptr(out), _lib.ptr(ws), need.value, D.stream())
         return out
 
+    def _multilevel_chain(self, cdt):
+        """Partial(Product(K.H, Diag(fftn(t')/prod L), K)), K = Kron(Fourier(L_i)),
+        for a multi-level generator (Circulant.pyx:138-215,
+        Toeplitz.pyx:261-316), with the spectrum rounded once to the compute
+        precision.  The per-level lengths L_i are powers of two."""
+        cdt = np.dtype(cdt)
+        chain = self._multi_chains.get(cdt)
+        if chain is None:
+            from .diag import Diag
+            from .fourier import Fourier
+            from .kron import Kron
+            from .partial import Partial
+            from .product import Product
+            Ls, t, rows, cols = self._multi
+            Lt = int(np.prod(Ls))
+            spec = (np.fft.fftn(t) / Lt).reshape(-1).astype(cdt)
+            K = Kron(*[Fourier(L, dtype=cdt) for L in Ls])
+            P = Product(K.H, Diag(spec), K)
+            chain = Partial(P, rows=None if len(rows) == Lt else rows,
+                            cols=None if len(cols) == Lt else cols)
+            self._multi_chains[cdt] = chain
+        return chain
+
+    def generator_grad(self, X, G):
+        if self._multi is not None:
+            raise NotImplementedError("generator gradient of a multi-level matrix")
+        return self._generator_grad_1d(X, G)
+
     def _strided_ok(self, dtype):
+        if self._multi is not None:
+            return False
         dt = D.np_dtype(dtype)
         n = self.numRows
         limit = 8192 if dt == np.complex64 else 4096
@@ -126,14 +167,26 @@ class Circulant(_ConvBase):
         c = D.squeeze1d(tenC)
         if tenC.ndim < 1 or c.ndim < 1:
             raise ValueError("Column-definition tensor must be at least 1D.")
-        if c.ndim > 1:
-            raise NotImplementedError("multi-level Circulant is not supported")
         self._optimize = options.pop("optimize", True)
         self._maxStage = options.pop("maxStage", 4)
         dtype = self._setup(c, options.pop("dtype", None), options)
         self._tenC = c
         n = c.size
         self._content = ()
+        if c.ndim > 1:
+            # per level: keep a power-of-two size, else embed into the next
+            # power of two >= 2 d - 1 as [t, 0 ... 0, t[1:]] along that axis
+            # (Circulant._preProcSlice, Circulant.pyx:243-280)
+            dims = c.shape
+            Ls = [d if (d & (d - 1)) == 0 else _next_pow2(2 * d - 1) for d in dims]
+            t = c.astype(np.complex128)
+            for ax, (d, L) in enumerate(zip(dims, Ls)):
+                if L != d:
+                    t = np.concatenate([t, np.zeros(t.shape[:ax] + (L - 2 * d + 1,) +
+                                                    t.shape[ax + 1:]),
+                                        np.take(t, np.arange(1, d), axis=ax)], axis=ax)
+            sel = _grid_selection(dims, Ls)
+            self._multi = (Ls, t, sel, sel)
         self._initProperties(n, n, dtype, **options)
 
     @property
@@ -142,6 +195,8 @@ class Circulant(_ConvBase):
 
     @property
     def fftLength(self):
+        if self._multi is not None:
+            return int(np.prod(self._multi[0]))
         n = self._tenC.size
         return n if (n & (n - 1)) == 0 else _next_pow2(2 * n - 1)
 
@@ -156,6 +211,8 @@ class Circulant(_ConvBase):
     def content(self):
         """The equivalent Partial(Product(F.H, Diag(fft(c')/L), F)) chain
         (Circulant.pyx:127-137, 217-221), GPU-backed."""
+        if self._multi is not None:
+            return (self._multilevel_chain(self.dtype),)
         if self._content_cache is None:
             from .diag import Diag
             from .fourier import Fourier
@@ -172,9 +229,7 @@ class Circulant(_ConvBase):
 
     def _reference(self):
         c = self._tenC.astype(self.dtype)
-        n = c.size
-        i = np.arange(n)
-        return c[np.mod(i[:, None] - i[None, :], n)]
+        return _multilevel_dense(c, c.shape, c.shape)
 
 
 class Toeplitz(_ConvBase):
@@ -184,7 +239,12 @@ class Toeplitz(_ConvBase):
         split = options.pop("split", None)
         arrSplit = np.array([] if split is None else split)
         if len(args) == 1:
-            tenT = np.asarray(args[0])
+            # the reference stores it in a typed ndarray slot, which rejects
+            # anything else with TypeError (Toeplitz.pyx:147-149)
+            if not isinstance(args[0], np.ndarray):
+                raise TypeError("Argument 'tenT' has incorrect type (expected "
+                                "numpy.ndarray, got %s)" % type(args[0]).__name__)
+            tenT = args[0]
         elif len(args) == 2:
             if not all(isinstance(a, np.ndarray) for a in args):
                 raise ValueError(
@@ -205,8 +265,6 @@ class Toeplitz(_ConvBase):
             raise ValueError(
                 "Invalid number of arguments to Toeplitz: Expecting exactly " +
                 "one or two fixed arguments")
-        if tenT.ndim != 1:
-            raise NotImplementedError("multi-level Toeplitz is not supported")
         arrDim = np.array(tenT.shape)
         arrSplit = np.atleast_1d(arrSplit)
         if arrSplit.size == 0:
@@ -222,13 +280,27 @@ class Toeplitz(_ConvBase):
             raise ValueError("Entry in split vector outside of defining tensor bounds")
         self._optimize = options.pop("optimize", True)
         self._maxStage = options.pop("maxStage", 4)
-        m = int(arrSplit[0])
-        n = int(arrDim[0]) - m + 1
         self._tenT = tenT
-        self._arrDimRows = np.array([m])
-        self._arrDimCols = np.array([n])
+        self._arrDimRows = np.array(arrSplit, dtype=np.int64)
+        self._arrDimCols = arrDim - self._arrDimRows + 1
+        m = int(np.prod(self._arrDimRows))
+        n = int(np.prod(self._arrDimCols))
         dtype = self._setup(tenT, options.pop("dtype", None), options)
         self._content = ()
+        if tenT.ndim > 1:
+            # per level: zeros inserted at the split point up to the next
+            # power of two (Toeplitz._preProcSlice, Toeplitz.pyx:331-369)
+            dims = tuple(int(d) for d in arrDim)
+            Ls = [_next_pow2(d) for d in dims]
+            t = tenT.astype(np.complex128)
+            for ax, (d, L, sp) in enumerate(zip(dims, Ls, self._arrDimRows)):
+                if L != d:
+                    t = np.concatenate([np.take(t, np.arange(sp), axis=ax),
+                                        np.zeros(t.shape[:ax] + (L - d,) + t.shape[ax + 1:]),
+                                        np.take(t, np.arange(sp, d), axis=ax)], axis=ax)
+            rows = _grid_selection(tuple(self._arrDimRows), Ls)
+            cols = _grid_selection(tuple(self._arrDimCols), Ls)
+            self._multi = (Ls, t, rows, cols)
         self._initProperties(m, n, dtype, **options)
 
     @property
@@ -237,14 +309,25 @@ class Toeplitz(_ConvBase):
 
     @property
     def vecC(self):
-        return self._tenT[:self.numRows]
+        """tenT up to the first level's split point; deprecated like
+        Toeplitz.pyx:63-69."""
+        import warnings
+        warnings.warn('vecC is deprecated.', FutureWarning)
+        return self._tenT[:self._arrDimRows[0]]
 
     @property
     def vecR(self):
-        return self._tenT[self.numRows:]
+        """The last cols-1 entries of tenT along the first level; deprecated
+        like Toeplitz.pyx:71-77 (same slice, so a one-column level returns
+        all of tenT there too)."""
+        import warnings
+        warnings.warn('vecR is deprecated.', FutureWarning)
+        return self._tenT[-int(self._arrDimCols[0]) + 1:]
 
     @property
     def fftLength(self):
+        if self._multi is not None:
+            return int(np.prod(self._multi[0]))
         return _next_pow2(self.numRows + self.numCols - 1)
 
     def _make_plan(self, prec):
@@ -255,8 +338,38 @@ class Toeplitz(_ConvBase):
         return h
 
     def _reference(self):
-        t = self._tenT.astype(self.dtype)
-        m, n = self.numRows, self.numCols
-        i = np.arange(m)[:, None]
-        j = np.arange(n)[None, :]
-        return t[np.mod(i - j, m + n - 1)]
+        return _multilevel_dense(self._tenT.astype(self.dtype), tuple(self._arrDimRows),
+                                 tuple(self._arrDimCols))
+
+
+def _grid_selection(dims, Ls):
+    """Flat indices (C order) of the grid prod(range(d_i)) inside the padded
+    grid prod(range(L_i)) (Circulant._genArrS, Circulant.pyx:292-347)."""
+    idx = np.zeros(1, dtype=np.int64)
+    for d, L in zip(dims, Ls):
+        idx = (idx[:, None] * L + np.arange(d)[None, :]).ravel()
+    return idx
+
+
+def _multilevel_dense(t, rows, cols):
+    """Dense multi-level Toeplitz/Circulant: entry ((i_1..i_d), (j_1..j_d)),
+    multi-indices in C order, is t[(i_1 - j_1) mod d_1, ..., (i_d - j_d) mod
+    d_d] with d_k = t.shape[k] (Circulant._refRecursion, Circulant.pyx:
+    344-412; Toeplitz._refRecursion, Toeplitz.pyx:628-719)."""
+    t = np.asarray(t)
+    rows = tuple(int(r) for r in np.atleast_1d(rows))
+    cols = tuple(int(c) for c in np.atleast_1d(cols))
+    if t.ndim == 1 and len(rows) == 1:
+        d = t.size
+        i = np.arange(rows[0])[:, None]
+        j = np.arange(cols[0])[None, :]
+        if d == rows[0] == cols[0]:
+            return t[np.mod(i - j, d)]
+        return t[np.mod(i - j, rows[0] + cols[0] - 1)]
+    nd = len(rows)
+    idx = np.meshgrid(*[np.arange(r) for r in rows], indexing="ij")
+    jdx = np.meshgrid(*[np.arange(c) for c in cols], indexing="ij")
+    I = [a.ravel()[:, None] for a in idx]
+    J = [b.ravel()[None, :] for b in jdx]
+    sub = tuple(np.mod(I[k] - J[k], t.shape[k]) for k in range(nd))
+    return t[sub]

@@ -57,6 +57,45 @@ def toeplitz(vecC, vecR):
     return toeplitz_tensor(np.hstack((vecC, vecR)), vecC.size)
 
 
+def circulant_multilevel(tenC):
+    """d-level circulant matrix of a d-dimensional tensor, built block by block
+    like Circulant._refRecursion (Circulant.pyx:349-412): level 0 holds
+    N_0 x N_0 blocks, block (r, c) is the (d-1)-level matrix of
+    tenC[(r - c) mod N_0]; a 1-D tensor gives ``circulant``."""
+    t = np.asarray(tenC)
+    if t.ndim == 1:
+        return circulant(t)
+    n0 = t.shape[0]
+    subs = [circulant_multilevel(t[p]) for p in range(n0)]
+    b = subs[0].shape[0]
+    out = np.zeros((n0 * b, n0 * b), dtype=subs[0].dtype)
+    for r in range(n0):
+        for c in range(n0):
+            out[r * b:(r + 1) * b, c * b:(c + 1) * b] = subs[(r - c) % n0]
+    return out
+
+
+def toeplitz_multilevel(tenT, split):
+    """d-level Toeplitz matrix of a d-dimensional tensor with per-level split
+    points (rows per level), built block by block like Toeplitz._refRecursion
+    (Toeplitz.pyx:637-719): level 0 has split[0] x (shape[0] - split[0] + 1)
+    blocks, block (r, c) is the (d-1)-level matrix of
+    tenT[(r - c) mod shape[0]]; a 1-D tensor gives ``toeplitz_tensor``."""
+    t = np.asarray(tenT)
+    split = [int(v) for v in np.atleast_1d(split)]
+    if t.ndim == 1:
+        return toeplitz_tensor(t, split[0])
+    d0, r0 = t.shape[0], split[0]
+    c0 = d0 - r0 + 1
+    subs = [toeplitz_multilevel(t[p], split[1:]) for p in range(d0)]
+    br, bc = subs[0].shape
+    out = np.zeros((r0 * br, c0 * bc), dtype=subs[0].dtype)
+    for r in range(r0):
+        for c in range(c0):
+            out[r * br:(r + 1) * br, c * bc:(c + 1) * bc] = subs[(r - c) % d0]
+    return out
+
+
 def hadamard(order, dtype=np.int64):
     """Sylvester Hadamard matrix H[i, j] = (-1)^popcount(i & j)
     (Hadamard.pyx:242-248 builds it with scipy.linalg.hadamard)."""

@@ -2,7 +2,8 @@
 
 Mirrors inspect/test.py:318-368 (forward vs reference() @ x, backward vs
 reference().T.conj() @ x, input not modified) with each class's _getTest()
-parameters (Fourier.pyx:249-270, Circulant.pyx:428-467, Toeplitz.pyx:737-800,
+parameters (Fourier.pyx:249-270, Circulant.pyx:428-467 including the
+multi-level shapes, Toeplitz.pyx:737-800 including the multi-level case,
 Hadamard.pyx:251-267, Kron.pyx:356-385, Product.pyx:279-317,
 Sum.pyx:189-219, Partial.pyx:306-362, Permutation.pyx:132-145): inputs drawn
 like arrTestDist (inspect/common.py:369-422), columns {1, 5}
@@ -113,13 +114,13 @@ def test_fourier(fm, n, optimize):
     check_transforms(F, dense.fourier(n), 3.0, name="Fourier")
 
 
-@pytest.mark.parametrize("n", [7, 41])
+@pytest.mark.parametrize("shape", [(7,), (41,), (3, 4), (3, 4, 5)])
 @pytest.mark.parametrize("mtype", FEWTYPES)
-def test_circulant(fm, n, mtype):
-    say("reference suite Circulant", n, np.dtype(mtype).name)
-    c = test_dist((n,), mtype)
+def test_circulant(fm, shape, mtype):
+    say("reference suite Circulant", shape, np.dtype(mtype).name)
+    c = test_dist(shape, mtype)
     C = fm.Circulant(c)
-    check_transforms(C, dense.circulant(c), 2.0, float(np.finfo(np.float64).eps),
+    check_transforms(C, dense.circulant_multilevel(c), 2.0, float(np.finfo(np.float64).eps),
                      name="Circulant")
 
 
@@ -133,6 +134,17 @@ def test_toeplitz(fm, ncols, th, tv):
     T = fm.Toeplitz(vecH, vecV)
     check_transforms(T, dense.toeplitz(vecH, vecV), 2.0, float(np.finfo(np.float64).eps),
                      name="Toeplitz")
+
+
+@pytest.mark.parametrize("optimize", [False, True])
+@pytest.mark.parametrize("mtype", FEWTYPES)
+def test_toeplitz_multilevel(fm, mtype, optimize):
+    say("reference suite Toeplitz multi-level", np.dtype(mtype).name, optimize)
+    shape, split = np.array([3, 3, 41]), np.array([2, 1, 4])
+    tenT = test_dist(tuple(shape), mtype)
+    T = fm.Toeplitz(tenT, split=split, optimize=optimize)
+    check_transforms(T, dense.toeplitz_multilevel(tenT, split), 2.0,
+                     float(np.finfo(np.float64).eps), name="Toeplitz ML")
 
 
 @pytest.mark.parametrize("order", [4, 6])

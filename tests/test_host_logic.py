@@ -1,0 +1,125 @@
+"""Host-side logic of the package, checked without a GPU.
+
+Covers the constructor behaviour the reference tests exactly
+(test/classes.py:275-320 for Toeplitz), the dense closed forms behind
+``reference()`` / ``array`` against the oracle, and the multi-level
+embedding (padding per level, row/column selection) evaluated in numpy FP64
+against the oracle's dense matrices.
+"""
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import dense
+
+RNG = np.random.default_rng(20261202 + 17)
+
+
+def crandn(*shape):
+    return RNG.standard_normal(shape) + 1j * RNG.standard_normal(shape)
+
+
+@pytest.fixture(scope="module")
+def fm():
+    import fastmat_amd
+    return fastmat_amd
+
+
+def embedding_dense(op):
+    """The multi-level chain Partial(Product(K^H, Diag(fftn(t')/L), K)) built
+    from the operator's embedding description, in numpy FP64."""
+    Ls, t, rows, cols = op._multi
+    K = np.ones((1, 1))
+    for L in Ls:
+        K = np.kron(K, np.fft.fft(np.eye(L), axis=0))
+    spec = np.fft.fftn(t).ravel() / np.prod(Ls)
+    full = K.conj().T @ np.diag(spec) @ K
+    return full[np.ix_(rows, cols)]
+
+
+def test_toeplitz_constructor_checks(fm):
+    # test/classes.py:275-320, dense comparisons through reference()
+    arrC = RNG.standard_normal((4, 10, 10))
+    arrR = RNG.standard_normal((6, 10, 10))
+    tenT = np.vstack([arrC, arrR])
+    split = arrC.shape[0]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", FutureWarning)
+        T1A = fm.Toeplitz(arrC[:, 0, 0], arrR[:, 0, 0])
+        np.testing.assert_array_equal(T1A.vecC, arrC[:, 0, 0])
+        np.testing.assert_array_equal(T1A.vecR, arrR[:, 0, 0])
+        with pytest.raises(ValueError):
+            fm.Toeplitz(arrC[:, 0, 0], arrR[:, 0, 0], split=5)
+        with pytest.raises(ValueError):
+            fm.Toeplitz(arrC[:, :, 0], arrR[:, :, 0])
+        with pytest.raises(ValueError):
+            fm.Toeplitz()
+        with pytest.raises(TypeError):
+            fm.Toeplitz('abc')
+        with pytest.raises(ValueError):
+            fm.Toeplitz('abc', 'def')
+        with pytest.raises(ValueError):
+            fm.Toeplitz('abc', 'def', 'ghi')
+        T1B = fm.Toeplitz(tenT[:, 0, 0], split=split)
+        np.testing.assert_array_equal(T1B.tenT, tenT[:, 0, 0])
+        np.testing.assert_array_equal(T1A[...], T1B[...])
+        np.testing.assert_array_equal(T1A[...], T1A.reference())
+        T2 = fm.Toeplitz(tenT[:, :, 0], split=2 * (split,))
+        np.testing.assert_array_equal(T2[...], T2.reference())
+        T3A = fm.Toeplitz(tenT, split=3 * (split,))
+        T3B = fm.Toeplitz(tenT[:9, :9, :9])
+        T3C = fm.Toeplitz(tenT, split=3 * (tenT.shape[0] - split,))
+        for T in (T3A, T3B, T3C):
+            np.testing.assert_array_equal(T[...], T.reference())
+        np.testing.assert_array_equal(T3A.vecC, arrC)
+        np.testing.assert_array_equal(T3A.vecR, arrR)
+        np.testing.assert_array_equal(T3A.tenT, tenT)
+        np.testing.assert_array_equal(T3B.tenT, tenT[:9, :9, :9])
+        np.testing.assert_array_equal(T3C.tenT, tenT)
+        with pytest.raises(ValueError):
+            fm.Toeplitz(tenT)
+        with pytest.raises(ValueError):
+            fm.Toeplitz(tenT, split=np.zeros((3, 3)))
+        with pytest.raises(ValueError):
+            fm.Toeplitz(tenT, split=4 * (split,))
+    # shapes of the multi-level cases
+    assert T3A.shape == (4 ** 3, 7 ** 3)
+    assert T3C.shape == (6 ** 3, 5 ** 3)
+    # the 3-level reference equals the oracle's block construction
+    np.testing.assert_allclose(T3B.reference(),
+                               dense.toeplitz_multilevel(tenT[:9, :9, :9], (5, 5, 5)),
+                               rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("shape", [(7,), (41,), (3, 4), (3, 4, 5), (8, 2)])
+def test_circulant_reference_and_embedding(fm, shape):
+    t = crandn(*shape)
+    C = fm.Circulant(t)
+    n = int(np.prod(shape))
+    assert C.shape == (n, n)
+    ref = dense.circulant_multilevel(t)
+    np.testing.assert_array_equal(C.reference(), ref)
+    if len(shape) > 1:
+        Ls = C._multi[0]
+        assert all((L & (L - 1)) == 0 for L in Ls)
+        np.testing.assert_allclose(embedding_dense(C), ref, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("shape,split", [((3, 3, 41), (2, 1, 4)), ((4, 5), (2, 3)),
+                                         ((9, 9, 9), (5, 5, 5)), ((8, 3), (8, 1))])
+def test_toeplitz_multilevel_reference_and_embedding(fm, shape, split):
+    t = crandn(*shape)
+    T = fm.Toeplitz(t, split=np.array(split))
+    ref = dense.toeplitz_multilevel(t, split)
+    assert T.shape == ref.shape
+    np.testing.assert_array_equal(T.reference(), ref)
+    np.testing.assert_allclose(embedding_dense(T), ref, atol=1e-12 * np.abs(ref).max())
+
+
+def test_multilevel_dtype_follows_generator(fm):
+    assert fm.Circulant(RNG.standard_normal((3, 4)).astype(np.float32)).dtype == np.complex64
+    assert fm.Circulant(RNG.standard_normal((3, 4))).dtype == np.complex128
+    for it in (np.int8, np.int16, np.int32):
+        T = fm.Toeplitz(np.ones((3, 5), it), split=np.array([2, 3]))
+        assert T.dtype == np.promote_types(it, np.complex64)

@@ -269,3 +269,82 @@ def test_acceptance_rule_rejects_planted_error():
     ok, err, _ = accept.passes(ref + 10 * tol * np.abs(ref).max(), ref, n, 3,
                                [np.complex64])
     assert not ok
+
+
+# ------------------------------------------------------ multi-level Circulant / Toeplitz
+def _shift_power(rows, cols, d, p):
+    """rows x cols 0/1 matrix with ones where (i - j) mod d == p."""
+    i = np.arange(rows)[:, None]
+    j = np.arange(cols)[None, :]
+    return (np.mod(i - j, d) == p).astype(float)
+
+
+@pytest.mark.parametrize("shape", [(3, 4), (3, 4, 5), (2, 1, 3)])
+def test_dense_circulant_multilevel_vs_kron_sum(shape):
+    # C = sum_p t[p] * kron_k S_{d_k}^{p_k}: an expansion over shift matrices,
+    # independent of the block placement the oracle restates
+    t = crandn(*shape)
+    expect = np.zeros((int(np.prod(shape)),) * 2, dtype=complex)
+    for p in np.ndindex(*shape):
+        term = np.ones((1, 1))
+        for d, pk in zip(shape, p):
+            term = np.kron(term, _shift_power(d, d, d, pk))
+        expect += t[p] * term
+    np.testing.assert_allclose(dense.circulant_multilevel(t), expect, atol=1e-13)
+
+
+@pytest.mark.parametrize("shape", [(3, 4), (4, 2, 5)])
+def test_dense_circulant_multilevel_diagonalised_by_kron_dft(shape):
+    # K C K^-1 = diag(fftn(t)) with K = kron of DFT matrices (C order)
+    t = crandn(*shape)
+    K = np.ones((1, 1))
+    for d in shape:
+        K = np.kron(K, np.fft.fft(np.eye(d), axis=0))
+    C = dense.circulant_multilevel(t)
+    lhs = K @ C @ np.linalg.inv(K)
+    np.testing.assert_allclose(lhs, np.diag(np.fft.fftn(t).ravel()), atol=1e-10)
+
+
+def test_dense_multilevel_reduce_to_one_level():
+    c = crandn(9)
+    np.testing.assert_array_equal(dense.circulant_multilevel(c), sla.circulant(c))
+    t = crandn(11)
+    np.testing.assert_array_equal(dense.toeplitz_multilevel(t, [4]),
+                                  dense.toeplitz_tensor(t, 4))
+    # a level of size 1 is a Kronecker factor of [t0]
+    t2 = crandn(1, 7)
+    np.testing.assert_array_equal(dense.toeplitz_multilevel(t2, [1, 3]),
+                                  dense.toeplitz_tensor(t2[0], 3))
+
+
+@pytest.mark.parametrize("shape,split", [((3, 3, 41), (2, 1, 4)), ((4, 5), (2, 3)),
+                                         ((5, 3), (5, 1))])
+def test_dense_toeplitz_multilevel_vs_kron_sum(shape, split):
+    t = crandn(*shape)
+    rows = [int(s) for s in split]
+    cols = [d - s + 1 for d, s in zip(shape, split)]
+    expect = np.zeros((int(np.prod(rows)), int(np.prod(cols))), dtype=complex)
+    for p in np.ndindex(*shape):
+        term = np.ones((1, 1))
+        for d, r, c, pk in zip(shape, rows, cols, p):
+            term = np.kron(term, _shift_power(r, c, d, pk))
+        expect += t[p] * term
+    np.testing.assert_allclose(dense.toeplitz_multilevel(t, split), expect, atol=1e-13)
+
+
+def test_dense_toeplitz_multilevel_embeds_in_circulant():
+    # a 2-level Toeplitz is the leading block of the 2-level circulant of its
+    # defining tensor with zeros inserted at the split points
+    t = crandn(4, 5)
+    split = (2, 3)
+    rows = [2, 3]
+    cols = [3, 3]
+    pad = np.zeros((8, 8), dtype=complex)
+    ii = np.r_[0:2, 8 - 2:8]
+    jj = np.r_[0:3, 8 - 2:8]
+    pad[np.ix_(ii, jj)] = t
+    C = dense.circulant_multilevel(pad)
+    r_idx = (np.arange(rows[0])[:, None] * 8 + np.arange(rows[1])[None, :]).ravel()
+    c_idx = (np.arange(cols[0])[:, None] * 8 + np.arange(cols[1])[None, :]).ravel()
+    np.testing.assert_allclose(C[np.ix_(r_idx, c_idx)], dense.toeplitz_multilevel(t, split),
+                               atol=1e-13)
