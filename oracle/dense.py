@@ -130,8 +130,12 @@ def kron(*refs):
 
 
 def product(refs, scalar=1):
-    """Product reference: scalar * R0 @ R1 @ ... (Product.pyx:243-257)."""
-    res = np.asarray(refs[-1]) * scalar
+    """Product reference: scalar * R0 @ R1 @ ... (Product.pyx:243-257); the
+    scalar is widened to at least double precision first, as the reference
+    widens it to promote(float64, dtype), so no step rounds to single."""
+    sc = np.asarray(scalar)
+    sc = sc.astype(np.promote_types(np.float64, sc.dtype))
+    res = np.asarray(refs[-1]) * sc[()]
     for r in refs[-2::-1]:
         res = np.asarray(r).dot(res)
     return res

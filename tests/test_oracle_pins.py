@@ -348,3 +348,39 @@ def test_dense_toeplitz_multilevel_embeds_in_circulant():
     c_idx = (np.arange(cols[0])[:, None] * 8 + np.arange(cols[1])[None, :]).ravel()
     np.testing.assert_allclose(C[np.ix_(r_idx, c_idx)], dense.toeplitz_multilevel(t, split),
                                atol=1e-13)
+
+
+# ------------------------------------------------------ Product / Sum / Partial
+def test_dense_product_entrywise_and_scalar_precision():
+    rng = np.random.default_rng(31)
+    a = rng.integers(-3, 4, (5, 7)).astype(np.int16)
+    b = rng.integers(-3, 4, (7, 5)).astype(np.int32)
+    c = rng.integers(-3, 4, (5, 7)).astype(np.int16)
+    s = np.complex64(1.3 - 0.7j)
+    P = dense.product([a, b, c], scalar=s)
+    # brute force: triple sum in exact integers, then one multiply by the
+    # scalar's exact complex128 value
+    exact = np.zeros((5, 7), dtype=np.int64)
+    for i in range(5):
+        for j in range(7):
+            exact[i, j] = sum(int(a[i, p]) * int(b[p, q]) * int(c[q, j])
+                              for p in range(7) for q in range(5))
+    expect = exact * np.complex128(s)
+    assert P.dtype == np.complex128
+    np.testing.assert_allclose(P, expect, rtol=1e-15, atol=1e-12)
+    # a scalar of 1 with one factor is that factor
+    np.testing.assert_array_equal(dense.product([a]), a)
+
+
+def test_dense_sum_and_partial():
+    rng = np.random.default_rng(32)
+    mats = [rng.integers(-5, 6, (6, 4)) for _ in range(3)]
+    S = dense.sum_(mats)
+    assert all(S[i, j] == mats[0][i, j] + mats[1][i, j] + mats[2][i, j]
+               for i in range(6) for j in range(4))
+    A = np.arange(48).reshape(6, 8)
+    rows, cols = np.array([4, 0, 5]), np.array([True, False, True, True, False, False, True, False])
+    Pa = dense.partial(A, rows=rows, cols=cols)
+    assert Pa.shape == (3, 4)
+    assert Pa[0, 0] == A[4, 0] and Pa[2, 3] == A[5, 6] and Pa[1, 1] == A[0, 2]
+    np.testing.assert_array_equal(dense.partial(A), A)
