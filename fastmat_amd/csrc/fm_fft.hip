@@ -289,10 +289,12 @@ static fm_status make_geometry(int prec, int logL, bool conv, hipStream_t stream
     g->tw2 = twiddle_table(prec, g->logN2, stream, &st);
     if (st) return st;
     g->twb_shift = (logL + 1) / 2;
-    // hi: W_L^{a 2^s} = W_{L/2^s}^a (full table), lo: W_L^b for b < 2^s
-    g->twb_hi = table(prec, logL - g->twb_shift, int64_t(1) << (logL - g->twb_shift), stream, &st);
+    // hi: W_L^{a 2^s} = W_{L/2^s}^a (full table), lo: W_L^b for b < 2^s; in
+    // FP64 for both precisions (the pass kernels form the twiddles in FP64)
+    g->twb_hi = table(FM_COMPLEX128, logL - g->twb_shift, int64_t(1) << (logL - g->twb_shift),
+                      stream, &st);
     if (st) return st;
-    g->twb_lo = table(prec, logL, int64_t(1) << g->twb_shift, stream, &st);
+    g->twb_lo = table(FM_COMPLEX128, logL, int64_t(1) << g->twb_shift, stream, &st);
     return st;
 }
 
@@ -362,7 +364,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.conj_in = io.conj_io;
         a.seqfast_in = a.B > 1;
         a.tw = (const cpx<T> *)g.tw1;
-        a.twb_hi = (const cpx<T> *)g.twb_hi; a.twb_lo = (const cpx<T> *)g.twb_lo;
+        a.twb_hi = (const cpx<double> *)g.twb_hi; a.twb_lo = (const cpx<double> *)g.twb_lo;
         a.twb_shift = g.twb_shift; a.twb_sign = -1;
         a.y = tmp;
         a.out_os = L; a.out_js = 1; a.out_ns = N2;
@@ -402,7 +404,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.D = (const cpx<T> *)io.D;
         a.d_gn = 1; a.d_gj = N2; a.conj_d = io.conj_d;  // spectrum in row-pass order
         a.o_fastest = 1;  // all columns of a row block reuse its spectrum rows
-        a.twb_hi = (const cpx<T> *)g.twb_hi; a.twb_lo = (const cpx<T> *)g.twb_lo;
+        a.twb_hi = (const cpx<double> *)g.twb_hi; a.twb_lo = (const cpx<double> *)g.twb_lo;
         a.twb_shift = g.twb_shift; a.twb_sign = +1;
         a.y = tmp;
         a.out_os = L; a.out_js = N2; a.out_ns = 1;

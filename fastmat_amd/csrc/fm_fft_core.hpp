@@ -176,7 +176,7 @@ template <typename T> struct TileArgs {
     int64_t d_gn, d_gj;
     int conj_d;
     // enclosing four-step twiddle applied before the store: W_L^{tw_sign * j * n}
-    const cpx<T> *twb_hi, *twb_lo;  // W_L^{a * 2^twb_shift}, W_L^{b}
+    const cpx<double> *twb_hi, *twb_lo;  // W_L^{a * 2^twb_shift}, W_L^{b} (FP64)
     int twb_shift;
     int twb_sign;
     // output
@@ -436,14 +436,22 @@ __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile
         if (!sp.ok) continue;
         cpx<T> w[RL];
         if (a.twb_hi || a.post) {
+            // four-step twiddle W_L^{j n} for n = i + r NRL: the FP64 base
+            // W_L^{j i} times successive FP64 powers of W_L^{j NRL} (two
+            // table lookups each, j n < L), rounded once to T
+            cpx<double> tw = mk(1.0, 0.0), step = mk(1.0, 0.0);
+            if (a.twb_hi) {
+                const int64_t e0 = sp.j * i, e1 = sp.j * NRL;
+                tw = cmul(a.twb_hi[e0 >> a.twb_shift], a.twb_lo[e0 & lo_mask]);
+                step = cmul(a.twb_hi[e1 >> a.twb_shift], a.twb_lo[e1 & lo_mask]);
+            }
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
                 const int n = i + r * NRL;
                 cpx<T> t = mk(T(1), T(0));
                 if (a.twb_hi) {
-                    const int64_t ex = sp.j * n;  // < L
-                    t = cmul(a.twb_hi[ex >> a.twb_shift], a.twb_lo[ex & lo_mask]);
-                    if (a.twb_sign > 0) t = conj(t);
+                    t = mk(T(tw.x), T(a.twb_sign > 0 ? -tw.y : tw.y));
+                    if (r + 1 < RL) tw = cmul(tw, step);
                 }
                 const int64_t gidx = n * a.out_gn + sp.j * a.out_gj;
                 if (a.post && gidx < a.m_valid) t = cmul(t, a.post[gidx]);
