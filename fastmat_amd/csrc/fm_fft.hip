@@ -151,6 +151,15 @@ const void *twiddle_table(int precision, int logn, hipStream_t stream, fm_status
 }
 
 // ------------------------------------------------------------ tile launch
+// Persistent software-pipelined pass kernels (FASTMAT_AMD_PIPE=0 turns them off).
+static bool pipe_enabled() {
+    static const bool on = [] {
+        const char *v = getenv("FASTMAT_AMD_PIPE");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 template <typename T, int LOGN, bool CONV>
 static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     constexpr int NT = (sizeof(T) == 4 ? LOGN >= 14 : LOGN >= 13) ? 512 : 256;
@@ -160,16 +169,36 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)fft_pass_kernel_pipe<T, LOGN, NT, CONV>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
     FM_REQUIRE(lds <= 160 * 1024, FM_ERR_VALUE, "tile of %d x %d needs %zu bytes of LDS", a.B, N,
                lds);
     FM_REQUIRE(a.ntiles > 0 && a.ntiles < (int64_t(1) << 31), FM_ERR_VALUE,
                "tile count %lld out of range", (long long)a.ntiles);
+    // persistent pipelined launch when there are several tiles per resident block
+    int64_t pgrid = 0;
+    if (pipe_enabled()) {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void *)fft_pass_kernel_pipe<T, LOGN, NT, CONV>, NT, lds) ==
+                hipSuccess &&
+            ncu > 0 && per_cu > 0) {
+            const int64_t g = (int64_t(ncu) * per_cu / kNumXcd) * kNumXcd;
+            if (g >= kNumXcd && a.ntiles >= 2 * g) pgrid = g;
+        }
+    }
     ProfToken tok;
     prof_begin(stream, &tok);
-    hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV>), dim3(unsigned(a.ntiles)), dim3(NT),
-                       lds, stream, a);
+    if (pgrid > 0)
+        hipLaunchKernelGGL((fft_pass_kernel_pipe<T, LOGN, NT, CONV>), dim3(unsigned(pgrid)),
+                           dim3(NT), lds, stream, a);
+    else
+        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV>), dim3(unsigned(a.ntiles)),
+                           dim3(NT), lds, stream, a);
     FM_CHECK_LAUNCH();
     if (tok.active) {
         const double span = double(a.J) * N;
@@ -276,7 +305,7 @@ static fm_status make_geometry(int prec, int logL, bool conv, hipStream_t stream
     fm_status st = FM_SUCCESS;
     if ((int64_t(1) << logL) <= single_max) {
         g->passes = 1;
-        g->tw_full = twiddle_table(prec, logL, stream, &st);
+        g->tw_full = twiddle_table(FM_COMPLEX128, logL, stream, &st);
         return st;
     }
     g->logN1 = logL / 2;
@@ -284,9 +313,9 @@ static fm_status make_geometry(int prec, int logL, bool conv, hipStream_t stream
     FM_REQUIRE((int64_t(1) << g->logN1) <= max_tile && (int64_t(1) << g->logN2) <= max_tile,
                FM_ERR_NOT_IMPLEMENTED, "FFT length 2^%d exceeds the two-level limit", logL);
     g->passes = conv ? 3 : 2;
-    g->tw1 = twiddle_table(prec, g->logN1, stream, &st);
+    g->tw1 = twiddle_table(FM_COMPLEX128, g->logN1, stream, &st);
     if (st) return st;
-    g->tw2 = twiddle_table(prec, g->logN2, stream, &st);
+    g->tw2 = twiddle_table(FM_COMPLEX128, g->logN2, stream, &st);
     if (st) return st;
     g->twb_shift = (logL + 1) / 2;
     // hi: W_L^{a 2^s} = W_{L/2^s}^a (full table), lo: W_L^b for b < 2^s; in
@@ -336,7 +365,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.in_gn = 1; a.in_gj = 0; a.n_valid = io.n_in;
         a.pre = (const cpx<T> *)io.pre;
         a.conj_in = io.conj_io;
-        a.tw = (const cpx<T> *)g.tw_full;
+        a.tw = (const cpx<double> *)g.tw_full;
         a.D = (const cpx<T> *)io.D;
         a.d_gn = 1; a.d_gj = 0; a.conj_d = io.conj_d;
         a.y = (cpx<T> *)io.y;
@@ -363,7 +392,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.pre = (const cpx<T> *)io.pre;
         a.conj_in = io.conj_io;
         a.seqfast_in = a.B > 1;
-        a.tw = (const cpx<T> *)g.tw1;
+        a.tw = (const cpx<double> *)g.tw1;
         a.twb_hi = (const cpx<double> *)g.twb_hi; a.twb_lo = (const cpx<double> *)g.twb_lo;
         a.twb_shift = g.twb_shift; a.twb_sign = -1;
         a.y = tmp;
@@ -380,7 +409,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.ntiles = ceil_div(io.k * N1, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = N2; a.in_ns = 1;
-        a.tw = (const cpx<T> *)g.tw2;
+        a.tw = (const cpx<double> *)g.tw2;
         a.y = (cpx<T> *)io.y;
         a.out_os = io.ldy; a.out_js = 1; a.out_ns = N1;
         a.out_gn = N1; a.out_gj = 1; a.m_valid = io.m_out;
@@ -400,7 +429,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.ntiles = ceil_div(io.k * N1, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = N2; a.in_ns = 1;
-        a.tw = (const cpx<T> *)g.tw2;
+        a.tw = (const cpx<double> *)g.tw2;
         a.D = (const cpx<T> *)io.D;
         a.d_gn = 1; a.d_gj = N2; a.conj_d = io.conj_d;  // spectrum in row-pass order
         a.o_fastest = 1;  // all columns of a row block reuse its spectrum rows
@@ -421,7 +450,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.in_os = L; a.in_js = 1; a.in_ns = N2;
         a.seqfast_in = a.B > 1;
         a.sign1 = +1;
-        a.tw = (const cpx<T> *)g.tw1;
+        a.tw = (const cpx<double> *)g.tw1;
         a.y = (cpx<T> *)io.y;
         a.out_os = io.ldy; a.out_js = 1; a.out_ns = N2;
         a.out_gn = N2; a.out_gj = 1; a.m_valid = io.m_out;
@@ -486,7 +515,7 @@ static fm_status run_strided(const Geometry &g, const void *x, void *y, int64_t 
     a.in_os = N * inner; a.in_js = 1; a.in_ns = inner;
     a.conj_in = conj_io;
     a.seqfast_in = (inner > 1 && a.B > 1);
-    a.tw = (const cpx<T> *)g.tw_full;
+    a.tw = (const cpx<double> *)g.tw_full;
     a.D = (const cpx<T> *)D;
     a.d_gn = 1; a.d_gj = 0; a.conj_d = conj_d;
     a.y = (cpx<T> *)y;

@@ -170,7 +170,7 @@ template <typename T> struct TileArgs {
     int seqfast_in;
     // transform
     int sign1;                      // -1 forward, +1 inverse
-    const cpx<T> *tw;               // W_N^e table
+    const cpx<double> *tw;          // W_N^e table (FP64)
     // spectral multiply + second transform (conv mode)
     const cpx<T> *D;
     int64_t d_gn, d_gj;
@@ -234,8 +234,8 @@ __device__ __forceinline__ void bfly_pos(int g, bool seqfast, int &b, int &i) {
 // 1 = read them from LDS.  DST: 0 = write outputs to LDS, 1 = leave them in v.
 template <typename T, int LOGN, int LOGR, int LOGP, int SIGN, int NT, int E, int LOGB, int SRC,
           int DST>
-__device__ __forceinline__ void fft_stage2(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw,
-                                           bool sf) {
+__device__ __forceinline__ void fft_stage2(cpx<T> *v, cpx<T> *lds,
+                                           const cpx<double> *__restrict__ tw, bool sf) {
     // sf: mapping of the butterflies held in v when SRC == 0 (see bfly_pos);
     // inputs read from LDS always use the butterfly-fastest mapping.
     if constexpr (SRC == 1) sf = false;
@@ -258,10 +258,16 @@ __device__ __forceinline__ void fft_stage2(cpx<T> *v, cpx<T> *lds, const cpx<T> 
         bfly_pos<LOGB, LOGNR>(threadIdx.x + e * NT, sf, b, i);
         if constexpr (LOGP > 0) {
             const int k = i & (P - 1);
-            // W_{pR}^{r k} = W_N^{r k N / (p R)}
+            // W_{pR}^{r k} = W_N^{r k N / (p R)}: one FP64 lookup of
+            // W_{pR}^k, its powers by FP64 products, rounded once to T
             const int step = k << (LOGN - LOGP - LOGR);
+            const cpx<double> w1 = tw[step];
+            cpx<double> w = w1;
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[e * R + r] = twiddle_mul<SIGN>(v[e * R + r], tw[r * step]);
+            for (int r = 1; r < R; ++r) {
+                v[e * R + r] = twiddle_mul<SIGN>(v[e * R + r], mk(T(w.x), T(w.y)));
+                if (r + 1 < R) w = cmul(w, w1);
+            }
         }
         dftr<LOGR, SIGN>(v + e * R);
     }
@@ -286,8 +292,8 @@ __device__ __forceinline__ void fft_stage2(cpx<T> *v, cpx<T> *lds, const cpx<T> 
 // in v (butterfly-fastest mapping).
 template <typename T, int LOGN, int LOGE, int SIGN, int NT, int LOGB, int FIRST_SRC, int LAST_DST,
           bool REV = false, int s = 0>
-__device__ __forceinline__ void fft_all(cpx<T> *v, cpx<T> *lds, const cpx<T> *__restrict__ tw,
-                                        bool sf0) {
+__device__ __forceinline__ void fft_all(cpx<T> *v, cpx<T> *lds,
+                                        const cpx<double> *__restrict__ tw, bool sf0) {
     using P = FftPlan<LOGN, LOGE, REV>;
     constexpr int E = 1 << LOGE;
     if constexpr (s < P::S) {
@@ -298,37 +304,82 @@ __device__ __forceinline__ void fft_all(cpx<T> *v, cpx<T> *lds, const cpx<T> *__
     }
 }
 
-// One tile of one pass: the body shared by the per-operator kernel and the
-// batched persistent kernel.  `lds` holds B * (N + N/32 + 1) elements.
-template <typename T, int LOGN, int NT, bool CONV>
-__device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile, cpx<T> *lds) {
-    constexpr int LOGE = sizeof(T) == 4 ? 5 : 4;
-    constexpr int E = 1 << LOGE;
-    constexpr int N = 1 << LOGN;
+// Compile-time shape of one tile of a length-2^LOGN pass with NT threads.
+template <typename T, int LOGN, int NT> struct TileShape {
+    static constexpr int LOGE = sizeof(T) == 4 ? 5 : 4;
+    static constexpr int E = 1 << LOGE;
+    static constexpr int N = 1 << LOGN;
     using P = FftPlan<LOGN, LOGE>;
-    constexpr int LOGR0 = P::S > 0 ? P::bits(0) : 0;
+    static constexpr int LOGR0 = P::S > 0 ? P::bits(0) : 0;
+    static constexpr int R0 = 1 << LOGR0;
+    static constexpr int LOGNT = NT == 64 ? 6 : NT == 128 ? 7 : NT == 256 ? 8 : NT == 512 ? 9 : 10;
+    // sequences per tile (NT * E / N, at least one)
+    static constexpr int LOGB = LOGNT + LOGE - LOGN;
+    static constexpr int B = 1 << LOGB;
+    static_assert(LOGB >= 0, "tile must hold at least one sequence");
+};
+
+// First sequence of a tile: consecutive tiles step through j-blocks of one o,
+// or (o_fastest) through the o's of one j-block, so that tiles sharing per-j
+// data (a spectrum row) run close together.
+template <typename T, int LOGB>
+__device__ __forceinline__ int64_t tile_first_seq(const TileArgs<T> &a, int64_t tile) {
+    return a.o_fastest ? (tile % a.O) * a.J + (tile / a.O) * (int64_t(1) << LOGB)
+                       : (tile << LOGB);
+}
+
+// Stage-0 inputs of one tile straight from global memory (zero outside the
+// valid range); only loads, so a caller can issue them ahead of compute.
+template <typename T, int LOGN, int NT>
+__device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile, cpx<T> *v) {
+    using S = TileShape<T, LOGN, NT>;
+    constexpr int NB = S::E / S::R0;
+    constexpr int LOGNR = LOGN - S::LOGR0, NR = 1 << LOGNR;
+    const int64_t s0 = tile_first_seq<T, S::LOGB>(a, tile);
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        int b, i;
+        bfly_pos<S::LOGB, LOGNR>(threadIdx.x + e * NT, a.seqfast_in, b, i);
+        const SeqPos sp = seq_pos(a, s0 + b);
+#pragma unroll
+        for (int r = 0; r < S::R0; ++r) {
+            const int n = i + r * NR;
+            const int64_t gidx = n * a.in_gn + sp.j * a.in_gj;
+            const bool ok = sp.ok && gidx < a.n_valid;
+            v[e * S::R0 + r] = ok ? a.x[sp.in_base + n * a.in_ns] : mk(T(0), T(0));
+        }
+    }
+}
+
+// Everything after the stage-0 loads of one tile: conj / pre-multiply, the
+// transform (conv: forward, spectrum, inverse), twiddle / post-multiply and
+// the stores.  `lds` holds B * (N + N/32 + 1) elements.
+template <typename T, int LOGN, int NT, bool CONV>
+__device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t tile, cpx<T> *v,
+                                                 cpx<T> *lds) {
+    using S = TileShape<T, LOGN, NT>;
+    constexpr int LOGE = S::LOGE;
+    constexpr int E = S::E;
+    constexpr int N = S::N;
+    using P = typename S::P;
+    constexpr int LOGR0 = S::LOGR0;
     // radix of the stage that produces the stored outputs (the inverse of a
     // conv runs the radices reversed, so it ends with bits(0))
     constexpr int LOGRL = P::S > 0 ? (CONV ? P::bits(0) : P::bits(P::S - 1)) : 0;
     constexpr int R0 = 1 << LOGR0, RL = 1 << LOGRL;
-    // sequences per tile: NT * E / N (a multiple of NT*E ... >= 1 by construction)
-    constexpr int LOGNT = NT == 64 ? 6 : NT == 128 ? 7 : NT == 256 ? 8 : NT == 512 ? 9 : 10;
-    constexpr int LOGB = LOGNT + LOGE - LOGN;
-    constexpr int B = 1 << LOGB;
+    constexpr int LOGB = S::LOGB;
+    constexpr int B = S::B;
     constexpr int SB = N + (N >> 5) + 1;
-    static_assert(LOGB >= 0, "tile must hold at least one sequence");
+    const int64_t s0 = tile_first_seq<T, LOGB>(a, tile);
 
-    // first sequence of the tile: consecutive tiles step through j-blocks of
-    // one o, or (o_fastest) through the o's of one j-block, so that tiles
-    // sharing per-j data (a spectrum row) run close together
-    const int64_t s0 = a.o_fastest ? (tile % a.O) * a.J + (tile / a.O) * B : (tile << LOGB);
-
-    cpx<T> v[E];
-    // ---- stage-0 inputs straight from global: zero-pad, conj, pre-multiply
-    {
+    // ---- conj, pre-multiply (zero-padded positions hold zeros already)
+    if (a.conj_in) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) v[q] = conj(v[q]);
+    }
+    if (a.pre) {
         constexpr int NB = E / R0;
         constexpr int LOGNR = LOGN - LOGR0, NR = 1 << LOGNR;
-        cpx<T> pv[E];
 #pragma unroll
         for (int e = 0; e < NB; ++e) {
             int b, i;
@@ -339,17 +390,8 @@ __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile
                 const int n = i + r * NR;
                 const int64_t gidx = n * a.in_gn + sp.j * a.in_gj;
                 const bool ok = sp.ok && gidx < a.n_valid;
-                v[e * R0 + r] = ok ? a.x[sp.in_base + n * a.in_ns] : mk(T(0), T(0));
-                if (a.pre) pv[e * R0 + r] = ok ? a.pre[gidx] : mk(T(0), T(0));
+                v[e * R0 + r] = cmul(v[e * R0 + r], ok ? a.pre[gidx] : mk(T(0), T(0)));
             }
-        }
-        if (a.conj_in) {
-#pragma unroll
-            for (int q = 0; q < E; ++q) v[q] = conj(v[q]);
-        }
-        if (a.pre) {
-#pragma unroll
-            for (int q = 0; q < E; ++q) v[q] = cmul(v[q], pv[q]);
         }
     }
     // mapping of the butterflies currently held in v (see bfly_pos)
@@ -469,6 +511,15 @@ __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile
             a.y[sp.out_base + n * a.out_ns] = scale(u, a.out_scale);
         }
     }
+}
+
+// One tile of one pass: the body shared by the per-operator kernels and the
+// batched persistent kernel.
+template <typename T, int LOGN, int NT, bool CONV>
+__device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile, cpx<T> *lds) {
+    cpx<T> v[TileShape<T, LOGN, NT>::E];
+    fft_tile_load<T, LOGN, NT>(a, tile, v);
+    fft_tile_compute<T, LOGN, NT, CONV>(a, tile, v, lds);
 }
 
 template <typename T, int LOGN, int NT, bool CONV>
