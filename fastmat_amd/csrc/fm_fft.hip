@@ -151,15 +151,6 @@ const void *twiddle_table(int precision, int logn, hipStream_t stream, fm_status
 }
 
 // ------------------------------------------------------------ tile launch
-// Persistent software-pipelined pass kernels (FASTMAT_AMD_PIPE=0 turns them off).
-static bool pipe_enabled() {
-    static const bool on = [] {
-        const char *v = getenv("FASTMAT_AMD_PIPE");
-        return !(v && v[0] == '0');
-    }();
-    return on;
-}
-
 template <typename T, int LOGN, bool CONV>
 static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     constexpr int NT = (sizeof(T) == 4 ? LOGN >= 14 : LOGN >= 13) ? 512 : 256;
@@ -169,36 +160,16 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)fft_pass_kernel_pipe<T, LOGN, NT, CONV>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
     FM_REQUIRE(lds <= 160 * 1024, FM_ERR_VALUE, "tile of %d x %d needs %zu bytes of LDS", a.B, N,
                lds);
     FM_REQUIRE(a.ntiles > 0 && a.ntiles < (int64_t(1) << 31), FM_ERR_VALUE,
                "tile count %lld out of range", (long long)a.ntiles);
-    // persistent pipelined launch when there are several tiles per resident block
-    int64_t pgrid = 0;
-    if (pipe_enabled()) {
-        int dev = 0, ncu = 0, per_cu = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void *)fft_pass_kernel_pipe<T, LOGN, NT, CONV>, NT, lds) ==
-                hipSuccess &&
-            ncu > 0 && per_cu > 0) {
-            const int64_t g = (int64_t(ncu) * per_cu / kNumXcd) * kNumXcd;
-            if (g >= kNumXcd && a.ntiles >= 2 * g) pgrid = g;
-        }
-    }
     ProfToken tok;
     prof_begin(stream, &tok);
-    if (pgrid > 0)
-        hipLaunchKernelGGL((fft_pass_kernel_pipe<T, LOGN, NT, CONV>), dim3(unsigned(pgrid)),
-                           dim3(NT), lds, stream, a);
-    else
-        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV>), dim3(unsigned(a.ntiles)),
-                           dim3(NT), lds, stream, a);
+    hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV>), dim3(unsigned(a.ntiles)), dim3(NT),
+                       lds, stream, a);
     FM_CHECK_LAUNCH();
     if (tok.active) {
         const double span = double(a.J) * N;
