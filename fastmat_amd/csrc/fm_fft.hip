@@ -433,12 +433,17 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
     }
 }
 
-// Columns per chunk of a multi-pass transform: chunks are sized so that one
-// chunk's intermediate (L x columns) fits the memory-side cache (MALL), which
-// then serves the second and third pass instead of HBM.
+// Columns per chunk of a multi-pass transform.  All columns go in one chunk
+// unless FASTMAT_AMD_CHUNK_MB sets a budget for one chunk's intermediate
+// (L x columns), e.g. to keep it in the memory-side cache (MALL).  Measured
+// on 2^20 x 64 and Bluestein 1,000,003 x 64 (complex64), a 96 MB budget was
+// 14% slower than one chunk: the smaller grids lose more to tails than the
+// cache saves.
 static int64_t chunk_columns(const Geometry &g, int64_t k, size_t esz) {
     if (g.passes == 1) return k;
-    const int64_t budget = int64_t(knob("FASTMAT_AMD_CHUNK_MB", 96)) << 20;
+    const int64_t mb = knob("FASTMAT_AMD_CHUNK_MB", 0);
+    if (mb <= 0) return k;
+    const int64_t budget = mb << 20;
     const int64_t per_col = (int64_t(1) << g.logL) * int64_t(esz);
     int64_t c = budget / per_col;
     if (c < 1) c = 1;
