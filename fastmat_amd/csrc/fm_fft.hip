@@ -151,6 +151,25 @@ const void *twiddle_table(int precision, int logn, hipStream_t stream, fm_status
 }
 
 // ------------------------------------------------------------ tile launch
+// Whether every element offset (x, y) and logical index (zero-pad / crop
+// bounds, pre / post / spectrum tables, twiddle exponents) of a pass stays
+// below 2^32 / sizeof(element), so the kernel may run 32-bit index math.
+template <typename T> static bool fits_idx32(const TileArgs<T> &a, int64_t N) {
+    const int64_t lim = int64_t((uint64_t(1) << 32) / sizeof(cpx<T>));
+    const int64_t O1 = a.O - 1, J1 = a.J - 1, N1 = N - 1;
+    const int64_t st[] = {a.in_os, a.in_js, a.in_ns, a.out_os, a.out_js, a.out_ns, a.in_gn,
+                          a.in_gj, a.out_gn, a.out_gj, a.d_gn, a.d_gj};
+    for (int64_t v : st)
+        if (v < 0 || v >= lim) return false;
+    auto below = [lim](double v) { return v < double(lim); };
+    return below(double(a.O) * double(a.J) + 512.0) &&
+           below(double(O1) * a.in_os + double(J1) * a.in_js + double(N1) * a.in_ns) &&
+           below(double(O1) * a.out_os + double(J1) * a.out_js + double(N1) * a.out_ns) &&
+           below(double(N1) * a.in_gn + double(J1) * a.in_gj) &&
+           below(double(N1) * a.out_gn + double(J1) * a.out_gj) &&
+           below(double(N1) * a.d_gn + double(J1) * a.d_gj) && below(double(a.J) * double(N));
+}
+
 template <typename T, int LOGN, bool CONV>
 static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     constexpr int NT = (sizeof(T) == 4 ? LOGN >= 14 : LOGN >= 13) ? 512 : 256;
@@ -158,7 +177,9 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     const size_t lds = size_t(a.B) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV>,
+        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
@@ -168,8 +189,12 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
                "tile count %lld out of range", (long long)a.ntiles);
     ProfToken tok;
     prof_begin(stream, &tok);
-    hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV>), dim3(unsigned(a.ntiles)), dim3(NT),
-                       lds, stream, a);
+    if (fits_idx32(a, N))
+        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, true>), dim3(unsigned(a.ntiles)),
+                           dim3(NT), lds, stream, a);
+    else
+        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, false>), dim3(unsigned(a.ntiles)),
+                           dim3(NT), lds, stream, a);
     FM_CHECK_LAUNCH();
     if (tok.active) {
         const double span = double(a.J) * N;

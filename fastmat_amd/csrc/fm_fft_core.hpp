@@ -29,6 +29,8 @@
 //            (n*out_gn + j*out_gj >= m_valid), post-multiply, conj, scale.
 #pragma once
 
+#include <type_traits>
+
 #include "fm_common.hpp"
 
 namespace fm {
@@ -191,26 +193,56 @@ template <typename T> struct TileArgs {
     int o_fastest;                  // tile t covers (o = t mod O, j-block t / O)
 };
 
+// Index arithmetic of a pass: int64_t in general, uint32_t when every
+// element offset and logical index of the launch is below 2^32 / sizeof
+// (32-bit offsets from a uniform base become saddr + voffset addressing).
+template <bool I32> using IdxT = typename std::conditional<I32, uint32_t, int64_t>::type;
+
+// Load / store element `idx` of a uniform base pointer.
+template <typename Ix, typename V>
+__device__ __forceinline__ V ldg(const V *base, Ix idx) {
+    if constexpr (sizeof(Ix) == 4)
+        return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) +
+                                            uint32_t(idx * uint32_t(sizeof(V))));
+    else
+        return base[idx];
+}
+template <typename Ix, typename V>
+__device__ __forceinline__ void stg(V *base, Ix idx, V val) {
+    if constexpr (sizeof(Ix) == 4)
+        *reinterpret_cast<V *>(reinterpret_cast<char *>(base) +
+                               uint32_t(idx * uint32_t(sizeof(V)))) = val;
+    else
+        base[idx] = val;
+}
+
+// Bound of a logical index in the launch's index type.
+template <typename Ix> __device__ __forceinline__ Ix idx_bound(int64_t v) {
+    if constexpr (sizeof(Ix) == 4) return v > int64_t(0xffffffffu) ? Ix(0xffffffffu) : Ix(v);
+    else return v;
+}
+
 // Sequence (o, j) of tile-local sequence b.
-struct SeqPos {
-    int64_t in_base, out_base, j;
+template <typename Ix> struct SeqPos {
+    Ix in_base, out_base, j;
     bool ok;
 };
 
-template <typename T>
-__device__ __forceinline__ SeqPos seq_pos(const TileArgs<T> &a, int64_t s) {
-    SeqPos p;
+template <typename Ix, typename T>
+__device__ __forceinline__ SeqPos<Ix> seq_pos(const TileArgs<T> &a, int64_t s) {
+    SeqPos<Ix> p;
     p.ok = s < a.O * a.J;
-    int64_t o, j;
+    const Ix si = Ix(s);
+    Ix o, j;
     if (a.logJ >= 0) {
-        o = s >> a.logJ;
-        j = s & ((int64_t(1) << a.logJ) - 1);
+        o = si >> a.logJ;
+        j = si & ((Ix(1) << a.logJ) - 1);
     } else {
-        o = s / a.J;
-        j = s - o * a.J;
+        o = si / Ix(a.J);
+        j = si - o * Ix(a.J);
     }
-    p.in_base = o * a.in_os + j * a.in_js;
-    p.out_base = o * a.out_os + j * a.out_js;
+    p.in_base = o * Ix(a.in_os) + j * Ix(a.in_js);
+    p.out_base = o * Ix(a.out_os) + j * Ix(a.out_js);
     p.j = j;
     return p;
 }
@@ -330,23 +362,26 @@ __device__ __forceinline__ int64_t tile_first_seq(const TileArgs<T> &a, int64_t 
 
 // Stage-0 inputs of one tile straight from global memory (zero outside the
 // valid range); only loads, so a caller can issue them ahead of compute.
-template <typename T, int LOGN, int NT>
+template <typename T, int LOGN, int NT, bool I32 = false>
 __device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile, cpx<T> *v) {
     using S = TileShape<T, LOGN, NT>;
+    using Ix = IdxT<I32>;
     constexpr int NB = S::E / S::R0;
     constexpr int LOGNR = LOGN - S::LOGR0, NR = 1 << LOGNR;
     const int64_t s0 = tile_first_seq<T, S::LOGB>(a, tile);
+    const Ix nv = idx_bound<Ix>(a.n_valid);
+    const Ix in_gn = Ix(a.in_gn), in_gj = Ix(a.in_gj), in_ns = Ix(a.in_ns);
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
         int b, i;
         bfly_pos<S::LOGB, LOGNR>(threadIdx.x + e * NT, a.seqfast_in, b, i);
-        const SeqPos sp = seq_pos(a, s0 + b);
+        const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
 #pragma unroll
         for (int r = 0; r < S::R0; ++r) {
-            const int n = i + r * NR;
-            const int64_t gidx = n * a.in_gn + sp.j * a.in_gj;
-            const bool ok = sp.ok && gidx < a.n_valid;
-            v[e * S::R0 + r] = ok ? a.x[sp.in_base + n * a.in_ns] : mk(T(0), T(0));
+            const Ix n = Ix(i + r * NR);
+            const Ix gidx = n * in_gn + sp.j * in_gj;
+            const bool ok = sp.ok && gidx < nv;
+            v[e * S::R0 + r] = ok ? ldg(a.x, Ix(sp.in_base + n * in_ns)) : mk(T(0), T(0));
         }
     }
 }
@@ -354,10 +389,11 @@ __device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile
 // Everything after the stage-0 loads of one tile: conj / pre-multiply, the
 // transform (conv: forward, spectrum, inverse), twiddle / post-multiply and
 // the stores.  `lds` holds B * (N + N/32 + 1) elements.
-template <typename T, int LOGN, int NT, bool CONV>
+template <typename T, int LOGN, int NT, bool CONV, bool I32 = false>
 __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t tile, cpx<T> *v,
                                                  cpx<T> *lds) {
     using S = TileShape<T, LOGN, NT>;
+    using Ix = IdxT<I32>;
     constexpr int LOGE = S::LOGE;
     constexpr int E = S::E;
     constexpr int N = S::N;
@@ -384,13 +420,13 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
         for (int e = 0; e < NB; ++e) {
             int b, i;
             bfly_pos<LOGB, LOGNR>(threadIdx.x + e * NT, a.seqfast_in, b, i);
-            const SeqPos sp = seq_pos(a, s0 + b);
+            const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                const int n = i + r * NR;
-                const int64_t gidx = n * a.in_gn + sp.j * a.in_gj;
-                const bool ok = sp.ok && gidx < a.n_valid;
-                v[e * R0 + r] = cmul(v[e * R0 + r], ok ? a.pre[gidx] : mk(T(0), T(0)));
+                const Ix n = Ix(i + r * NR);
+                const Ix gidx = n * Ix(a.in_gn) + sp.j * Ix(a.in_gj);
+                const bool ok = sp.ok && gidx < idx_bound<Ix>(a.n_valid);
+                v[e * R0 + r] = cmul(v[e * R0 + r], ok ? ldg(a.pre, gidx) : mk(T(0), T(0)));
             }
         }
     }
@@ -403,9 +439,9 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
             for (int e = 0; e < E; ++e) {
                 int b, i;
                 bfly_pos<LOGB, 0>(threadIdx.x + e * NT, cur_sf, b, i);
-                const SeqPos sp = seq_pos(a, s0 + b);
+                const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
                 if (sp.ok) {
-                    const cpx<T> d = a.D[sp.j * a.d_gj];
+                    const cpx<T> d = ldg(a.D, Ix(sp.j * Ix(a.d_gj)));
                     v[e] = a.conj_d ? cmulc(v[e], d) : cmul(v[e], d);
                 }
             }
@@ -429,12 +465,12 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
             for (int e = 0; e < NBF; ++e) {
                 int b, i;
                 bfly_pos<LOGB, LOGNRF>(threadIdx.x + e * NT, cur_sf, b, i);
-                const SeqPos sp = seq_pos(a, s0 + b);
+                const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
                 if (!sp.ok) continue;
 #pragma unroll
                 for (int r = 0; r < RF; ++r) {
-                    const int n = i + r * NRF;
-                    const cpx<T> d = a.D[n * a.d_gn + sp.j * a.d_gj];
+                    const Ix n = Ix(i + r * NRF);
+                    const cpx<T> d = ldg(a.D, Ix(n * Ix(a.d_gn) + sp.j * Ix(a.d_gj)));
                     v[e * RF + r] = a.conj_d ? cmulc(v[e * RF + r], d) : cmul(v[e * RF + r], d);
                 }
             }
@@ -469,12 +505,14 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
         }
         cur_sf = want_sf;
     }
-    const int lo_mask = (1 << a.twb_shift) - 1;
+    const Ix lo_mask = (Ix(1) << a.twb_shift) - 1;
+    const Ix mv = idx_bound<Ix>(a.m_valid);
+    const Ix out_gn = Ix(a.out_gn), out_gj = Ix(a.out_gj), out_ns = Ix(a.out_ns);
 #pragma unroll
     for (int e = 0; e < NBL; ++e) {
         int b, i;
         bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, cur_sf, b, i);
-        const SeqPos sp = seq_pos(a, s0 + b);
+        const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
         if (!sp.ok) continue;
         cpx<T> w[RL];
         if (a.twb_hi || a.post) {
@@ -483,50 +521,48 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
             // table lookups each, j n < L), rounded once to T
             cpx<double> tw = mk(1.0, 0.0), step = mk(1.0, 0.0);
             if (a.twb_hi) {
-                const int64_t e0 = sp.j * i, e1 = sp.j * NRL;
-                tw = cmul(a.twb_hi[e0 >> a.twb_shift], a.twb_lo[e0 & lo_mask]);
-                step = cmul(a.twb_hi[e1 >> a.twb_shift], a.twb_lo[e1 & lo_mask]);
+                const Ix e0 = sp.j * Ix(i), e1 = sp.j * Ix(NRL);
+                tw = cmul(ldg(a.twb_hi, Ix(e0 >> a.twb_shift)), ldg(a.twb_lo, Ix(e0 & lo_mask)));
+                step = cmul(ldg(a.twb_hi, Ix(e1 >> a.twb_shift)), ldg(a.twb_lo, Ix(e1 & lo_mask)));
             }
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                const int n = i + r * NRL;
+                const Ix n = Ix(i + r * NRL);
                 cpx<T> t = mk(T(1), T(0));
                 if (a.twb_hi) {
                     t = mk(T(tw.x), T(a.twb_sign > 0 ? -tw.y : tw.y));
                     if (r + 1 < RL) tw = cmul(tw, step);
                 }
-                const int64_t gidx = n * a.out_gn + sp.j * a.out_gj;
-                if (a.post && gidx < a.m_valid) t = cmul(t, a.post[gidx]);
+                const Ix gidx = n * out_gn + sp.j * out_gj;
+                if (a.post && gidx < mv) t = cmul(t, ldg(a.post, gidx));
                 w[r] = t;
             }
         }
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
-            const int n = i + r * NRL;
-            const int64_t gidx = n * a.out_gn + sp.j * a.out_gj;
-            if (gidx >= a.m_valid) continue;
+            const Ix n = Ix(i + r * NRL);
+            const Ix gidx = n * out_gn + sp.j * out_gj;
+            if (gidx >= mv) continue;
             cpx<T> u = v[e * RL + r];
             if (a.twb_hi || a.post) u = cmul(u, w[r]);
             if (a.conj_out) u = conj(u);
-            a.y[sp.out_base + n * a.out_ns] = scale(u, a.out_scale);
+            stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
         }
     }
 }
 
-// One tile of one pass: the body shared by the per-operator kernels and the
-// batched persistent kernel.
-template <typename T, int LOGN, int NT, bool CONV>
+template <typename T, int LOGN, int NT, bool CONV, bool I32 = false>
 __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile, cpx<T> *lds) {
     cpx<T> v[TileShape<T, LOGN, NT>::E];
-    fft_tile_load<T, LOGN, NT>(a, tile, v);
-    fft_tile_compute<T, LOGN, NT, CONV>(a, tile, v, lds);
+    fft_tile_load<T, LOGN, NT, I32>(a, tile, v);
+    fft_tile_compute<T, LOGN, NT, CONV, I32>(a, tile, v, lds);
 }
 
-template <typename T, int LOGN, int NT, bool CONV>
+template <typename T, int LOGN, int NT, bool CONV, bool I32>
 __global__ __launch_bounds__(NT) void fft_pass_kernel(const TileArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    fft_pass_body<T, LOGN, NT, CONV>(a, xcd_remap(blockIdx.x, gridDim.x),
-                                     reinterpret_cast<cpx<T> *>(smem_raw));
+    fft_pass_body<T, LOGN, NT, CONV, I32>(a, xcd_remap(blockIdx.x, gridDim.x),
+                                          reinterpret_cast<cpx<T> *>(smem_raw));
 }
 
 // Host-side launcher for one tile pass (defined in fm_fft.hip).
