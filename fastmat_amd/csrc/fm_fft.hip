@@ -220,7 +220,7 @@ template <typename T> static fm_status normalize_pass(TileArgs<T> &a) {
     a.B = tile_batch<T>(a.logN);
     a.ntiles = ceil_div(a.O * a.J, a.B);
     a.logJ = is_pow2(a.J) ? ilog2(a.J) : -1;
-    if (a.o_fastest && (a.J % a.B) != 0) a.o_fastest = 0;
+    if (a.o_group > 1 && ((a.J % a.B) != 0 || (a.O % a.o_group) != 0)) a.o_group = 1;
     return FM_SUCCESS;
 }
 
@@ -428,7 +428,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         a.tw = (const cpx<double> *)g.tw2;
         a.D = (const cpx<T> *)io.D;
         a.d_gn = 1; a.d_gj = N2; a.conj_d = io.conj_d;  // spectrum in row-pass order
-        a.o_fastest = 1;  // all columns of a row block reuse its spectrum rows
+        a.o_group = io.k;  // all columns of a row block reuse its spectrum rows
         a.twb_hi = (const cpx<double> *)g.twb_hi; a.twb_lo = (const cpx<double> *)g.twb_lo;
         a.twb_shift = g.twb_shift; a.twb_sign = +1;
         a.y = tmp;

@@ -190,7 +190,7 @@ template <typename T> struct TileArgs {
     T out_scale;
     int seqfast_out;
     int logJ;                       // log2(J) when J is a power of two, else -1
-    int o_fastest;                  // tile t covers (o = t mod O, j-block t / O)
+    int64_t o_group;                // > 1: tiles walk groups of o_group o's (see tile_first_seq)
 };
 
 // Index arithmetic of a pass: int64_t in general, uint32_t when every
@@ -351,13 +351,20 @@ template <typename T, int LOGN, int NT> struct TileShape {
     static_assert(LOGB >= 0, "tile must hold at least one sequence");
 };
 
-// First sequence of a tile: consecutive tiles step through j-blocks of one o,
-// or (o_fastest) through the o's of one j-block, so that tiles sharing per-j
-// data (a spectrum row) run close together.
+// First sequence of a tile.  Plain order: consecutive tiles step through the
+// j-blocks of one o.  With o_group = G > 1 (G divides O, B divides J) the o's
+// form groups of G and, inside a group, consecutive tiles step through the G
+// o's of one j-block before the next j-block: tiles that share per-j data (a
+// spectrum row, chirp or twiddle values) run close together and reuse it
+// from cache, while each o still streams through neighbouring j-blocks.
 template <typename T, int LOGB>
 __device__ __forceinline__ int64_t tile_first_seq(const TileArgs<T> &a, int64_t tile) {
-    return a.o_fastest ? (tile % a.O) * a.J + (tile / a.O) * (int64_t(1) << LOGB)
-                       : (tile << LOGB);
+    if (a.o_group <= 1) return tile << LOGB;
+    const int64_t G = a.o_group;
+    const int64_t per = G * (a.J >> LOGB);  // tiles of one group
+    const int64_t grp = tile / per, rem = tile - grp * per;
+    const int64_t jb = rem / G, o = grp * G + (rem - jb * G);
+    return o * a.J + (jb << LOGB);
 }
 
 // Stage-0 inputs of one tile straight from global memory (zero outside the
