@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_MFMA_PEAK_TFLOPS = 157.3
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 
 
 def parse():
@@ -374,9 +374,9 @@ def time_cpu(legs):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(columns / operators shard with no exchange; see the design notes).",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the columns of "
+              "a block and the operators of a batch would shard across GPUs with no "
+              "exchange, one process per GPU).", file=sys.stderr)
         sys.exit(2)
     import torch
     import fastmat_amd  # noqa: F401
@@ -420,11 +420,23 @@ def main():
     avg_ms = rec["ms"] / rec["launches"]
     bytes_per_launch = rec["bytes"] / rec["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(name), "kernel": name,
-                "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
-                "algorithmic_bytes_per_launch": bytes_per_launch, "timing": roof_note}
+    if name.startswith("gemm_"):
+        # dense factor: FP32 MFMA-bound (2 M K N flop over far fewer bytes)
+        flops_per_launch = rec["flops"] / rec["launches"]
+        tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(tflops, 2), "peak": FP32_MFMA_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tflops / FP32_MFMA_PEAK_TFLOPS, 4),
+                    "traffic": traffic_from_profiles(name), "kernel": name,
+                    "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
+                    "algorithmic_flops_per_launch": flops_per_launch,
+                    "algorithmic_bytes_per_launch": bytes_per_launch,
+                    "hbm_GBps": round(achieved, 1), "timing": roof_note}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profiles(name), "kernel": name,
+                    "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
+                    "algorithmic_bytes_per_launch": bytes_per_launch, "timing": roof_note}
     kernels = {k: {"launches": v["launches"], "avg_us": round(v["ms"] / v["launches"] * 1e3, 2),
                    "GBps": round(v["bytes"] / v["ms"] / 1e6, 1)}
                for k, v in results.items()}
