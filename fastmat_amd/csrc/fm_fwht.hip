@@ -222,6 +222,33 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
     // sequence-fastest when the last stage is also the first
     constexpr int LRL = P::bits(P::S - 1), RL = 1 << LRL, QL = P::q(P::S - 1);
     constexpr int NBL = E / RL, LOGUL = LOGN - LRL;
+    if (seqfast && P::S > 1) {
+        // strided vectors: re-stage the last stage through LDS so that
+        // neighbouring lanes store neighbouring vectors (B-element runs per
+        // row) instead of one element per row each
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < NBL; ++e) {
+            int b, u;
+            wht_map<LOGB, LOGUL>(threadIdx.x + e * NT, false, b, u);
+#pragma unroll
+            for (int t = 0; t < RL; ++t) lds[b * SB + lpad(wht_pos<QL, LRL>(u, t))] = v[e * RL + t];
+        }
+        __syncthreads();
+        // idx = tid + e NT: vector b = idx mod B is the same for all e
+        const int b = threadIdx.x & (B - 1);
+        bool ok;
+        int64_t base;
+        wht_seq(a, s0 + b, ok, base);
+        if (ok) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int p = (threadIdx.x >> LOGB) + e * (NT >> LOGB);
+                y[base + int64_t(p) * a.ns] = to_s<S, C>(lds[b * SB + lpad(p)]);
+            }
+        }
+        return;
+    }
     const bool sfl = P::S == 1 ? seqfast : false;
 #pragma unroll
     for (int e = 0; e < NBL; ++e) {
@@ -237,11 +264,10 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
     }
 }
 
-template <typename S, int LOGN>
-static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
+template <typename S, int LOGN, int NT>
+static fm_status wht_launch_nt(WhtArgs a, hipStream_t stream) {
     using C = typename WhtTraits<S>::C;
     constexpr int LOGE = wht_loge<C>();
-    constexpr int NT = LOGN > LOGE + 8 ? 512 : 256;
     constexpr int B = (NT << LOGE) >> LOGN;
     constexpr int N = 1 << LOGN;
     const size_t lds = size_t(B) * (N + (N >> 5) + 1) * sizeof(C);
@@ -267,6 +293,22 @@ static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
         prof_end(stream, &tok, name, 2.0 * elems * sizeof(S), elems * LOGN);
     }
     return FM_SUCCESS;
+}
+
+template <typename S, int LOGN>
+static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
+    using C = typename WhtTraits<S>::C;
+    constexpr int LOGE = wht_loge<C>();
+    constexpr int NT = LOGN > LOGE + 8 ? 512 : 256;
+    constexpr int N = 1 << LOGN;
+    // strided vectors: 512-thread tiles when they still leave room for two
+    // tiles per CU, so every row segment a wave moves is twice as long
+    if constexpr (NT == 256) {
+        constexpr int B2 = (512 << LOGE) >> LOGN;
+        constexpr size_t lds2 = size_t(B2) * (N + (N >> 5) + 1) * sizeof(C);
+        if (a.seqfast && a.J >= B2 && lds2 <= 80 * 1024) return wht_launch_nt<S, LOGN, 512>(a, stream);
+    }
+    return wht_launch_nt<S, LOGN, NT>(a, stream);
 }
 
 template <typename S> constexpr int wht_maxlog() {
