@@ -984,7 +984,7 @@ template <typename T> struct BatchDesc {
 
 // One persistent kernel per (length, conv) class: grid-strided over the
 // class's (descriptor, tile) items, most expensive first.
-template <typename T, int LOGN, bool CONV>
+template <typename T, int LOGN, bool CONV, bool I32>
 __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchDesc<T> *__restrict__ descs,
                                                          const int2 *__restrict__ items,
                                                          int64_t nitems) {
@@ -993,7 +993,7 @@ __global__ __launch_bounds__(256) void batch_pass_kernel(const BatchDesc<T> *__r
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 w = items[it];
         const TileArgs<T> a = descs[w.x].a;
-        fft_pass_body<T, LOGN, 256, CONV>(a, w.y, lds);
+        fft_pass_body<T, LOGN, 256, CONV, I32>(a, w.y, lds);
         __syncthreads();  // the next item reuses the LDS tile
     }
 }
@@ -1003,6 +1003,7 @@ template <typename T> constexpr int batch_max_log() { return sizeof(T) == 4 ? 13
 // Work items of one (phase, length, conv) class, in launch order.
 struct BatchGroup {
     int phase, logN, conv;
+    int idx32;  // every descriptor of the group fits 32-bit index arithmetic
     void *items;
     int64_t nitems;
     double bytes;
@@ -1129,9 +1130,14 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
     // largest first, items inside a group in descriptor order
     std::map<std::tuple<int, int, int>, std::vector<int2>> groups;
     std::map<std::tuple<int, int, int>, double> gbytes;
+    std::map<std::tuple<int, int, int>, bool> gfit;
     for (size_t di = 0; di < descs.size(); ++di) {
         const TileArgs<T> &a = descs[di].a;
         auto key = std::make_tuple(phase_of[di], -a.logN, descs[di].conv ? 0 : 1);
+        const bool fit = fits_idx32(a, int64_t(1) << a.logN);
+        auto fit_it = gfit.find(key);
+        if (fit_it == gfit.end()) gfit[key] = fit;
+        else fit_it->second = fit_it->second && fit;
         auto &v = groups[key];
         for (int64_t t = 0; t < a.ntiles; ++t) {
             int2 w;
@@ -1154,6 +1160,7 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
         g.conv = std::get<2>(kv.first) == 0 ? 1 : 0;
         g.nitems = int64_t(kv.second.size());
         g.bytes = gbytes[kv.first];
+        g.idx32 = gfit[kv.first] ? 1 : 0;
         g.items = nullptr;
         FM_CHECK_HIP(hipMalloc(&g.items, sizeof(int2) * kv.second.size()));
         FM_CHECK_HIP(hipMemcpyAsync(g.items, kv.second.data(), sizeof(int2) * kv.second.size(),
@@ -1187,7 +1194,9 @@ static fm_status batch_launch(const fm_batch_plan *p, const BatchDir &d, const B
     const size_t lds = size_t(tile_batch<T>(LOGN)) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)batch_pass_kernel<T, LOGN, CONV>,
+        (void)hipFuncSetAttribute((const void *)batch_pass_kernel<T, LOGN, CONV, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)batch_pass_kernel<T, LOGN, CONV, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
@@ -1195,8 +1204,13 @@ static fm_status batch_launch(const fm_batch_plan *p, const BatchDir &d, const B
     if (grid > g.nitems) grid = g.nitems;
     ProfToken tok;
     prof_begin(s, &tok);
-    hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV>), dim3(unsigned(grid)), dim3(256), lds, s,
-                       (const BatchDesc<T> *)d.descs, (const int2 *)g.items, g.nitems);
+    if (g.idx32)
+        hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, true>), dim3(unsigned(grid)), dim3(256),
+                           lds, s, (const BatchDesc<T> *)d.descs, (const int2 *)g.items, g.nitems);
+    else
+        hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, false>), dim3(unsigned(grid)),
+                           dim3(256), lds, s, (const BatchDesc<T> *)d.descs,
+                           (const int2 *)g.items, g.nitems);
     FM_CHECK_LAUNCH();
     if (tok.active) {
         char name[80];
