@@ -172,14 +172,15 @@ template <typename T> static bool fits_idx32(const TileArgs<T> &a, int64_t N) {
 
 template <typename T, int LOGN, bool CONV>
 static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
-    constexpr int NT = (sizeof(T) == 4 ? LOGN >= 14 : LOGN >= 13) ? 512 : 256;
+    constexpr int NT = PassCfg<T, LOGN>::NT;
+    constexpr int LE = PassCfg<T, LOGN>::LE;
     constexpr int N = 1 << LOGN;
     const size_t lds = size_t(a.B) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV, false>,
+        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV, false, LE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV, true>,
+        (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, NT, CONV, true, LE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
@@ -190,11 +191,11 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
     ProfToken tok;
     prof_begin(stream, &tok);
     if (fits_idx32(a, N))
-        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, true>), dim3(unsigned(a.ntiles)),
+        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, true, LE>), dim3(unsigned(a.ntiles)),
                            dim3(NT), lds, stream, a);
     else
-        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, false>), dim3(unsigned(a.ntiles)),
-                           dim3(NT), lds, stream, a);
+        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, false, LE>),
+                           dim3(unsigned(a.ntiles)), dim3(NT), lds, stream, a);
     FM_CHECK_LAUNCH();
     if (tok.active) {
         const double span = double(a.J) * N;
@@ -985,15 +986,14 @@ template <typename T> struct BatchDesc {
 // One persistent kernel per (length, conv) class: grid-strided over the
 // class's (descriptor, tile) items, most expensive first.
 template <typename T, int LOGN, bool CONV, bool I32>
-__global__ __launch_bounds__(256) void batch_pass_kernel(const BatchDesc<T> *__restrict__ descs,
-                                                         const int2 *__restrict__ items,
-                                                         int64_t nitems) {
+__global__ __launch_bounds__((PassCfg<T, LOGN>::NT)) void batch_pass_kernel(
+    const BatchDesc<T> *__restrict__ descs, const int2 *__restrict__ items, int64_t nitems) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 w = items[it];
         const TileArgs<T> a = descs[w.x].a;
-        fft_pass_body<T, LOGN, 256, CONV, I32>(a, w.y, lds);
+        fft_pass_body<T, LOGN, PassCfg<T, LOGN>::NT, CONV, I32, PassCfg<T, LOGN>::LE>(a, w.y, lds);
         __syncthreads();  // the next item reuses the LDS tile
     }
 }
@@ -1204,12 +1204,13 @@ static fm_status batch_launch(const fm_batch_plan *p, const BatchDir &d, const B
     if (grid > g.nitems) grid = g.nitems;
     ProfToken tok;
     prof_begin(s, &tok);
+    constexpr int NT = PassCfg<T, LOGN>::NT;
     if (g.idx32)
-        hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, true>), dim3(unsigned(grid)), dim3(256),
+        hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, true>), dim3(unsigned(grid)), dim3(NT),
                            lds, s, (const BatchDesc<T> *)d.descs, (const int2 *)g.items, g.nitems);
     else
         hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, false>), dim3(unsigned(grid)),
-                           dim3(256), lds, s, (const BatchDesc<T> *)d.descs,
+                           dim3(NT), lds, s, (const BatchDesc<T> *)d.descs,
                            (const int2 *)g.items, g.nitems);
     FM_CHECK_LAUNCH();
     if (tok.active) {

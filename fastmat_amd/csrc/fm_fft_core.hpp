@@ -337,8 +337,9 @@ __device__ __forceinline__ void fft_all(cpx<T> *v, cpx<T> *lds,
 }
 
 // Compile-time shape of one tile of a length-2^LOGN pass with NT threads.
-template <typename T, int LOGN, int NT> struct TileShape {
-    static constexpr int LOGE = sizeof(T) == 4 ? 5 : 4;
+template <typename T, int LOGN, int NT, int LE = -1> struct TileShape {
+    // elements per thread: 2^LE, by default 32 (complex64) / 16 (complex128)
+    static constexpr int LOGE = LE >= 0 ? LE : (sizeof(T) == 4 ? 5 : 4);
     static constexpr int E = 1 << LOGE;
     static constexpr int N = 1 << LOGN;
     using P = FftPlan<LOGN, LOGE>;
@@ -369,9 +370,9 @@ __device__ __forceinline__ int64_t tile_first_seq(const TileArgs<T> &a, int64_t 
 
 // Stage-0 inputs of one tile straight from global memory (zero outside the
 // valid range); only loads, so a caller can issue them ahead of compute.
-template <typename T, int LOGN, int NT, bool I32 = false>
+template <typename T, int LOGN, int NT, bool I32 = false, int LE = -1>
 __device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile, cpx<T> *v) {
-    using S = TileShape<T, LOGN, NT>;
+    using S = TileShape<T, LOGN, NT, LE>;
     using Ix = IdxT<I32>;
     constexpr int NB = S::E / S::R0;
     constexpr int LOGNR = LOGN - S::LOGR0, NR = 1 << LOGNR;
@@ -396,10 +397,10 @@ __device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile
 // Everything after the stage-0 loads of one tile: conj / pre-multiply, the
 // transform (conv: forward, spectrum, inverse), twiddle / post-multiply and
 // the stores.  `lds` holds B * (N + N/32 + 1) elements.
-template <typename T, int LOGN, int NT, bool CONV, bool I32 = false>
+template <typename T, int LOGN, int NT, bool CONV, bool I32 = false, int LE = -1>
 __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t tile, cpx<T> *v,
                                                  cpx<T> *lds) {
-    using S = TileShape<T, LOGN, NT>;
+    using S = TileShape<T, LOGN, NT, LE>;
     using Ix = IdxT<I32>;
     constexpr int LOGE = S::LOGE;
     constexpr int E = S::E;
@@ -558,18 +559,18 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
     }
 }
 
-template <typename T, int LOGN, int NT, bool CONV, bool I32 = false>
+template <typename T, int LOGN, int NT, bool CONV, bool I32 = false, int LE = -1>
 __device__ __forceinline__ void fft_pass_body(const TileArgs<T> &a, int64_t tile, cpx<T> *lds) {
-    cpx<T> v[TileShape<T, LOGN, NT>::E];
-    fft_tile_load<T, LOGN, NT, I32>(a, tile, v);
-    fft_tile_compute<T, LOGN, NT, CONV, I32>(a, tile, v, lds);
+    cpx<T> v[TileShape<T, LOGN, NT, LE>::E];
+    fft_tile_load<T, LOGN, NT, I32, LE>(a, tile, v);
+    fft_tile_compute<T, LOGN, NT, CONV, I32, LE>(a, tile, v, lds);
 }
 
-template <typename T, int LOGN, int NT, bool CONV, bool I32>
+template <typename T, int LOGN, int NT, bool CONV, bool I32, int LE = -1>
 __global__ __launch_bounds__(NT) void fft_pass_kernel(const TileArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    fft_pass_body<T, LOGN, NT, CONV, I32>(a, xcd_remap(blockIdx.x, gridDim.x),
-                                          reinterpret_cast<cpx<T> *>(smem_raw));
+    fft_pass_body<T, LOGN, NT, CONV, I32, LE>(a, xcd_remap(blockIdx.x, gridDim.x),
+                                              reinterpret_cast<cpx<T> *>(smem_raw));
 }
 
 // Host-side launcher for one tile pass (defined in fm_fft.hip).
@@ -580,6 +581,20 @@ fm_status launch_tile(const TileArgs<T> &a, bool conv, hipStream_t stream);
 template <typename T> constexpr int max_tile_elems() {
     return sizeof(T) == 4 ? 16384 : 8192;
 }
+
+// Launch shape of a length-2^LOGN pass.  Up to 1024 points a thread holds 32
+// complex64 (16 complex128) elements; for 2048-8192 complex64 points it holds
+// 16 with twice the threads, so the tile (and its LDS) is unchanged but
+// registers drop from ~220 to ~120 per thread and a CU runs twice the waves
+// (measured: the 2048-point spectrum pass of Bluestein 1,000,003 x 64 ran
+// about 20% faster; at 1024 points the 32-element shape stays faster).
+template <typename T, int LOGN> struct PassCfg {
+    // (16384 points would need 1024 threads at <= 128 registers: it spills)
+    static constexpr bool e16 = sizeof(T) == 4 && LOGN >= 11 && LOGN <= 13;
+    static constexpr int LE = e16 ? 4 : -1;
+    static constexpr int NT =
+        e16 ? 512 : ((sizeof(T) == 4 ? LOGN >= 14 : LOGN >= 13) ? 512 : 256);
+};
 
 // Sequences per tile of the pass kernel for a length-2^logN transform.
 template <typename T> inline int tile_batch(int logN) {
