@@ -188,14 +188,35 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
                lds);
     FM_REQUIRE(a.ntiles > 0 && a.ntiles < (int64_t(1) << 31), FM_ERR_VALUE,
                "tile count %lld out of range", (long long)a.ntiles);
+    const bool i32 = fits_idx32(a, N);
+    // 1024-point complex64 passes that also multiply by a chirp (Bluestein
+    // pass A / C) run the 16-element, 512-thread shape: with the chirp loads
+    // they gain more from the extra waves than they lose to the extra stage
+    // (plain 1024-point passes are faster with 32 elements per thread)
+    bool e16 = false;
+    if constexpr (sizeof(T) == 4 && LOGN == 10) e16 = i32 && (a.pre || a.post);
     ProfToken tok;
     prof_begin(stream, &tok);
-    if (fits_idx32(a, N))
-        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, true, LE>), dim3(unsigned(a.ntiles)),
-                           dim3(NT), lds, stream, a);
-    else
-        hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, false, LE>),
-                           dim3(unsigned(a.ntiles)), dim3(NT), lds, stream, a);
+    if constexpr (sizeof(T) == 4 && LOGN == 10) {
+        if (e16) {
+            static bool attr16 = false;
+            if (!attr16) {
+                (void)hipFuncSetAttribute((const void *)fft_pass_kernel<T, LOGN, 512, CONV, true, 4>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                attr16 = true;
+            }
+            hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, 512, CONV, true, 4>),
+                               dim3(unsigned(a.ntiles)), dim3(512), lds, stream, a);
+        }
+    }
+    if (!e16) {
+        if (i32)
+            hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, true, LE>),
+                               dim3(unsigned(a.ntiles)), dim3(NT), lds, stream, a);
+        else
+            hipLaunchKernelGGL((fft_pass_kernel<T, LOGN, NT, CONV, false, LE>),
+                               dim3(unsigned(a.ntiles)), dim3(NT), lds, stream, a);
+    }
     FM_CHECK_LAUNCH();
     if (tok.active) {
         const double span = double(a.J) * N;
@@ -205,7 +226,8 @@ static fm_status launch_pass(const TileArgs<T> &a, hipStream_t stream) {
         const double spec = CONV ? (a.d_gj ? span : double(N)) : 0.0;
         char name[96];
         snprintf(name, sizeof(name), "fft_pass_kernel<%s,%d,%d,%s>",
-                 sizeof(T) == 4 ? "float" : "double", LOGN, NT, CONV ? "conv" : "fft");
+                 sizeof(T) == 4 ? "float" : "double", LOGN, e16 ? 512 : NT,
+                 CONV ? "conv" : "fft");
         const double nseq = double(a.O) * double(a.J);
         const double flops = nseq * 5.0 * N * LOGN * (CONV ? 2.0 : 1.0);
         prof_end(stream, &tok, name, (in + out + spec) * esz, flops);
