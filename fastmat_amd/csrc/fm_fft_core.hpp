@@ -582,15 +582,18 @@ template <typename T> constexpr int max_tile_elems() {
     return sizeof(T) == 4 ? 16384 : 8192;
 }
 
-// Launch shape of a length-2^LOGN pass.  Up to 1024 points a thread holds 32
-// complex64 (16 complex128) elements; for 2048-8192 complex64 points it holds
-// 16 with twice the threads, so the tile (and its LDS) is unchanged but
-// registers drop from ~220 to ~120 per thread and a CU runs twice the waves
-// (measured: the 2048-point spectrum pass of Bluestein 1,000,003 x 64 ran
-// about 20% faster; at 1024 points the 32-element shape stays faster).
+// Launch shape of a length-2^LOGN pass.  A thread holds 32 complex64 (16
+// complex128) elements by default.  Complex64 passes of 2-256 and of
+// 2048-8192 points hold 16 with twice the threads instead: the tile (and its
+// LDS) is unchanged, registers drop to ~90-125 per thread and a CU runs twice
+// the waves.  Up to 256 points the stage count is the same either way; at
+// 2048-8192 the extra waves pay for the extra stage; at 512 and 1024 points
+// the 32-element shape's one LDS exchange wins (measured on the uneven batch,
+// the Kron modes and the 2048-point Bluestein spectrum pass).
 template <typename T, int LOGN> struct PassCfg {
     // (16384 points would need 1024 threads at <= 128 registers: it spills)
-    static constexpr bool e16 = sizeof(T) == 4 && LOGN >= 11 && LOGN <= 13;
+    static constexpr bool e16 =
+        sizeof(T) == 4 && ((LOGN >= 11 && LOGN <= 13) || (LOGN >= 1 && LOGN <= 8));
     static constexpr int LE = e16 ? 4 : -1;
     static constexpr int NT =
         e16 ? 512 : ((sizeof(T) == 4 ? LOGN >= 14 : LOGN >= 13) ? 512 : 256);
