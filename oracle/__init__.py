@@ -3,8 +3,13 @@
 TEST INFRASTRUCTURE ONLY.  Nothing in the product (``fastmat_amd``) imports,
 calls or links anything in this package.  Only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use
-it, and only as the checker (or, for ``cpu_baseline``, as the timed CPU
+it, and only to check results (or, for ``cpu_baseline``, as the timed CPU
 restatement of the reference's own CPU path).
+
+Every function here is pinned by ``tests/test_oracle_pins.py`` against
+something that does not restate its own formula (a library routine, a closed
+form or invariant, brute force on tiny inputs, or the reference's own test
+statements); none is left unpinned.
 
 Modules
 -------
