@@ -384,3 +384,30 @@ def test_dense_sum_and_partial():
     assert Pa.shape == (3, 4)
     assert Pa[0, 0] == A[4, 0] and Pa[2, 3] == A[5, 6] and Pa[1, 1] == A[0, 2]
     np.testing.assert_array_equal(dense.partial(A), A)
+
+
+# ------------------------------------------------------------ Permutation / Diag
+def test_dense_permutation_brute_force():
+    # P x = x[sigma] (Permutation.pyx:121-122) checked entry by entry, P^T is
+    # the inverse (backward uses tau = argsort(sigma), :124-125)
+    rng = np.random.default_rng(33)
+    sigma = rng.permutation(9)
+    P = dense.permutation(sigma)
+    x = rng.standard_normal(9)
+    y = [sum(P[i, j] * x[j] for j in range(9)) for i in range(9)]
+    assert all(y[i] == x[sigma[i]] for i in range(9))
+    np.testing.assert_array_equal(P.T @ P, np.eye(9))
+    tau = np.argsort(sigma)
+    np.testing.assert_array_equal(P.T @ x, x[tau])
+
+
+def test_dense_diag_and_cpu_hadamard():
+    d = crandn(6)
+    D = dense.diag(d)
+    x = crandn(6)
+    np.testing.assert_array_equal(D @ x, d * x)
+    assert np.count_nonzero(D - np.diag(np.diag(D))) == 0
+    # the timed CPU Hadamard leg equals the dense Sylvester matrix product
+    xi = RNG.integers(-8, 9, (64, 3)).astype(np.int32)
+    np.testing.assert_array_equal(cpu_path.hadamard_forward(xi),
+                                  (sla.hadamard(64).astype(np.int64) @ xi).astype(np.int32))
