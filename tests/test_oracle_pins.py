@@ -405,7 +405,7 @@ def test_dense_diag_and_cpu_hadamard():
     d = crandn(6)
     D = dense.diag(d)
     x = crandn(6)
-    np.testing.assert_array_equal(D @ x, d * x)
+    np.testing.assert_allclose(D @ x, d * x, rtol=1e-15, atol=0)
     assert np.count_nonzero(D - np.diag(np.diag(D))) == 0
     # the timed CPU Hadamard leg equals the dense Sylvester matrix product
     xi = RNG.integers(-8, 9, (64, 3)).astype(np.int32)
