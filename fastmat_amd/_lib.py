@@ -73,6 +73,9 @@ _PROTOTYPES = {
                                ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _vp, _c_i64, _vp]),
     "fm_batch_apply": (_c_int, [_vp, _c_int, _vp]),
     "fm_batch_plan_destroy": (_c_int, [_vp]),
+    "fm_shrink_step": (_c_int, [_c_int, _c_i64, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
+                                _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
+    "fm_counter_advance": (_c_int, [_vp, _vp]),
     "fm_profiler_enable": (_c_int, [_c_int]),
     "fm_profiler_reset": (_c_int, []),
     "fm_profiler_count": (_c_int, [ctypes.POINTER(_c_int)]),

@@ -1,0 +1,249 @@
+"""Shrinkage solvers ISTA and FISTA with device-resident iterations.
+
+Mirrors fastmat.algorithms (algorithms/Algorithm.pyx, ISTA.py, FISTA.py):
+same class names, parameters (numLambda, numMaxSteps), callbacks (cbStep,
+cbTrace, cbResult), ``process`` / ``_process`` and ``softThreshold``.  The
+reference runs every step as numpy arithmetic around the operator's forward
+and backward (ISTA.py:152-168, FISTA.py:154-178).  Here the iterate, the
+measurements and all intermediates stay on the device: each step is the
+operator's forward apply, the residual (fm_axpby), the adjoint apply and one
+fused kernel (fm_shrink_step) that forms the gradient step, the soft
+threshold and, for FISTA, the momentum update.  Without per-step callbacks
+the first step runs eagerly (plans, workspaces) and one step is captured in a
+HIP graph that is replayed for the remaining steps; FISTA's momentum per step
+comes from a device table indexed by a device step counter the graph
+advances (fm_counter_advance).
+
+Results: numpy in -> numpy out, device tensor in -> device tensor out; the
+result is 2-D (N, k) also for 1-D input, like the reference.
+"""
+import copy
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _lib
+from .matrix import Matrix
+
+
+class Algorithm:
+    """Algorithm.pyx:28-176: parameters, callbacks and the trace."""
+
+    def __init__(self):
+        if type(self) is Algorithm:
+            raise NotImplementedError("Algorithm baseclass cannot be instantiated.")
+
+    @property
+    def cbTrace(self):
+        return getattr(self, "_cbTrace", None)
+
+    @cbTrace.setter
+    def cbTrace(self, value):
+        self._cbTrace = value
+
+    @property
+    def cbResult(self):
+        return getattr(self, "_cbResult", None)
+
+    @cbResult.setter
+    def cbResult(self, value):
+        self._cbResult = value
+
+    @property
+    def trace(self):
+        if getattr(self, "_trace", None) is None:
+            self._trace = []
+        return self._trace
+
+    @trace.setter
+    def trace(self, value):
+        if not isinstance(value, list):
+            raise TypeError("Algorithm trace must be a list")
+        self._trace = value
+
+    def updateParameters(self, **kwargs):
+        """Algorithm.pyx:96-121: unknown parameters raise AttributeError."""
+        if getattr(self, "_attributes", None) is None:
+            self._attributes = kwargs.copy()
+        for key, value in kwargs.items():
+            if not hasattr(self, key) and key not in self._attributes:
+                raise AttributeError("Attribute '%s' not defined in %s" % (
+                    key, self.__class__.__name__))
+            setattr(self, key, value)
+
+    def process(self, arrB, **kwargs):
+        """Algorithm.pyx:123-140."""
+        self.updateParameters(**kwargs)
+        arrResult = self._process(arrB)
+        self.handleCallback(self.cbResult)
+        return arrResult
+
+    def _process(self, arrB):
+        raise NotImplementedError("Algorithm is not implemented yet.")
+
+    def snapshot(self):
+        """Algorithm.pyx:151-163: append a copy of the state to the trace."""
+        trace, self._trace = getattr(self, "_trace", None), []
+        if trace is None:
+            trace = []
+        trace.append(copy.copy(self))
+        self._trace = trace
+
+    def handleCallback(self, callback):
+        if callback is not None:
+            return callback(self)
+        return None
+
+
+class _Shrinkage(Algorithm):
+    """Common device loop of ISTA (momentum 0) and FISTA."""
+
+    _name = "ISTA"
+    _momentum = False
+
+    def __init__(self, fmatA, **kwargs):
+        if not isinstance(fmatA, Matrix):
+            raise TypeError("fmatA must be a fastmat matrix")
+        self.fmatA = fmatA
+        self.numLambda = 0.1
+        self.numMaxSteps = 100
+        self.cbStep = None
+        self.updateParameters(**kwargs)
+
+    def softThreshold(self, arrX, numAlpha):
+        """ISTA.py:113-123 (host numpy, as in the reference)."""
+        self.arrM = np.maximum(np.abs(arrX) - numAlpha, 0)
+        return np.multiply((self.arrM / (self.arrM + numAlpha)), arrX)
+
+    # ------------------------------------------------------------ helpers
+    def _work_dtype(self, bdt):
+        """The iterate's dtype: promote(float32, b) widened by whatever the
+        forward / residual / adjoint products return (the reference's numpy
+        promotion of ``x - L * A^H (A x - b)``), found by applying the
+        operator to one zero column until the dtype is stable."""
+        A = self.fmatA
+        w = np.promote_types(np.float32, bdt)
+        for _ in range(4):
+            f = A._apply(D.empty(1, A.numCols, w).zero_(), False)
+            r = np.promote_types(D.np_dtype(f.dtype), bdt)
+            g = A._apply(D.empty(1, A.numRows, r).zero_(), True)
+            w2 = np.promote_types(w, D.np_dtype(g.dtype))
+            if w2 == w:
+                break
+            w = w2
+        if w not in (np.dtype(np.float32), np.dtype(np.float64), np.dtype(np.complex64),
+                     np.dtype(np.complex128)):
+            raise TypeError("%s needs a floating-point problem, got %s" % (self._name, w))
+        return w
+
+    def _step(self, B, X, Y, Step, L, alpha, betas, counter, wdt):
+        """One iteration on the device: Y is the point the gradient is taken
+        at (X itself for ISTA)."""
+        A = self.fmatA
+        k = X.shape[0]
+        F = A._apply(Y, False)
+        if D.np_dtype(F.dtype) != wdt:
+            F = F.to(D.torch_dtype(wdt))
+        # F <- F - B  (alpha x + beta y with x = B, y = F)
+        _lib.call("fm_axpby", _lib.NP_TO_FM[np.dtype(wdt)], A.numRows, k, _lib.dvec(-1.0),
+                  _lib.ptr(B), A.numRows, 0, _lib.dvec(1.0), _lib.ptr(F), A.numRows, D.stream())
+        G = A._apply(F, True)
+        if D.np_dtype(G.dtype) != wdt:
+            G = G.to(D.torch_dtype(wdt))
+        G = G.contiguous()
+        _lib.call("fm_shrink_step", _lib.NP_TO_FM[np.dtype(wdt)], X.numel(), _lib.ptr(Y),
+                  _lib.ptr(G), float(L), float(alpha), _lib.ptr(Step), _lib.ptr(X),
+                  _lib.ptr(betas) if betas is not None else None,
+                  _lib.ptr(counter) if counter is not None else None, 0.0,
+                  _lib.ptr(Y) if self._momentum else None, D.stream())
+        if counter is not None:
+            _lib.call("fm_counter_advance", _lib.ptr(counter), D.stream())
+
+    # ------------------------------------------------------------ solver
+    def _process(self, arrB):
+        A = self.fmatA
+        on_device = torch.is_tensor(arrB)
+        b = arrB
+        if b.ndim > 2 or b.ndim < 1:
+            raise ValueError("Only n x m arrays are supported for %s" % self._name)
+        if self.numMaxSteps <= 0:
+            raise ValueError("%s would like to do at least one step for you" % self._name)
+        if b.ndim == 1:
+            b = b.reshape((-1, 1))
+        self.arrB = b
+        self.numL = 1.0 / (A.largestSingularValue ** 2)
+        alpha = self.numL * self.numLambda * 0.5
+        bdt = D.np_dtype(b.dtype)
+        wdt = self._work_dtype(bdt)
+        Bd, _ = D.to_block(b, A.numRows)
+        Bd = Bd.to(D.torch_dtype(wdt)).contiguous()
+        k = Bd.shape[0]
+        X = D.empty(k, A.numCols, wdt).zero_()
+        Y = X.clone() if self._momentum else X
+        Step = torch.zeros_like(X)
+        betas = counter = None
+        if self._momentum:
+            t, bl = 1.0, []
+            for _ in range(int(self.numMaxSteps)):
+                t_old = t
+                t = (1 + np.sqrt(1 + 4 * t ** 2)) / 2
+                bl.append((t_old - 1) / t)
+            betas = torch.tensor(bl, dtype=torch.float64, device=X.device)
+            counter = torch.zeros(1, dtype=torch.int64, device=X.device)
+        callbacks = self.cbStep is not None or self.cbTrace is not None
+        steps = int(self.numMaxSteps)
+        self.numStep = 0
+        self._step(Bd, X, Y, Step, self.numL, alpha, betas, counter, wdt)
+        if callbacks:
+            self._expose(X, Step, on_device)
+            self.handleCallback(self.cbStep)
+            self.handleCallback(self.cbTrace)
+            for self.numStep in range(1, steps):
+                self._step(Bd, X, Y, Step, self.numL, alpha, betas, counter, wdt)
+                self._expose(X, Step, on_device)
+                self.handleCallback(self.cbStep)
+                self.handleCallback(self.cbTrace)
+        elif steps > 1:
+            graph = torch.cuda.CUDAGraph()
+            cur = torch.cuda.current_stream()
+            side = torch.cuda.Stream()
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(graph, stream=side):
+                    self._step(Bd, X, Y, Step, self.numL, alpha, betas, counter, wdt)
+            cur.wait_stream(side)
+            for self.numStep in range(1, steps):
+                graph.replay()
+            del graph
+        res = torch.where(X != 0, Step, X)
+        self._expose(X, Step, on_device)
+        if on_device:
+            self.arrResult = res.t()
+        else:
+            torch.cuda.synchronize()
+            self.arrResult = np.ascontiguousarray(res.cpu().numpy().T)
+        return self.arrResult
+
+    def _expose(self, X, Step, on_device):
+        """The reference's per-step attributes (arrX, arrStep), for callbacks
+        and after the run: device views for device input, numpy otherwise."""
+        if on_device:
+            self.arrX, self.arrStep = X.t(), Step.t()
+        else:
+            self.arrX = np.ascontiguousarray(X.cpu().numpy().T)
+            self.arrStep = np.ascontiguousarray(Step.cpu().numpy().T)
+
+
+class ISTA(_Shrinkage):
+    """Iterative soft thresholding (ISTA.py:24-168):
+    min ||A x - b||_2^2 + lambda ||x||_1, step 1 / sigma_max(A)^2."""
+    _name = "ISTA"
+    _momentum = False
+
+
+class FISTA(_Shrinkage):
+    """Fast iterative soft thresholding (FISTA.py:24-178): ISTA with the
+    momentum y = x + (t_k - 1) / t_{k+1} (x - x_old)."""
+    _name = "FISTA"
+    _momentum = True

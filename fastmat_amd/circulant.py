@@ -189,6 +189,11 @@ class Circulant(_ConvBase):
             self._multi = (Ls, t, sel, sel)
         self._initProperties(n, n, dtype, **options)
 
+    def _getLargestSingularValue(self):
+        # a (multi-level) circulant matrix is normal and diagonalised by the
+        # (multi-dimensional) DFT: sigma_max = max |fftn(c)|
+        return float(np.max(np.abs(np.fft.fftn(self._tenC.astype(np.complex128)))))
+
     @property
     def tenC(self):
         return self._tenC

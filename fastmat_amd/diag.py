@@ -21,6 +21,9 @@ class Diag(Matrix):
         self._initProperties(d.size, d.size, d.dtype, **options)
         self._forceContiguousInput = True
 
+    def _getLargestSingularValue(self):
+        return float(np.max(np.abs(self._vecD))) if self.numRows else 0.0
+
     @property
     def vecD(self):
         return self._vecD

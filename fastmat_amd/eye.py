@@ -9,6 +9,9 @@ class Eye(Matrix):
         self._content = ()
         self._initProperties(int(order), int(order), np.int8, **options)
 
+    def _getLargestSingularValue(self):
+        return 1.0
+
     def _fwd(self, X):
         return X
 

@@ -74,6 +74,10 @@ class Fourier(Matrix):
         self._plans = _PlanCache(self._make_plan, "fm_fourier_plan_destroy")
         self._initProperties(order, order, dtype, **options)
 
+    def _getLargestSingularValue(self):
+        # F^H F = N I (Fourier.pyx:188-189): every singular value is sqrt(N)
+        return float(np.sqrt(self.numRows))
+
     @property
     def order(self):
         return self._order

@@ -27,6 +27,10 @@ class Hadamard(Matrix):
         self._initProperties(2 ** order, 2 ** order, np.int8, **options)
         self._forceContiguousInput = True
 
+    def _getLargestSingularValue(self):
+        # H H = N I: every singular value is sqrt(N)
+        return float(np.sqrt(self.numRows))
+
     @property
     def order(self):
         return self._order

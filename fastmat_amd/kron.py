@@ -34,6 +34,14 @@ class Kron(Matrix):
         self._initProperties(numRows, numRows, dt, **options)
         self._widenInputDatatype = True
 
+    def _getLargestSingularValue(self):
+        # singular values of a Kronecker product are the products of the
+        # factors' singular values
+        out = 1.0
+        for f in self._content:
+            out *= f.largestSingularValue
+        return float(out)
+
     def _run(self, X, adjoint):
         k = X.shape[0]
         dims = self._dims

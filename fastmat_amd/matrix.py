@@ -85,6 +85,29 @@ class Matrix:
             "bypassAutoArray": self.bypassAutoArray,
         }
 
+    # ------------------------------------------------------------ spectrum
+    @property
+    def largestSingularValue(self):
+        """sigma_max(A) (Matrix.pyx:824-834), computed once and kept."""
+        if getattr(self, "_largestSV", None) is None:
+            return self.getLargestSingularValue()
+        return self._largestSV
+
+    def getLargestSingularValue(self):
+        """Matrix.pyx:836-899: the reference runs scipy's svds (ARPACK) on the
+        operator.  Here classes with a closed form override
+        `_getLargestSingularValue`; the generic form is a power iteration of
+        A^H A on the device (one forward and one adjoint apply per step)."""
+        result = float(self._getLargestSingularValue())
+        if not np.isnan(result):
+            self._largestSV = result
+        return result
+
+    def _getLargestSingularValue(self):
+        if getattr(self, "_array", None) is not None and max(self.shape) <= 4096:
+            return float(np.linalg.svd(np.atleast_2d(self._array), compute_uv=False)[0])
+        return power_iteration_sv(self)
+
     # ------------------------------------------------------------ basics
     @property
     def shape(self):
@@ -474,3 +497,33 @@ def accumulate_block(Y, X):
     _lib.call("fm_axpby", D.fm_type(X.dtype), n, k, _lib.dvec(1.0), _lib.ptr(X), n, 0,
               _lib.dvec(1.0), _lib.ptr(Y), n, D.stream())
     return Y
+
+
+def power_iteration_sv(M, tol=1e-13, max_steps=5000, seed=20261202):
+    """sigma_max of M by power iteration of M^H M on the device: x <- M^H M x
+    / ||M^H M x||, estimate = the Rayleigh quotient x^H M^H M x, until it
+    changes by < tol (relative).  Works in complex128 (or float64 for real
+    operators) so the estimate reaches double precision: the solvers' step
+    size 1 / sigma^2 then matches an SVD-based one to ~1e-13."""
+    wdt = np.complex128 if np.iscomplexobj(np.zeros(1, M.dtype)) else np.float64
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    x = torch.randn(1, M.numCols, generator=g, dtype=torch.float64)
+    if wdt == np.complex128:
+        x = torch.complex(x, torch.randn(1, M.numCols, generator=g, dtype=torch.float64))
+    x = x.to(D.device())
+    x = x / torch.linalg.vector_norm(x)
+    est = 0.0
+    for _ in range(max_steps):
+        z = M._apply(M._apply(x, False), True)
+        if D.np_dtype(z.dtype) != np.dtype(wdt):
+            z = z.to(D.torch_dtype(wdt))
+        rq = float(torch.sum(x.conj() * z).real)  # x normalised
+        nz = float(torch.linalg.vector_norm(z))
+        if nz == 0.0:
+            return 0.0
+        x = z / nz
+        if abs(rq - est) <= tol * abs(rq):
+            est = rq
+            break
+        est = rq
+    return float(np.sqrt(max(est, 0.0)))

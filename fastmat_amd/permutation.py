@@ -22,6 +22,9 @@ class Permutation(Matrix):
         self._content = ()
         self._initProperties(n, n, np.int8, **options)
 
+    def _getLargestSingularValue(self):
+        return 1.0
+
     @property
     def sigma(self):
         return self._sigma

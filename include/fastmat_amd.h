@@ -220,6 +220,26 @@ fm_status fm_axpby(int dtype, int64_t n, int64_t k, const double *alpha,
                    const void *x, int64_t ldx, int conj_x, const double *beta,
                    void *y, int64_t ldy, void *stream);
 
+/* ------------------------------------------------------------ solvers
+ * One shrinkage step of ISTA / FISTA (algorithms/ISTA.py:152-168,
+ * algorithms/FISTA.py:154-178, softThreshold ISTA.py:113-123), fused after
+ * the operator's forward and adjoint applies produced g = A^H (A y - b).
+ * Over n contiguous elements of a float32 / float64 / complex64 / complex128
+ * block, in that precision:
+ *   step  = y - L g
+ *   x     = step * m / (m + alpha),  m = max(|step| - alpha, 0)   (x in/out)
+ *   y_next = x_new + beta (x_new - x_old)  if y_next is not NULL (FISTA;
+ *            x_old is x's value on entry; y_next may alias y)
+ * beta is beta_table[*step_counter] when beta_table is not NULL (device
+ * pointers; step_counter may be NULL for index 0), else the argument `beta`.
+ * fm_counter_advance increments *step_counter on the stream, so a captured
+ * iteration reads the next beta on each replay. */
+fm_status fm_shrink_step(int dtype, int64_t n, const void *y, const void *g, double L,
+                         double alpha, void *step, void *x, const double *beta_table,
+                         const int64_t *step_counter, double beta, void *y_next,
+                         void *stream);
+fm_status fm_counter_advance(int64_t *step_counter, void *stream);
+
 /* ------------------------------------------------------------ profiler
  * No counterpart in the reference.  For benchmarks: when enabled, each of the
  * library's kernel launches is bracketed by HIP events on its own stream and

@@ -411,3 +411,61 @@ def test_dense_diag_and_cpu_hadamard():
     xi = RNG.integers(-8, 9, (64, 3)).astype(np.int32)
     np.testing.assert_array_equal(cpu_path.hadamard_forward(xi),
                                   (sla.hadamard(64).astype(np.int64) @ xi).astype(np.int32))
+
+
+# ------------------------------------------------------------ shrinkage solvers
+from oracle import algorithms as oalg  # noqa: E402
+
+
+def test_soft_threshold_closed_form():
+    # x m/(m+a) with m = max(|x|-a, 0) equals x (1 - a/|x|) for |x| > a, else 0
+    x = np.array([-3.0, -0.5, 0.0, 0.2, 0.5, 2.0])
+    a = 0.5
+    expect = np.where(np.abs(x) > a, x * (1 - a / np.maximum(np.abs(x), 1e-300)), 0.0)
+    np.testing.assert_allclose(oalg.soft_threshold(x, a), expect, rtol=1e-15, atol=0)
+    z = np.array([3 + 4j, 0.1 + 0.1j])
+    np.testing.assert_allclose(oalg.soft_threshold(z, 1.0), [z[0] * 0.8, 0], rtol=1e-15)
+
+
+def test_ista_identity_reaches_shrunk_measurement():
+    # A = I: sigma = 1, L = 1, one step gives soft(b, lambda/2), a fixed point
+    b = np.array([[2.0], [-0.1], [0.7]])
+    res, x, _ = oalg.ista(np.eye(3), b, num_lambda=1.0, num_max_steps=5)
+    np.testing.assert_allclose(x, oalg.soft_threshold(b, 0.5))
+
+
+def test_ista_diagonal_converges_to_separable_minimiser():
+    # min sum (d_i x_i - b_i)^2 + lam |x_i|: x_i = soft(d_i b_i, lam/2) / d_i^2
+    d = np.array([2.0, 1.0, 0.5, 1.5])
+    b = np.array([3.0, -0.2, 1.0, -2.0])
+    lam = 0.4
+    _, x, _ = oalg.ista(np.diag(d), b.astype(np.float64), num_lambda=lam, num_max_steps=4000)
+    expect = oalg.soft_threshold(d * b, lam / 2) / d ** 2
+    np.testing.assert_allclose(x[:, 0], expect, atol=1e-10)
+    _, xf, _ = oalg.fista(np.diag(d), b.astype(np.float64), num_lambda=lam, num_max_steps=4000)
+    np.testing.assert_allclose(xf[:, 0], expect, atol=1e-10)
+
+
+def test_ista_objective_decreases_and_fista_betas():
+    rng = np.random.default_rng(34)
+    A = rng.standard_normal((20, 40))
+    b = rng.standard_normal(20)
+    lam = 0.3
+    L = 1.0 / oalg.largest_singular_value(A) ** 2
+    x = np.zeros(40)
+    prev = np.inf
+    for _ in range(50):
+        x = oalg.soft_threshold(x - L * A.T @ (A @ x - b), L * lam * 0.5)
+        obj = np.sum((A @ x - b) ** 2) + lam * np.sum(np.abs(x))
+        assert obj <= prev + 1e-12
+        prev = obj
+    # t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2 from t_0 = 1: beta_0 = 0, beta_k -> 1
+    bet = oalg.fista_betas(200)
+    assert bet[0] == 0.0 and np.all(np.diff(bet) > 0) and bet[-1] < 1
+    t = 1.0
+    for k in range(5):
+        t1 = (1 + np.sqrt(1 + 4 * t * t)) / 2
+        assert abs(bet[k] - (t - 1) / t1) < 1e-15
+        t = t1
+    # the oracle's sigma is numpy's SVD value
+    assert abs(oalg.largest_singular_value(np.diag([3.0, -5.0, 1.0])) - 5.0) < 1e-14
