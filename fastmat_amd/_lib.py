@@ -76,6 +76,13 @@ _PROTOTYPES = {
     "fm_shrink_step": (_c_int, [_c_int, _c_i64, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
                                 _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     "fm_counter_advance": (_c_int, [_vp, _vp]),
+    "fm_stela_prox": (_c_int, [_c_int, _c_int, _c_i64, _c_i64, _vp, _vp, _vp, ctypes.c_double,
+                               _vp, _vp, _vp, _c_i64, _vp]),
+    "fm_stela_line": (_c_int, [_c_int, _c_i64, _c_i64, _vp, _vp, _vp, _c_i64, ctypes.c_double,
+                               ctypes.c_double, _vp, _vp, _vp]),
+    "fm_stela_update": (_c_int, [_c_int, _c_i64, _c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp]),
+    "fm_stela_select": (_c_int, [_c_int, _c_i64, _c_i64, _vp, _vp, _vp, _vp]),
     "fm_profiler_enable": (_c_int, [_c_int]),
     "fm_profiler_reset": (_c_int, []),
     "fm_profiler_count": (_c_int, [ctypes.POINTER(_c_int)]),

@@ -1,6 +1,8 @@
-"""Shrinkage solvers ISTA and FISTA with device-resident iterations.
+"""Sparse-recovery solvers ISTA, FISTA and STELA with device-resident
+iterations.
 
-Mirrors fastmat.algorithms (algorithms/Algorithm.pyx, ISTA.py, FISTA.py):
+Mirrors fastmat.algorithms (algorithms/Algorithm.pyx, ISTA.py, FISTA.py,
+STELA.py):
 same class names, parameters (numLambda, numMaxSteps), callbacks (cbStep,
 cbTrace, cbResult), ``process`` / ``_process`` and ``softThreshold``.  The
 reference runs every step as numpy arithmetic around the operator's forward
@@ -12,7 +14,9 @@ threshold and, for FISTA, the momentum update.  Without per-step callbacks
 the first step runs eagerly (plans, workspaces) and one step is captured in a
 HIP graph that is replayed for the remaining steps; FISTA's momentum per step
 comes from a device table indexed by a device step counter the graph
-advances (fm_counter_advance).
+advances (fm_counter_advance).  STELA's step (per-column exact line search)
+is four fused kernels around the forward and adjoint applies, captured the
+same way.
 
 Results: numpy in -> numpy out, device tensor in -> device tensor out; the
 result is 2-D (N, k) also for 1-D input, like the reference.
@@ -247,3 +251,135 @@ class FISTA(_Shrinkage):
     momentum y = x + (t_k - 1) / t_{k+1} (x - x_old)."""
     _name = "FISTA"
     _momentum = True
+
+
+class STELA(Algorithm):
+    """Soft thresholding with simplified exact line search (STELA.py:24-267).
+
+    Per column of b: bx = soft(d x - z, lambda) / d with d = 1 / colNorms^2
+    and z = A^H (A x - b), then the exact line-search step gamma in [0, 1]
+    along bx - x; columns whose stopping value drops to numMaxError freeze.
+    Works in promote(float64, b) like the reference.  Each step is four fused
+    kernels (fm_stela_prox, fm_stela_line, fm_stela_update, fm_stela_select)
+    around one forward and one adjoint apply; without callbacks the first
+    step runs eagerly and one captured step is replayed for the rest (once
+    every column is frozen the replays change nothing, which is the
+    reference's early return).  With callbacks each step runs eagerly and the
+    loop returns as soon as no column is active, as the reference does."""
+
+    _name = "STELA"
+
+    def __init__(self, fmatA, **kwargs):
+        if not isinstance(fmatA, Matrix):
+            raise TypeError("fmatA must be a fastmat matrix")
+        self.fmatA = fmatA
+        self.numLambda = 0.1
+        self.numMaxSteps = 100
+        self.numMaxError = 1e-6
+        self.cbStep = None
+        self.updateParameters(**kwargs)
+
+    def softThreshold(self, arrX, numAlpha):
+        """STELA.py:117-126 (host numpy, as in the reference)."""
+        arrM = np.maximum(np.abs(arrX) - numAlpha, 0)
+        return np.multiply((arrM / (arrM + numAlpha)), arrX)
+
+    def _step(self, s):
+        A = self.fmatA
+        n, m, k = A.numCols, A.numRows, s["k"]
+        xt, zt = s["xt"], s["zt"]
+        lam = float(self.numLambda)
+        _lib.call("fm_stela_prox", xt, zt, n, k, _lib.ptr(s["X"]), _lib.ptr(s["Z"]),
+                  _lib.ptr(s["D"]), lam, _lib.ptr(s["Bx"]), _lib.ptr(s["Dlt"]),
+                  _lib.ptr(s["part"]), s["parts"], D.stream())
+        F = A._apply(s["Dlt"], False)
+        if F.is_complex() and not s["X"].is_complex():
+            F = F.real  # the reference assigns into its real array
+        F = F.to(s["X"].dtype).contiguous()
+        _lib.call("fm_stela_line", xt, m, k, _lib.ptr(s["Res"]), _lib.ptr(F),
+                  _lib.ptr(s["part"]), s["parts"], lam, float(self.numMaxError),
+                  _lib.ptr(s["gamma"]), _lib.ptr(s["active"]), D.stream())
+        _lib.call("fm_stela_update", xt, n, m, k, _lib.ptr(s["Bx"]), _lib.ptr(s["X"]),
+                  _lib.ptr(F), _lib.ptr(s["Res"]), _lib.ptr(s["gamma"]), _lib.ptr(s["active"]),
+                  D.stream())
+        Zn = A._apply(s["Res"], True)
+        if Zn.dtype != s["Z"].dtype:
+            Zn = Zn.to(s["Z"].dtype)
+        Zn = Zn.contiguous()
+        _lib.call("fm_stela_select", zt, n, k, _lib.ptr(s["Z"]), _lib.ptr(Zn),
+                  _lib.ptr(s["active"]), D.stream())
+
+    def _process(self, arrB):
+        A = self.fmatA
+        on_device = torch.is_tensor(arrB)
+        if arrB.ndim > 2 or arrB.ndim < 1:
+            raise ValueError("Only n x m arrays are supported for STELA")
+        b = arrB.reshape((-1, 1)) if arrB.ndim == 1 else arrB
+        self.arrB = b
+        if self.numMaxSteps <= 0:
+            raise ValueError("STELA would like to do at least one step for you")
+        wdt = np.promote_types(np.float64, D.np_dtype(b.dtype))
+        if wdt not in (np.dtype(np.float64), np.dtype(np.complex128)):
+            raise TypeError("STELA needs a floating-point problem, got %s" % wdt)
+        n, m = A.numCols, A.numRows
+        Bd, _ = D.to_block(b, m)
+        k = Bd.shape[0]
+        if n == 0 or m == 0 or k == 0:
+            self.arrX = np.zeros((n, k), dtype=wdt)
+            return self.arrX if not on_device else torch.zeros(k, n, dtype=D.torch_dtype(wdt),
+                                                                 device=D.device()).t()
+        tw = D.torch_dtype(wdt)
+        Res = (-Bd.to(tw)).contiguous()
+        Z = A._apply(Res, True)
+        zdt = D.np_dtype(Z.dtype)
+        if zdt not in (np.dtype(np.float64), np.dtype(np.complex128)):
+            zdt = np.promote_types(zdt, np.float64)
+            Z = Z.to(D.torch_dtype(zdt))
+        X = D.empty(k, n, wdt).zero_()
+        norms = np.asarray(A._getColNorms(), dtype=np.float64)
+        parts = int(min(256, max(1, -(-max(n, m) // 4096))))
+        s = dict(k=k, xt=_lib.NP_TO_FM[np.dtype(wdt)], zt=_lib.NP_TO_FM[np.dtype(zdt)],
+                 X=X, Z=Z.contiguous(), Res=Res, Bx=torch.zeros_like(X),
+                 Dlt=torch.zeros_like(X), parts=parts,
+                 D=torch.tensor(1.0 / norms ** 2, dtype=torch.float64, device=X.device),
+                 part=torch.zeros(4 * k * parts, dtype=torch.float64, device=X.device),
+                 gamma=torch.zeros(k, dtype=torch.float64, device=X.device),
+                 active=torch.ones(k, dtype=torch.int32, device=X.device))
+        callbacks = self.cbStep is not None or self.cbTrace is not None
+        steps = int(self.numMaxSteps)
+        self.numStep = 0
+        if callbacks:
+            for self.numStep in range(steps):
+                self._step(s)
+                active = s["active"].cpu().numpy() != 0
+                self.arrActive = active
+                if not active.any():
+                    break
+                self._expose(s, on_device)
+                self.handleCallback(self.cbStep)
+                self.handleCallback(self.cbTrace)
+        else:
+            self._step(s)
+            if steps > 1:
+                graph = torch.cuda.CUDAGraph()
+                cur = torch.cuda.current_stream()
+                side = torch.cuda.Stream()
+                side.wait_stream(cur)
+                with torch.cuda.stream(side):
+                    with torch.cuda.graph(graph, stream=side):
+                        self._step(s)
+                cur.wait_stream(side)
+                for self.numStep in range(1, steps):
+                    graph.replay()
+                del graph
+            self.arrActive = s["active"].cpu().numpy() != 0
+        self._expose(s, on_device)
+        return self.arrX
+
+    def _expose(self, s, on_device):
+        if on_device:
+            self.arrX = s["X"].t()
+            self.arrGamma = s["gamma"]
+        else:
+            self.arrX = np.ascontiguousarray(s["X"].cpu().numpy().T)
+            self.arrGamma = s["gamma"].cpu().numpy()

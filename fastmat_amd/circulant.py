@@ -194,6 +194,10 @@ class Circulant(_ConvBase):
         # (multi-dimensional) DFT: sigma_max = max |fftn(c)|
         return float(np.max(np.abs(np.fft.fftn(self._tenC.astype(np.complex128)))))
 
+    def _getColNorms(self):
+        # Circulant.pyx:229-230: every column holds all of tenC
+        return np.full((self._tenC.size,), np.linalg.norm(self._tenC))
+
     @property
     def tenC(self):
         return self._tenC
@@ -346,6 +350,11 @@ class Toeplitz(_ConvBase):
         return _multilevel_dense(self._tenT.astype(self.dtype), tuple(self._arrDimRows),
                                  tuple(self._arrDimCols))
 
+    def _getColNorms(self):
+        # the closed form of Toeplitz._normalizeColCore (Toeplitz.pyx:371-470)
+        return _multilevel_col_norms(self._tenT, tuple(self._arrDimRows),
+                                     tuple(self._arrDimCols))
+
 
 def _grid_selection(dims, Ls):
     """Flat indices (C order) of the grid prod(range(d_i)) inside the padded
@@ -378,3 +387,25 @@ def _multilevel_dense(t, rows, cols):
     J = [b.ravel()[None, :] for b in jdx]
     sub = tuple(np.mod(I[k] - J[k], t.shape[k]) for k in range(nd))
     return t[sub]
+
+
+def _multilevel_col_norms(t, rows, cols):
+    """Column norms of the matrix of `_multilevel_dense`: along level k,
+    column j_k holds the cyclic window of rows_k entries of t starting at
+    (-j_k) mod d_k, so the squared norm of column (j_1..j_d) is the sum of
+    |t|^2 over the product of those windows.  The window sums are formed one
+    level at a time from prefix sums (in extended precision)."""
+    a = np.abs(np.asarray(t)).astype(np.longdouble) ** 2
+    rows = tuple(int(r) for r in np.atleast_1d(rows))
+    cols = tuple(int(c) for c in np.atleast_1d(cols))
+    if a.ndim == 1 and len(rows) == 1:
+        a = a.reshape(-1)
+    for ax, r in enumerate(rows):
+        d = a.shape[ax]
+        ext = np.concatenate([a, np.take(a, np.arange(r), axis=ax)], axis=ax)
+        c = np.cumsum(ext, axis=ax)
+        c = np.concatenate([np.zeros_like(np.take(c, [0], axis=ax)), c], axis=ax)
+        s = np.arange(d)
+        a = np.take(c, s + r, axis=ax) - np.take(c, s, axis=ax)
+    idx = [np.mod(-np.arange(c), a.shape[k]) for k, c in enumerate(cols)]
+    return np.sqrt(np.maximum(a[np.ix_(*idx)].reshape(-1), 0)).astype(np.float64)

@@ -24,6 +24,10 @@ class Diag(Matrix):
     def _getLargestSingularValue(self):
         return float(np.max(np.abs(self._vecD))) if self.numRows else 0.0
 
+    def _getColNorms(self):
+        # Diag.pyx:126-127
+        return np.abs(self._vecD)
+
     @property
     def vecD(self):
         return self._vecD

@@ -12,6 +12,10 @@ class Eye(Matrix):
     def _getLargestSingularValue(self):
         return 1.0
 
+    def _getColNorms(self):
+        # Eye.pyx:92-93
+        return np.ones((self.numCols,), dtype=self.dtype)
+
     def _fwd(self, X):
         return X
 

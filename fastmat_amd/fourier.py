@@ -78,6 +78,10 @@ class Fourier(Matrix):
         # F^H F = N I (Fourier.pyx:188-189): every singular value is sqrt(N)
         return float(np.sqrt(self.numRows))
 
+    def _getColNorms(self):
+        # Fourier.pyx:176-177: every column has norm sqrt(N)
+        return np.full((self.numCols,), np.sqrt(self.numCols))
+
     @property
     def order(self):
         return self._order

@@ -42,6 +42,13 @@ class Kron(Matrix):
             out *= f.largestSingularValue
         return float(out)
 
+    def _getColNorms(self):
+        # Kron.pyx:165-172: the Kronecker product of the factors' norms
+        out = self._content[0].colNorms
+        for f in self._content[1:]:
+            out = np.kron(out, f.colNorms)
+        return out
+
     def _run(self, X, adjoint):
         k = X.shape[0]
         dims = self._dims

@@ -108,6 +108,60 @@ class Matrix:
             return float(np.linalg.svd(np.atleast_2d(self._array), compute_uv=False)[0])
         return power_iteration_sv(self)
 
+    @property
+    def colNorms(self):
+        """Euclidean norm of every column, float64 (Matrix.pyx:1031-1046),
+        computed once and kept."""
+        if getattr(self, "_colNorms", None) is None:
+            return self.getColNorms()
+        return self._colNorms
+
+    def getColNorms(self):
+        self._colNorms = self._getColNorms()
+        return self._colNorms
+
+    def _getColNorms(self):
+        """Matrix.pyx:1048-1087: the operator applied to slices of the
+        float64 identity, one column norm per output column.  The slices
+        are applied on the device, at most 1024 columns and about 2^24
+        output elements at a time; classes with a closed form override
+        this."""
+        n, m = self.numCols, self.numRows
+        out = np.empty(n, dtype=np.float64)
+        if n == 0:
+            return out
+        if m == 0:
+            out[:] = 0.0
+            return out
+        s = max(1, min(1024, (1 << 24) // m))
+        dev = D.device()
+        for i in range(0, n, s):
+            j = min(n, i + s)
+            E = torch.zeros(j - i, n, dtype=torch.float64, device=dev)
+            E[torch.arange(j - i, device=dev), torch.arange(i, j, device=dev)] = 1.0
+            F = self._apply(E, False)
+            out[i:j] = torch.linalg.vector_norm(F.to(torch.complex128)
+                                                if F.is_complex() else F.double(),
+                                                dim=1).cpu().numpy()
+        return out
+
+    @property
+    def scipyLinearOperator(self):
+        """scipy.sparse.linalg.LinearOperator over forward / backward
+        (Matrix.pyx:977-1006)."""
+        if getattr(self, "_scipyLinearOperator", None) is None:
+            return self.getScipyLinearOperator()
+        return self._scipyLinearOperator
+
+    def getScipyLinearOperator(self):
+        self._scipyLinearOperator = self._getScipyLinearOperator()
+        return self._scipyLinearOperator
+
+    def _getScipyLinearOperator(self):
+        from scipy.sparse.linalg import LinearOperator
+        return LinearOperator(shape=self.shape, matvec=self.forward, rmatvec=self.backward,
+                              matmat=self.forward, dtype=self.dtype)
+
     # ------------------------------------------------------------ basics
     @property
     def shape(self):

@@ -53,6 +53,14 @@ class Partial(Matrix):
     def colSelection(self):
         return self._colSelection
 
+    def _getColNorms(self):
+        # Partial.pyx:234-241: without a row selection the norms are the
+        # parent's, restricted to the selected columns
+        if self._rowSelection is not None:
+            return super()._getColNorms()
+        norms = self._content[0].colNorms
+        return norms if self._colSelection is None else norms[self._colSelection]
+
     def _idx(self, name, sel):
         t = self._dev.get(name)
         if t is None:

@@ -25,6 +25,10 @@ class Permutation(Matrix):
     def _getLargestSingularValue(self):
         return 1.0
 
+    def _getColNorms(self):
+        # Permutation.pyx:104-105
+        return np.ones((self.numCols,), dtype=self.dtype)
+
     @property
     def sigma(self):
         return self._sigma

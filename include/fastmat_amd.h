@@ -240,6 +240,36 @@ fm_status fm_shrink_step(int dtype, int64_t n, const void *y, const void *g, dou
                          void *stream);
 fm_status fm_counter_advance(int64_t *step_counter, void *stream);
 
+/* STELA (algorithms/STELA.py:128-267): soft thresholding with exact line
+ * search, one call per phase of a step, all on the stream, no allocation.
+ * The k x n iterate x, bx, dlt and the k x m residual res / forward f are of
+ * dtype_x (FM_FLOAT64 or FM_COMPLEX128, the reference's promote(float64, b));
+ * z (k x n, A^H res) is of dtype_z (equal to dtype_x, or FM_COMPLEX128 with a
+ * real iterate when the operator is complex).  d[n] = 1 / colNorms^2.
+ * partials is a 4 x k x parts float64 scratch; column sums are formed in two
+ * deterministic stages (parts blocks per column, then in order).
+ *   fm_stela_prox:   bx = soft(d x - z, lambda) / d, dlt = bx - x, and the
+ *                    partial sums of ||z - clamp(z - x, +-lambda)||^2 and of
+ *                    sum(|bx| - |x|) (STELA.py:179-211)
+ *   fm_stela_line:   partial sums of Re<res, f> and ||f||^2 for f = A dlt,
+ *                    then per column active = stop > max_error and the step
+ *                    gamma = clamp(-(Re<res,f> + lambda sum(|bx|-|x|)) /
+ *                    ||f||^2, 0, 1) (STELA.py:213-248)
+ *   fm_stela_update: on active columns x += gamma (bx - x), res += gamma f
+ *                    (STELA.py:250-257)
+ *   fm_stela_select: on active columns z = zn (zn = A^H res, STELA.py:258-261) */
+fm_status fm_stela_prox(int dtype_x, int dtype_z, int64_t n, int64_t k, const void *x,
+                        const void *z, const double *d, double lambda, void *bx, void *dlt,
+                        double *partials, int64_t parts, void *stream);
+fm_status fm_stela_line(int dtype_x, int64_t m, int64_t k, const void *res, const void *f,
+                        double *partials, int64_t parts, double lambda, double max_error,
+                        double *gamma, int32_t *active, void *stream);
+fm_status fm_stela_update(int dtype_x, int64_t n, int64_t m, int64_t k, const void *bx, void *x,
+                          const void *f, void *res, const double *gamma, const int32_t *active,
+                          void *stream);
+fm_status fm_stela_select(int dtype_z, int64_t n, int64_t k, void *z, const void *zn,
+                          const int32_t *active, void *stream);
+
 /* ------------------------------------------------------------ profiler
  * No counterpart in the reference.  For benchmarks: when enabled, each of the
  * library's kernel launches is bracketed by HIP events on its own stream and

@@ -137,3 +137,137 @@ def test_largest_singular_value(fm):
     with pytest.raises(TypeError):
         import fastmat_amd.algorithms as fma
         fma.ISTA(np.eye(3))
+
+
+# ------------------------------------------------------------------ STELA
+@pytest.mark.parametrize("typeA", [np.float32, np.float64, np.complex64, np.complex128])
+def test_stela_reference_configuration(fm, typeA):
+    """STELA.py:268-330: Product(Matrix(U(-100, 100)) 64 x 64, Hadamard(6),
+    typeExpansion=typeA), 6-sparse truth, lambda 0.1, 3 steps.  STELA works
+    in float64 / complex128, so the oracle gets the exact dense product."""
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(61)
+    say("STELA reference config", np.dtype(typeA).name)
+    M = rng.uniform(-100, 100, (64, 64)).astype(typeA)
+    A = fm.Product(fm.Matrix(M), fm.Hadamard(6), typeExpansion=typeA)
+    Ad = A.reference()
+    x0 = sparse_truth(rng, 64, 6, 2, typeA)
+    b = (Ad @ x0).astype(typeA)
+    out = fma.STELA(A, numLambda=0.1, numMaxSteps=3).process(b)
+    ref, _, _ = oalg.stela(Ad, b, num_lambda=0.1, num_max_steps=3)
+    assert out.shape == ref.shape and out.dtype == ref.dtype, (out.dtype, ref.dtype)
+    assert accept.max_rel_error(out, ref) < 1e-9
+
+
+@pytest.mark.parametrize("typeB", [np.float64, np.complex128])
+def test_stela_circulant_deconvolution(fm, typeB):
+    """The docstring example of STELA.py:41-66 (lambda 0.005, 100 steps),
+    real data and a complex variant."""
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(62)
+    n = 512
+    t = np.linspace(0, 20 * np.pi, n)
+    c = np.cos(2 * t) * np.exp(-t ** 2)
+    if typeB == np.complex128:
+        c = c + 0.5j * np.sin(3 * t) * np.exp(-t ** 2)
+    C = fm.Circulant(c)
+    x = np.zeros(n, dtype=typeB)
+    x[rng.choice(n, 3, replace=False)] = 1
+    b = C.reference() @ x
+    say("STELA deconvolution", np.dtype(typeB).name)
+    out = fma.STELA(C, numLambda=0.005, numMaxSteps=100).process(b)
+    ref, _, _ = oalg.stela(C.reference(), b, num_lambda=0.005, num_max_steps=100,
+                           col_norms_=C.colNorms)
+    assert out.shape == (n, 1) and out.dtype == ref.dtype
+    assert accept.max_rel_error(out, ref) < 1e-8
+
+
+def test_stela_complex_operator_real_data(fm):
+    """A complex operator with real measurements: z = A^H r is complex while
+    the iterate stays real (the reference keeps the real part on
+    assignment)."""
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(63)
+    F = fm.Fourier(64)
+    b = rng.standard_normal((64, 3))
+    say("STELA complex operator, real data")
+    out = fma.STELA(F, numLambda=0.5, numMaxSteps=6).process(b)
+    ref, _, _ = oalg.stela(F.reference(), b, num_lambda=0.5, num_max_steps=6)
+    assert out.dtype == np.float64 and ref.dtype == np.float64
+    assert accept.max_rel_error(out, ref) < 1e-9
+
+
+def test_stela_graph_equals_eager_and_early_stop(fm):
+    import torch
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(64)
+    n = 2048
+    c = rng.standard_normal(n)
+    C = fm.Circulant(c)
+    x0 = sparse_truth(rng, n, 10, 3, np.float64)
+    b = np.asarray(C.forward(x0))
+    say("STELA graph vs eager")
+    graphed = fma.STELA(C, numLambda=0.05, numMaxSteps=20).process(b)
+    seen = []
+    alg = fma.STELA(C, numLambda=0.05, numMaxSteps=20)
+    alg.cbStep = lambda a: seen.append(a.numStep)
+    eager = alg.process(b)
+    np.testing.assert_array_equal(graphed, eager)
+    ref, steps, _ = oalg.stela(C.reference(), b, num_lambda=0.05, num_max_steps=20,
+                               col_norms_=C.colNorms)
+    assert seen == list(range(steps))
+    assert accept.max_rel_error(graphed, ref) < 1e-8
+    # device in -> device out
+    bd = torch.from_numpy(np.ascontiguousarray(b.T)).cuda().t()
+    dev = fma.STELA(C, numLambda=0.05, numMaxSteps=20).process(bd)
+    assert torch.is_tensor(dev) and dev.is_cuda
+    np.testing.assert_array_equal(dev.cpu().numpy(), graphed)
+    # an identity problem converges in one step: the eager loop returns at
+    # step 1 before its callbacks, like the reference; the graph replays
+    # that follow change nothing
+    E = fm.Diag(np.ones(16))
+    bb = rng.standard_normal((16, 2))
+    seen.clear()
+    alg = fma.STELA(E, numLambda=0.5, numMaxSteps=10)
+    alg.cbStep = lambda a: seen.append(a.numStep)
+    out_e = alg.process(bb)
+    out_g = fma.STELA(E, numLambda=0.5, numMaxSteps=10).process(bb)
+    assert seen == [0]
+    np.testing.assert_array_equal(out_e, out_g)
+    np.testing.assert_allclose(out_g, oalg.soft_threshold(bb, 0.5), rtol=1e-15, atol=1e-16)
+
+
+def test_column_norms_and_scipy_operator(fm):
+    rng = np.random.default_rng(65)
+    say("colNorms / scipyLinearOperator")
+    cases = [
+        fm.Fourier(24), fm.Hadamard(5), fm.Circulant(rng.standard_normal(12)),
+        fm.Circulant(rng.standard_normal((3, 4))),
+        fm.Toeplitz(rng.standard_normal(7), rng.standard_normal(10)),
+        fm.Toeplitz(rng.standard_normal((5, 7)), split=np.array([3, 2])),
+        fm.Diag(rng.standard_normal(9)), fm.Eye(6),
+        fm.Permutation(rng.permutation(8)),
+        fm.Kron(fm.Fourier(4), fm.Diag(rng.standard_normal(3))),
+        fm.Partial(fm.Fourier(16), cols=np.array([1, 4, 9])),
+        fm.Partial(fm.Fourier(16), rows=np.array([0, 3, 5, 6])),
+        fm.Product(fm.Matrix(rng.standard_normal((10, 12))),
+                   fm.Matrix(rng.standard_normal((12, 7)))),
+        fm.Sum(fm.Matrix(rng.standard_normal((8, 8))), fm.Circulant(rng.standard_normal(8))),
+        fm.Matrix(rng.standard_normal((5, 3)) + 1j * rng.standard_normal((5, 3))),
+    ]
+    for A in cases:
+        ref = np.linalg.norm(A.reference().astype(np.complex128), axis=0)
+        np.testing.assert_allclose(np.asarray(A.colNorms, dtype=np.float64), ref,
+                                   rtol=1e-6 if A.dtype in (np.float32, np.complex64)
+                                   else 1e-12, err_msg=repr(A))
+    # scipy LinearOperator over forward / backward (Matrix.pyx:998-1006)
+    C = fm.Circulant(rng.standard_normal(16) + 1j * rng.standard_normal(16))
+    L = C.scipyLinearOperator
+    v = rng.standard_normal(16) + 1j * rng.standard_normal(16)
+    V = rng.standard_normal((16, 3))
+    Cd = C.reference()
+    assert L.shape == (16, 16) and L.dtype == C.dtype
+    assert accept.max_rel_error(L.matvec(v), Cd @ v) < 1e-5
+    assert accept.max_rel_error(L.rmatvec(v), Cd.conj().T @ v) < 1e-5
+    assert accept.max_rel_error(L.matmat(V), Cd @ V) < 1e-5
+    assert C.scipyLinearOperator is L

@@ -123,3 +123,18 @@ def test_multilevel_dtype_follows_generator(fm):
     for it in (np.int8, np.int16, np.int32):
         T = fm.Toeplitz(np.ones((3, 5), it), split=np.array([2, 3]))
         assert T.dtype == np.promote_types(it, np.complex64)
+
+
+@pytest.mark.parametrize("shape,rows,cols", [
+    ((9,), (4,), (6,)), ((7,), (1,), (7,)), ((7,), (7,), (1,)),
+    ((5, 7), (3, 2), (3, 6)), ((3, 3, 41), (2, 1, 4), (2, 3, 38)), ((4, 6), (4, 6), (4, 6)),
+])
+def test_multilevel_column_norms_match_dense(shape, rows, cols):
+    # the window-sum closed form of the Toeplitz / Circulant column norms
+    # against the norms of the dense multi-level matrix
+    from fastmat_amd.circulant import _multilevel_col_norms, _multilevel_dense
+    rng = np.random.default_rng(sum(shape))
+    t = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    M = _multilevel_dense(t, rows, cols)
+    np.testing.assert_allclose(_multilevel_col_norms(t, rows, cols),
+                               np.linalg.norm(M, axis=0), rtol=1e-14)

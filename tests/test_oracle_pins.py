@@ -469,3 +469,62 @@ def test_ista_objective_decreases_and_fista_betas():
         t = t1
     # the oracle's sigma is numpy's SVD value
     assert abs(oalg.largest_singular_value(np.diag([3.0, -5.0, 1.0])) - 5.0) < 1e-14
+
+
+def test_stela_identity_one_step_and_stop():
+    # A = I: d = 1, z = -b, one step gives soft(b, lambda) (gamma = 1), after
+    # which z - clamp(z - x) = 0 for every column: the loop returns at step 1
+    b = np.array([[2.0, 0.1], [-0.1, -3.0], [0.7, 0.0]])
+    x, steps, gamma = oalg.stela(np.eye(3), b, num_lambda=0.5, num_max_steps=10)
+    np.testing.assert_allclose(x, oalg.soft_threshold(b, 0.5), rtol=1e-15, atol=1e-16)
+    assert steps == 1
+    np.testing.assert_allclose(gamma, [1.0, 1.0])
+
+
+def test_stela_unitary_complex_one_step():
+    # a unitary A (normalised DFT): ||A x - b|| = ||x - A^H b||, so the first
+    # step is soft(A^H b, lambda); a dropped conjugate or transpose fails
+    n = 16
+    F = np.fft.fft(np.eye(n)) / np.sqrt(n)
+    rng = np.random.default_rng(51)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x, _, gamma = oalg.stela(F, b, num_lambda=0.3, num_max_steps=1)
+    np.testing.assert_allclose(x[:, 0], oalg.soft_threshold(F.conj().T @ b, 0.3), atol=1e-14)
+    np.testing.assert_allclose(gamma, [1.0], rtol=1e-13)
+    # a fixed point after that (the stopping value clamps real and imaginary
+    # parts separately, so for complex data it need not reach zero)
+    x5, _, _ = oalg.stela(F, b, num_lambda=0.3, num_max_steps=5)
+    np.testing.assert_allclose(x5, x, atol=1e-13)
+
+
+def test_stela_single_column_closed_form():
+    # one column a with ||a|| >= 1: the exact line search lands on the
+    # minimiser of 0.5 ||a x - b||^2 + lambda |x|, soft(a^H b, lambda) / ||a||^2,
+    # in the first step (gamma = 1 / ||a||^4 along bx = soft(a^H b) ||a||^2)
+    rng = np.random.default_rng(52)
+    a = (rng.standard_normal(7) + 1j * rng.standard_normal(7)).reshape(7, 1)
+    b = rng.standard_normal(7) + 1j * rng.standard_normal(7)
+    lam = 0.2
+    x, _, gamma = oalg.stela(a, b, num_lambda=lam, num_max_steps=1)
+    na2 = np.sum(np.abs(a) ** 2)
+    g = np.vdot(a[:, 0], b)
+    np.testing.assert_allclose(x[0, 0], oalg.soft_threshold(g, lam) / na2, rtol=1e-13)
+    np.testing.assert_allclose(gamma[0], 1 / na2 ** 2, rtol=1e-13)
+
+
+def test_stela_objective_monotone():
+    # the step minimises an upper bound of 0.5||Ax-b||^2 + lambda||x||_1
+    # that is tight at gamma = 0, so the objective never increases
+    rng = np.random.default_rng(53)
+    for cplx in (False, True):
+        A = rng.standard_normal((12, 30))
+        b = rng.standard_normal((12, 2))
+        if cplx:
+            A = A + 1j * rng.standard_normal((12, 30))
+            b = b + 1j * rng.standard_normal((12, 2))
+        prev = oalg.stela_objective(A, b, np.zeros((30, 2)), 0.2)
+        for s in range(1, 25):
+            x, _, _ = oalg.stela(A, b, num_lambda=0.2, num_max_steps=s, num_max_error=0.0)
+            obj = oalg.stela_objective(A, b, x, 0.2)
+            assert np.all(obj <= prev + 1e-12), (cplx, s)
+            prev = obj
