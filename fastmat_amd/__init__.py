@@ -16,11 +16,12 @@ from .fourier import Fourier
 from .circulant import Circulant, Toeplitz
 from .hadamard import Hadamard
 from .kron import Kron
+from .blocks import BlockDiag, Blocks
 
 __version__ = "0.1.0"
 
 __all__ = [
     "Matrix", "Hermitian", "Conjugate", "Transpose", "flags", "Product", "Sum",
     "Partial", "Diag", "Eye", "Permutation", "Fourier", "Circulant", "Toeplitz",
-    "Hadamard", "Kron",
+    "Hadamard", "Kron", "BlockDiag", "Blocks",
 ]

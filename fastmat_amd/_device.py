@@ -144,6 +144,13 @@ def stream():
     return _lib.stream_handle(device())
 
 
+def ld(t):
+    """Leading dimension (elements between rows) of a (k, w) view with unit
+    element stride; a single row may report any row stride, so its width is
+    used."""
+    return int(t.shape[1]) if t.shape[0] == 1 else int(t.stride(0))
+
+
 def workspace(nbytes):
     """Device scratch of at least nbytes (uint8 tensor; caller keeps it alive
     until the stream has consumed it, which torch's caching allocator

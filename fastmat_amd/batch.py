@@ -172,6 +172,36 @@ class ConvBatch:
         self._keep = getattr(self, "_keep", {})
         self._keep[int(adjoint)] = (list(ins), list(outs))
 
+    def bind_views(self, ins, outs, adjoint=False):
+        """Bind column slices of wider blocks (unit element stride, row
+        stride >= width) without copying them; the rows' stride is passed as
+        each operator's leading dimension.  Used by BlockDiag."""
+        key = tuple((t.data_ptr(), D.ld(t)) for t in ins) + \
+            tuple((t.data_ptr(), D.ld(t)) for t in outs)
+        if self._bound[int(adjoint)] == key:
+            return
+        for i, (op, k) in enumerate(zip(self.ops, self.ks)):
+            nin = op.numRows if adjoint else op.numCols
+            mout = op.numCols if adjoint else op.numRows
+            if tuple(ins[i].shape) != (k, nin) or tuple(outs[i].shape) != (k, mout):
+                raise ValueError("ConvBatch: view %d has the wrong shape" % i)
+            for t, w in ((ins[i], nin), (outs[i], mout)):
+                if t.stride(1) != 1 or D.ld(t) < w:
+                    raise ValueError("ConvBatch: view %d is not a column slice" % i)
+                if D.np_dtype(t.dtype) != self.dtype:
+                    raise TypeError("ConvBatch: buffers must be %s" % self.dtype)
+        n = len(self.ops)
+        pin = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ins])
+        pout = (ctypes.c_void_p * n)(*[t.data_ptr() for t in outs])
+        ldin = (ctypes.c_int64 * n)(*[D.ld(t) for t in ins])
+        ldout = (ctypes.c_int64 * n)(*[D.ld(t) for t in outs])
+        ws = None if self._ws is None else _lib.ptr(self._ws)
+        _lib.call("fm_batch_bind", self._h, int(adjoint), pin, ldin, pout, ldout, ws,
+                  self._ws_bytes, D.stream())
+        # no reference to the views is kept: the binding is reused only
+        # while the same addresses are handed in again
+        self._bound[int(adjoint)] = key
+
     def set_grid(self, workgroups_per_cu):
         _lib.call("fm_batch_set_grid", self._h, int(workgroups_per_cu))
         self.grid_per_cu = int(workgroups_per_cu)

@@ -82,6 +82,19 @@ class _ConvBase(Matrix):
                   None if ws is None else _lib.ptr(ws), need.value, D.stream())
         return Y
 
+    def _apply_views(self, X, Y, adjoint):
+        cdt = self.compute_dtype(X.dtype)
+        if self._multi is not None or D.np_dtype(X.dtype) != cdt or D.np_dtype(Y.dtype) != cdt:
+            return False
+        h = self._plans.get(precision_of(cdt))
+        k = X.shape[0]
+        need = ctypes.c_int64()
+        _lib.call("fm_conv_workspace_size", h, k, ctypes.byref(need))
+        ws = D.workspace(need.value)
+        _lib.call("fm_conv_apply", h, int(adjoint), _lib.ptr(X), D.ld(X), _lib.ptr(Y),
+                  D.ld(Y), k, None if ws is None else _lib.ptr(ws), need.value, D.stream())
+        return True
+
     def _getArray(self):
         # the dense array is the closed form (Circulant.pyx:241-242,
         # Toeplitz.pyx:325-326)

@@ -49,6 +49,15 @@ class Diag(Matrix):
                   _lib.ptr(Y), n, k, D.stream())
         return Y
 
+    def _apply_views(self, X, Y, adjoint):
+        dt = D.promote(X.dtype, self.dtype)
+        if D.np_dtype(X.dtype) != dt or D.np_dtype(Y.dtype) != dt:
+            return False
+        k, n = X.shape
+        _lib.call("fm_diag_apply", D.fm_type(dt), int(adjoint), n, _lib.ptr(self._d(dt)),
+                  _lib.ptr(X), D.ld(X), _lib.ptr(Y), D.ld(Y), k, D.stream())
+        return True
+
     def _fwd(self, X):
         return self._run(X, False)
 

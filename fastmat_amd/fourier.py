@@ -123,6 +123,19 @@ class Fourier(Matrix):
                   None if ws is None else _lib.ptr(ws), need.value, D.stream())
         return Y
 
+    def _apply_views(self, X, Y, adjoint):
+        cdt = self.compute_dtype(X.dtype)
+        if D.np_dtype(X.dtype) != cdt or D.np_dtype(Y.dtype) != cdt:
+            return False
+        h = self._plans.get(precision_of(cdt))
+        k, n = X.shape
+        need = ctypes.c_int64()
+        _lib.call("fm_fourier_workspace_size", h, k, ctypes.byref(need))
+        ws = D.workspace(need.value)
+        _lib.call("fm_fourier_apply", h, int(adjoint), _lib.ptr(X), D.ld(X), _lib.ptr(Y),
+                  D.ld(Y), k, None if ws is None else _lib.ptr(ws), need.value, D.stream())
+        return True
+
     def _fwd(self, X):
         return self._run(X, False)
 

@@ -54,6 +54,14 @@ class Hadamard(Matrix):
 
     _bwd = _fwd
 
+    def _apply_views(self, X, Y, adjoint):
+        dt = D.promote(X.dtype, self.dtype)
+        if D.np_dtype(X.dtype) != dt or D.np_dtype(Y.dtype) != dt:
+            return False
+        _lib.call("fm_hadamard_apply", D.fm_type(dt), self._order, _lib.ptr(X), D.ld(X),
+                  _lib.ptr(Y), D.ld(Y), X.shape[0], D.stream())
+        return True
+
     def _apply_strided(self, X, adjoint, outer, inner):
         Y = X.new_empty(X.shape)
         _lib.call("fm_hadamard_apply_strided", D.fm_type(X.dtype), self._order, _lib.ptr(X),

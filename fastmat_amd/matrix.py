@@ -212,6 +212,24 @@ class Matrix:
             X = X.to(D.torch_dtype(fin))
         return self._bwd(X) if adjoint else self._fwd(X)
 
+    def _apply_into(self, X, Y, adjoint):
+        """Write the transform of the (k, n) block X into the (k, m) block Y.
+        X and Y may be column slices of wider blocks (unit element stride,
+        row stride >= width), as the block compositions hand them over; Y has
+        the composition's output dtype.  Classes whose kernels take leading
+        dimensions override this to read and write the slices in place; the
+        generic form applies to a packed copy and copies the result in."""
+        if _views_direct(X, Y) and D.np_dtype(Y.dtype) == self._out_dtype(X.dtype) and \
+                self._apply_views(X, Y, adjoint):
+            return
+        T = self._apply(X if X.is_contiguous() else X.contiguous(), adjoint)
+        Y.copy_(T)
+
+    def _apply_views(self, X, Y, adjoint):
+        """Leading-dimension apply of X into Y (same dtypes as _apply would
+        produce); returns False when the class has none."""
+        return False
+
     # dense implementation (subclasses override)
     def _fwd(self, X):
         return _dense_apply(self, X, False)
@@ -357,6 +375,15 @@ class Matrix:
 
     def _reference(self):
         return self._array
+
+
+def _views_direct(X, Y):
+    """X, Y are row-major (k, w) views with unit element stride whose rows
+    are at least their width apart (a column slice of a wider block)."""
+    return (X.dim() == 2 and Y.dim() == 2 and X.shape[0] == Y.shape[0] and
+            X.shape[0] > 0 and X.shape[1] > 0 and Y.shape[1] > 0 and
+            X.stride(1) == 1 and Y.stride(1) == 1 and
+            D.ld(X) >= X.shape[1] and D.ld(Y) >= Y.shape[1])
 
 
 def _realified(A):

@@ -160,3 +160,35 @@ def partial(ref, rows=None, cols=None):
     if cols is not None:
         a = a[:, np.asarray(cols)]
     return a
+
+
+def block_diag(refs):
+    """BlockDiag reference: the blocks along the diagonal, zeros elsewhere
+    (BlockDiag.pyx:165-181)."""
+    refs = [np.asarray(r) for r in refs]
+    dt = np.result_type(*refs)
+    res = np.zeros((sum(r.shape[0] for r in refs), sum(r.shape[1] for r in refs)), dtype=dt)
+    i = j = 0
+    for r in refs:
+        res[i:i + r.shape[0], j:j + r.shape[1]] = r
+        i += r.shape[0]
+        j += r.shape[1]
+    return res
+
+
+def blocks(grid):
+    """Blocks reference: block (r, c) of the result is grid[r][c]
+    (Blocks.pyx:291-312)."""
+    grid = [[np.asarray(t) for t in row] for row in grid]
+    dt = np.result_type(*[t for row in grid for t in row])
+    heights = [row[0].shape[0] for row in grid]
+    widths = [t.shape[1] for t in grid[0]]
+    res = np.zeros((sum(heights), sum(widths)), dtype=dt)
+    i = 0
+    for row, h in zip(grid, heights):
+        j = 0
+        for t, w in zip(row, widths):
+            res[i:i + h, j:j + w] = t
+            j += w
+        i += h
+    return res

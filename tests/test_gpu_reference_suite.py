@@ -208,3 +208,53 @@ def test_partial(fm, ci, ri):
     P = fm.Partial(fm.Hadamard(4), rows=SUBROWS[ri], cols=SUBCOLS[ci])
     ref = dense.partial(dense.hadamard(4, np.int8), SUBROWS[ri], SUBCOLS[ci])
     check_transforms(P, ref, 1.0, name="Partial")
+
+
+@pytest.mark.parametrize("t1", ALLTYPES)
+@pytest.mark.parametrize("t2", FEWTYPES)
+def test_blockdiag(fm, t1, t2):
+    """BlockDiag.pyx:192-230: Matrix(arr1), -2 Matrix(arr2), 2 Matrix(arr3),
+    5 x 5 each, TOL_POWER 3."""
+    say("reference suite BlockDiag", np.dtype(t1).name, np.dtype(t2).name)
+    a1, a2, a3 = test_dist((5, 5), t1), test_dist((5, 5), t2), test_dist((5, 5), t1)
+    B = fm.BlockDiag(fm.Matrix(a1), -2. * fm.Matrix(a2), 2. * fm.Matrix(a3))
+    ref = dense.block_diag([a1, dense.product([a2], scalar=-2.), dense.product([a3], scalar=2.)])
+    check_transforms(B, ref, 3.0, name="BlockDiag")
+
+
+@pytest.mark.parametrize("t1", ALLTYPES)
+@pytest.mark.parametrize("t2", FEWTYPES)
+def test_blocks(fm, t1, t2):
+    """Blocks.pyx:314-352: [[Matrix(arr1), Matrix(arr2)], [Matrix(arr3),
+    Matrix(arr4)]], 4 x 4 each, types (t1, t2; t2, t1)."""
+    say("reference suite Blocks", np.dtype(t1).name, np.dtype(t2).name)
+    a1, a2, a3, a4 = (test_dist((4, 4), t1), test_dist((4, 4), t2), test_dist((4, 4), t2),
+                      test_dist((4, 4), t1))
+    B = fm.Blocks([[fm.Matrix(a1), fm.Matrix(a2)], [fm.Matrix(a3), fm.Matrix(a4)]])
+    check_transforms(B, dense.blocks([[a1, a2], [a3, a4]]), 1.0, name="Blocks")
+
+
+@pytest.mark.parametrize("c", [64, 100])
+def test_block_benchmark_compositions(fm, c):
+    """The reference's benchmark compositions (BlockDiag.pyx:232-252,
+    Blocks.pyx:354-372): structured blocks, so the 1-D Circulant blocks of the
+    BlockDiag run through one batch bound to slices of the caller's block and
+    the others through their leading-dimension applies."""
+    say("block benchmark compositions", c)
+    rng = np.random.default_rng(c)
+    c1, c2, d = rng.standard_normal(c), rng.standard_normal(c), rng.standard_normal(c)
+    B = fm.BlockDiag(fm.Circulant(c1), fm.Circulant(c2), fm.Fourier(c), fm.Diag(d))
+    ref = dense.block_diag([dense.circulant(c1), dense.circulant(c2), dense.fourier(c),
+                            dense.diag(d)])
+    check_transforms(B, ref, 3.0, name="BlockDiag-structured")
+    G = fm.Blocks([[fm.Circulant(c1), fm.Circulant(c2)], [fm.Fourier(c), fm.Diag(d)]])
+    ref = dense.blocks([[dense.circulant(c1), dense.circulant(c2)],
+                        [dense.fourier(c), dense.diag(d)]])
+    check_transforms(G, ref, 3.0, name="Blocks-structured")
+    # mixed: Toeplitz and Hadamard blocks, nested BlockDiag, a Blocks term
+    T = fm.Toeplitz(rng.standard_normal(5), rng.standard_normal(8))
+    H = fm.Hadamard(3)
+    N = fm.BlockDiag(T, fm.BlockDiag(H, fm.Circulant(c1)), G)
+    refN = dense.block_diag([dense.toeplitz(T.tenT[:5], T.tenT[5:]), dense.hadamard(3),
+                             dense.circulant(c1), ref])
+    check_transforms(N, refN, 3.0, name="BlockDiag-nested")

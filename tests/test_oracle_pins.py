@@ -528,3 +528,17 @@ def test_stela_objective_monotone():
             obj = oalg.stela_objective(A, b, x, 0.2)
             assert np.all(obj <= prev + 1e-12), (cplx, s)
             prev = obj
+
+
+def test_block_compositions_against_scipy_and_numpy():
+    import scipy.linalg
+    rng = np.random.default_rng(71)
+    a = rng.standard_normal((3, 4))
+    b = (rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))
+    c = rng.integers(-3, 4, (5, 1)).astype(np.int16)
+    np.testing.assert_array_equal(dense.block_diag([a, b, c]), scipy.linalg.block_diag(a, b, c))
+    g = [[a, rng.standard_normal((3, 2))], [rng.standard_normal((1, 4)), b[:1, :]]]
+    np.testing.assert_array_equal(dense.blocks(g), np.block(g))
+    # a block diagonal matrix is the grid with zero off-diagonal blocks
+    z = [[a, np.zeros((3, 2))], [np.zeros((2, 4)), b]]
+    np.testing.assert_array_equal(dense.block_diag([a, b]), dense.blocks(z))
