@@ -17,11 +17,12 @@ from .circulant import Circulant, Toeplitz
 from .hadamard import Hadamard
 from .kron import Kron
 from .blocks import BlockDiag, Blocks
+from .lfsr import LFSRCirculant
 
 __version__ = "0.1.0"
 
 __all__ = [
     "Matrix", "Hermitian", "Conjugate", "Transpose", "flags", "Product", "Sum",
     "Partial", "Diag", "Eye", "Permutation", "Fourier", "Circulant", "Toeplitz",
-    "Hadamard", "Kron", "BlockDiag", "Blocks",
+    "Hadamard", "Kron", "BlockDiag", "Blocks", "LFSRCirculant",
 ]

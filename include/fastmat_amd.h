@@ -240,6 +240,17 @@ fm_status fm_shrink_step(int dtype, int64_t n, const void *y, const void *g, dou
                          void *stream);
 fm_status fm_counter_advance(int64_t *step_counter, void *stream);
 
+/* LFSRCirculant (LFSRCirculant.pyx:159-237): host-side register sequences.
+ * Runs the generating register from `start` (masked to `order` bits) until
+ * it returns, with the reference's checks (order 1..31, non-zero start, a
+ * sequence that repeats its start state; LFSRCirculant.pyx:196-222), and
+ * returns the period.  With states/taps non-NULL (capacity >= period) it
+ * also writes states[t] = the state after t generator steps and taps[t] =
+ * the tap-side state after t steps from 1, the scatter and gather indices
+ * of the Hadamard-based apply (LFSRCirculant.pyx:316-396). */
+fm_status fm_lfsr_sequence(uint32_t polynomial, uint32_t start, int order, int64_t *period,
+                           int64_t *states, int64_t *taps, int64_t capacity);
+
 /* STELA (algorithms/STELA.py:128-267): soft thresholding with exact line
  * search, one call per phase of a step, all on the stream, no allocation.
  * The k x n iterate x, bx, dlt and the k x m residual res / forward f are of

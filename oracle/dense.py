@@ -192,3 +192,33 @@ def blocks(grid):
             j += w
         i += h
     return res
+
+
+def lfsr_sequence(polynomial, start):
+    """LFSRCirculant's register sequence in pure Python (the reference's
+    _reference loop, LFSRCirculant.pyx:409-430): order from the polynomial's
+    highest bit, output +1 for an even state and -1 for an odd one, next state
+    = (state | mask * parity(state & polynomial)) >> 1, one period."""
+    order, mask = 0, 1
+    while (~mask & polynomial) > mask:
+        mask <<= 1
+        order += 1
+    mask = 1 << order
+    state = first = start & (mask - 1)
+    out = []
+    while True:
+        out.append(-1 if state & 1 else 1)
+        par = bin(state & polynomial).count("1") & 1
+        if par:
+            state |= mask
+        state >>= 1
+        if state == first or len(out) >= mask:
+            break
+    return np.array(out, dtype=np.int8)
+
+
+def lfsr_circulant(polynomial, start):
+    """LFSRCirculant reference: column j is the sequence rolled by j
+    (LFSRCirculant.pyx:431-438)."""
+    s = lfsr_sequence(polynomial, start)
+    return np.stack([np.roll(s, j) for j in range(s.size)], axis=1)

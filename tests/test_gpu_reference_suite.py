@@ -258,3 +258,39 @@ def test_block_benchmark_compositions(fm, c):
     refN = dense.block_diag([dense.toeplitz(T.tenT[:5], T.tenT[5:]), dense.hadamard(3),
                              dense.circulant(c1), ref])
     check_transforms(N, refN, 3.0, name="BlockDiag-nested")
+
+
+@pytest.mark.parametrize("poly", [0x19, 0x17])
+@pytest.mark.parametrize("start", [0xD, 0x1])
+def test_lfsr_circulant(fm, poly, start):
+    """LFSRCirculant.pyx:440-458: polynomials 0x19 (period 15) and 0x17
+    (period 7), starts 0xD and 0x1."""
+    say("reference suite LFSRCirculant", hex(poly), hex(start))
+    L = fm.LFSRCirculant(poly, start)
+    check_transforms(L, dense.lfsr_circulant(poly, start), 1.0, name="LFSRCirculant")
+
+
+def test_lfsr_circulant_order_20(fm):
+    """The benchmark's order-20 register (polynomial 0x120001, start
+    0xFFFFFFFF, N = 2^20 - 1): integer data exactly against the circular
+    convolution (float64 FFT, rounded: all sums are integers below 2^22),
+    float32 within 1e-5 of it, forward and backward."""
+    import torch
+    L = fm.LFSRCirculant(0x120001, 0xFFFFFFFF)
+    n = L.numRows
+    assert n == 2 ** 20 - 1
+    s = dense.lfsr_sequence(0x120001, 0xFFFFFFFF).astype(np.float64)
+    S = np.fft.fft(s)
+    rng = np.random.default_rng(77)
+    x = rng.choice([-2, -1, 1, 2], (n, 3)).astype(np.int32)
+    say("LFSRCirculant order 20")
+    fwd = np.real(np.fft.ifft(S[:, None] * np.fft.fft(x.astype(np.float64), axis=0), axis=0))
+    bwd = np.real(np.fft.ifft(np.conj(S)[:, None] * np.fft.fft(x.astype(np.float64), axis=0),
+                              axis=0))
+    y = L.forward(x)
+    assert y.dtype == np.int32
+    np.testing.assert_array_equal(y, np.rint(fwd).astype(np.int32))
+    np.testing.assert_array_equal(L.backward(x), np.rint(bwd).astype(np.int32))
+    xf = x.astype(np.float32)
+    yf = L.forward(torch.from_numpy(np.ascontiguousarray(xf.T)).cuda().t())
+    assert accept.max_rel_error(yf.cpu().numpy(), fwd) < 1e-5

@@ -138,3 +138,26 @@ def test_multilevel_column_norms_match_dense(shape, rows, cols):
     M = _multilevel_dense(t, rows, cols)
     np.testing.assert_allclose(_multilevel_col_norms(t, rows, cols),
                                np.linalg.norm(M, axis=0), rtol=1e-14)
+
+
+def test_lfsr_host_sequences_and_errors():
+    """LFSRCirculant's constructor runs the library's host-side register
+    routine (no device work): periods, sequence and the reference's errors
+    (LFSRCirculant.pyx:196-222)."""
+    import fastmat_amd as fm
+    L = fm.LFSRCirculant(0b11001, 0b1010)
+    assert L.shape == (15, 15) and L.order == 4 and L.dtype == np.int8
+    np.testing.assert_array_equal(L.vecC, dense.lfsr_sequence(0b11001, 0b1010))
+    assert L.states.dtype == np.uint32 and L.states[0] == 0b1010
+    np.testing.assert_array_equal(L.reference(), dense.lfsr_circulant(0b11001, 0b1010))
+    assert fm.LFSRCirculant(0x17, 0xD).period == 7
+    big = fm.LFSRCirculant(0x120001, 0xFFFFFFFF)
+    assert big.period == 2 ** 20 - 1
+    with pytest.raises(ValueError, match="order 1 to 31"):
+        fm.LFSRCirculant(0x1, 0x1)
+    with pytest.raises(ValueError, match="non-zero"):
+        fm.LFSRCirculant(0x19, 0x10)
+    with pytest.raises(ValueError, match="invalid sequence"):
+        fm.LFSRCirculant(0x10, 0x3)
+    with pytest.warns(FutureWarning):
+        assert L.taps == 0b11001

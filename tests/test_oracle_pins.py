@@ -542,3 +542,32 @@ def test_block_compositions_against_scipy_and_numpy():
     # a block diagonal matrix is the grid with zero off-diagonal blocks
     z = [[a, np.zeros((3, 2))], [np.zeros((2, 4)), b]]
     np.testing.assert_array_equal(dense.block_diag([a, b]), dense.blocks(z))
+
+
+def test_lfsr_docstring_sequence_and_periods():
+    # the sequence printed in LFSRCirculant's docstring (LFSRCirculant.pyx:
+    # 95-112) for polynomial 0b11001, start 0b1010
+    s = dense.lfsr_sequence(0b11001, 0b1010)
+    np.testing.assert_array_equal(
+        s, [+1, -1, +1, -1, -1, +1, +1, -1, +1, +1, +1, -1, -1, -1, -1])
+    # the periods its test table states (LFSRCirculant.pyx:440-452)
+    assert dense.lfsr_sequence(0x19, 0xD).size == 15
+    assert dense.lfsr_sequence(0x17, 0x1).size == 7
+    # the benchmark's polynomials are maximal: period 2^c - 1
+    # (LFSRCirculant.pyx:460-480)
+    db = [0x3, 0x7, 0xB, 0x19, 0x25, 0x61, 0x89, 0x1C3, 0x211, 0x481, 0xB03, 0x1C11]
+    for c, p in enumerate(db, start=1):
+        if c == 1:
+            continue
+        assert dense.lfsr_sequence(p, 0xFFFFFFFF).size == 2 ** c - 1, hex(p)
+
+
+def test_lfsr_circulant_structure():
+    # circulant: C[i, j] = s[(i - j) mod N]; an m-sequence has the two-valued
+    # autocorrelation N, -1, so C^T C = (N + 1) I - J
+    C = dense.lfsr_circulant(0x19, 0xD).astype(np.int64)
+    s = dense.lfsr_sequence(0x19, 0xD).astype(np.int64)
+    n = s.size
+    i = np.arange(n)
+    np.testing.assert_array_equal(C, s[(i[:, None] - i[None, :]) % n])
+    np.testing.assert_array_equal(C.T @ C, (n + 1) * np.eye(n, dtype=np.int64) - 1)
