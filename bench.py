@@ -10,7 +10,11 @@ data.  ``value`` is whole-job throughput in GB/s over algorithmic bytes
 
 Other workloads (``--config``): hadamard (order 20, 64 columns, int32 and
 float32), circulant (N=4096, 32 columns, complex128), kron (Fourier(64) x
-Circulant(128) x dense 128x128, 16 columns, complex64).
+Circulant(128) x dense 128x128, 16 columns, complex64), batch (256 uneven
+Circulant/Toeplitz operators), multilevel (two-level Circulant 1024 x 1024,
+16 columns), fista (50 FISTA iterations on a Circulant of 2^20, 8 columns),
+blockdiag (Circulant, Circulant, Fourier, Diag blocks of 2^18, 32 columns),
+lfsr (LFSRCirculant of order 20, 64 columns float32).
 
 The JSON line also carries
   roofline      dominant kernel: algorithmic bytes per launch / average launch
@@ -42,7 +46,8 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="fourier",
-                   choices=["fourier", "hadamard", "circulant", "kron", "batch", "copy"])
+                   choices=["fourier", "hadamard", "circulant", "kron", "batch", "copy",
+                            "multilevel", "fista", "blockdiag", "lfsr"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cols", type=int, default=None, help="override the column count")
     return p.parse_args()
@@ -349,8 +354,199 @@ def workload_copy(args, rng):
                 dtype="f32", metric="device copy GB/s (calibration)")
 
 
+def workload_multilevel(args, rng):
+    """Two-level Circulant, 1024 x 1024 generator (N = 2^20), 16 columns."""
+    import torch
+    import fastmat_amd as fm
+    k = args.cols or 16
+    t = crandn(rng, (1024, 1024))
+    C = fm.Circulant(t)
+    N = 1 << 20
+    X = device_block(crandn(rng, (N, k)), torch.complex64)
+    bytes_step = 2 * (2 * N * k * 8 + N * 8)
+
+    def step():
+        return [C._apply(X, False), C._apply(X, True)]
+
+    def check():
+        from oracle import accept
+        x = X[0].cpu().numpy().astype(np.complex128).reshape(1024, 1024)
+        ref = np.fft.ifft2(np.fft.fft2(t.astype(np.complex128)) * np.fft.fft2(x))
+        return {"fwd_vs_fp64_fft2": accept.max_rel_error(
+            C._apply(X[:1].contiguous(), False).cpu().numpy().reshape(1024, 1024), ref)}
+
+    def cpu():
+        from oracle import cpu_path
+        op = cpu_path.CirculantMultilevelCPU(t)
+        x = crandn(np.random.default_rng(7), (N, 2))
+        return [(op, x, 2 * (2 * N * 2 * 8 + N * 8))], \
+            "two-level Circulant 1024x1024 x 2 cols (Kron-of-Fourier path), fwd+adj"
+
+    cfg = {"workload": "circulant_2level_1024x1024_c64_fwd_adj", "N": N, "cols": k}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=cpu, cfg=cfg,
+                dtype="c64", metric=METRIC)
+
+
+def workload_fista(args, rng):
+    """FISTA (50 iterations per step) on a Circulant of N = 2^20 with a real
+    generator (complex64 operator), 8 measurement columns, device-resident
+    and graph-replayed."""
+    import torch
+    import fastmat_amd as fm
+    from fastmat_amd import algorithms as fma
+    k = args.cols or 8
+    N = 1 << 20
+    iters = 50
+    tt = np.linspace(0, 40 * np.pi, N)
+    c = (np.cos(2 * tt) * np.exp(-((tt - 20 * np.pi) / 4) ** 2)).astype(np.float32)
+    C = fm.Circulant(c)
+    x0 = np.zeros((N, k), dtype=np.float32)
+    for j in range(k):
+        x0[rng.choice(N, 64, replace=False), j] = 1
+    B = C._apply(device_block(x0, torch.float32), False).contiguous()  # (k, N) complex64
+    Bv = B.t()  # (N, k) device view, the solver's device input
+    _ = C.largestSingularValue
+    # per iteration: forward and adjoint conv (input, output, spectrum),
+    # residual (read 2, write 1) and the fused shrink (read 3, write 3)
+    bytes_iter = N * k * 8 * (2 + 3 + 2 + 6) + 2 * C.fftLength * 8
+    bytes_step = iters * bytes_iter
+
+    def step():
+        return fma.FISTA(C, numLambda=0.05, numMaxSteps=iters).process(Bv)
+
+    def eager_step():
+        alg = fma.FISTA(C, numLambda=0.05, numMaxSteps=iters)
+        alg.cbStep = lambda a: None
+        return alg.process(Bv)
+
+    def check():
+        g = step()
+        e = eager_step()
+        r = C._apply(g.t().contiguous(), False) - B
+        return {"graph_equals_eager": bool(torch.equal(g, e)),
+                "rel_residual": float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(B))}
+
+    def cpu():
+        from oracle import cpu_path
+        op = cpu_path.circulant(c)
+        b = np.ascontiguousarray(B[:1].cpu().numpy().T)
+        L = 1.0 / C.largestSingularValue ** 2
+        n_it = 3
+
+        class Leg:
+            def forward(self, _x):
+                x = np.zeros((N, 1), dtype=np.complex64)
+                y, t_ = x.copy(), 1.0
+                for _ in range(n_it):  # FISTA.py:154-172 with the CPU operator
+                    xo = x
+                    step_ = y - L * op.backward(op.forward(y) - b)
+                    m = np.maximum(np.abs(step_) - L * 0.05 * 0.5, 0)
+                    x = (m / (m + L * 0.05 * 0.5)) * step_
+                    t0, t_ = t_, float((1 + np.sqrt(1 + 4 * t_ ** 2)) / 2)
+                    y = x + ((t0 - 1) / t_) * (x - xo)
+                return x
+
+            def backward(self, _x):
+                return None
+        return [(Leg(), None, n_it * (N * 1 * 8 * 13 + 2 * C.fftLength * 8))], \
+            "FISTA, 3 iterations x 1 column on the CPU operator path"
+
+    cfg = {"workload": "fista_circulant_2p20_c64_50iter", "N": N, "cols": k,
+           "iterations_per_step": iters}
+    return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check, plain=None,
+                cpu=cpu, cfg=cfg, dtype="c64", metric=METRIC)
+
+
+def workload_blockdiag(args, rng):
+    """BlockDiag(Circulant, Circulant, Fourier, Diag), blocks of 2^18, 32
+    columns, complex64: the reference's benchmark composition at N = 2^20."""
+    import torch
+    import fastmat_amd as fm
+    k = args.cols or 32
+    n = 1 << 18
+    c1, c2 = crandn(rng, (n,)), crandn(rng, (n,))
+    d = crandn(rng, (n,))
+    B = fm.BlockDiag(fm.Circulant(c1), fm.Circulant(c2), fm.Fourier(n), fm.Diag(d))
+    X = device_block(crandn(rng, (4 * n, k)), torch.complex64)
+    bytes_step = 2 * (2 * 4 * n * k * 8 + 2 * B.content[0].fftLength * 8 + n * 8)
+
+    def step():
+        return [B._apply(X, False), B._apply(X, True)]
+
+    def check():
+        from oracle import accept
+        x = X[0].cpu().numpy().astype(np.complex128)
+        ref = np.concatenate([
+            np.fft.ifft(np.fft.fft(c1.astype(np.complex128)) * np.fft.fft(x[:n])),
+            np.fft.ifft(np.fft.fft(c2.astype(np.complex128)) * np.fft.fft(x[n:2 * n])),
+            np.fft.fft(x[2 * n:3 * n]), d.astype(np.complex128) * x[3 * n:]])
+        return {"fwd": accept.max_rel_error(B._apply(X[:1].contiguous(), False).cpu().numpy()[0],
+                                            ref)}
+
+    def cpu():
+        from oracle import cpu_path
+        ops = [cpu_path.circulant(c1), cpu_path.circulant(c2), cpu_path.FourierCPU(n)]
+
+        class Leg:
+            def _run(self, x, adj):
+                outs = [(o.backward if adj else o.forward)(x[i * n:(i + 1) * n])
+                        for i, o in enumerate(ops)]
+                dd = np.conj(d) if adj else d
+                return np.concatenate(outs + [dd[:, None] * x[3 * n:]])
+
+            def forward(self, x):
+                return self._run(x, False)
+
+            def backward(self, x):
+                return self._run(x, True)
+        x = crandn(np.random.default_rng(7), (4 * n, 2))
+        return [(Leg(), x, 2 * (2 * 4 * n * 2 * 8))], \
+            "BlockDiag(Circulant, Circulant, Fourier, Diag) 4 x 2^18 x 2 cols, fwd+adj"
+
+    cfg = {"workload": "blockdiag_C_C_F_D_4x2p18_c64_fwd_adj", "block": n, "cols": k}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=cpu, cfg=cfg,
+                dtype="c64", metric=METRIC)
+
+
+def workload_lfsr(args, rng):
+    """LFSRCirculant of order 20 (polynomial 0x120001, the reference's
+    benchmark register), N = 2^20 - 1, 64 columns float32."""
+    import torch
+    import fastmat_amd as fm
+    k = args.cols or 64
+    L = fm.LFSRCirculant(0x120001, 0xFFFFFFFF)
+    n = L.numRows
+    X = device_block(rng.standard_normal((n, k), dtype=np.float32), torch.float32)
+    bytes_step = 2 * (2 * n * k * 4)
+
+    def step():
+        return [L._apply(X, False), L._apply(X, True)]
+
+    def check():
+        from oracle import accept
+        s = L.vecC.astype(np.float64)
+        x = X[0].cpu().numpy().astype(np.float64)
+        ref = np.real(np.fft.ifft(np.fft.fft(s) * np.fft.fft(x)))
+        return {"fwd": accept.max_rel_error(L._apply(X[:1].contiguous(), False).cpu().numpy()[0],
+                                            ref)}
+
+    def cpu():
+        from oracle import cpu_path
+        st, tp = L._sequences()
+        op = cpu_path.LFSRCirculantCPU(20, st, tp)
+        x = np.random.default_rng(7).standard_normal((n, 2)).astype(np.float32)
+        return [(op, x, 2 * (2 * n * 2 * 4))], \
+            "LFSRCirculant order 20 x 2 cols float32 (scatter, C Hadamard loop, gather), fwd+adj"
+
+    cfg = {"workload": "lfsr_circulant_order20_f32_fwd_adj", "N": n, "cols": k}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=cpu, cfg=cfg,
+                dtype="f32", metric=METRIC)
+
+
 WORKLOADS = {"copy": workload_copy, "fourier": workload_fourier, "hadamard": workload_hadamard,
-             "circulant": workload_circulant, "kron": workload_kron, "batch": workload_batch}
+             "circulant": workload_circulant, "kron": workload_kron, "batch": workload_batch,
+             "multilevel": workload_multilevel, "fista": workload_fista,
+             "blockdiag": workload_blockdiag, "lfsr": workload_lfsr}
 
 
 def time_cpu(legs):
@@ -405,14 +601,25 @@ def main():
     roof_note = "HIP events around the library's launches in the timed region"
     if not results and w.get("eager_step") is not None:
         # graph replays record no per-launch events: time the same kernels
-        # launched eagerly, same number of steps
-        with _lib.Profiler() as prof2:
-            for _ in range(args.steps):
-                w["eager_step"]()
-            torch.cuda.synchronize()
+        # launched eagerly, same number of steps, one length class at a time
+        # (in the replays the classes of a phase overlap on side streams, so
+        # an overlapped launch's duration would not be its own)
+        prev = os.environ.get("FASTMAT_AMD_BATCH_STREAMS")
+        os.environ["FASTMAT_AMD_BATCH_STREAMS"] = "1"
+        try:
+            with _lib.Profiler() as prof2:
+                for _ in range(args.steps):
+                    w["eager_step"]()
+                torch.cuda.synchronize()
+        finally:
+            if prev is None:
+                del os.environ["FASTMAT_AMD_BATCH_STREAMS"]
+            else:
+                os.environ["FASTMAT_AMD_BATCH_STREAMS"] = prev
         results = prof2.results
-        roof_note = ("HIP events around the same launches issued eagerly (the timed "
-                     "region replays HIP graphs)")
+        roof_note = ("HIP events around the same launches issued eagerly, one length "
+                     "class at a time (the timed region replays HIP graphs in which "
+                     "the classes of a phase run concurrently)")
     if not results:  # calibration copy: no library kernel
         print(json.dumps({"metric": w["metric"], "value": round(w["bytes"] / (ms * 1e-3) / 1e9, 2),
                           "unit": "GB/s", "ms_per_step": round(ms, 4), "config": w["cfg"]}))

@@ -63,10 +63,15 @@ static fm_status shrink_t(int64_t n, const void *y, const void *g, double L, dou
                           double beta, void *y_next, hipStream_t s) {
     int64_t blocks = ceil_div(n, 256);
     if (blocks > 8192) blocks = 8192;
+    ProfToken tok;
+    prof_begin(s, &tok);
     hipLaunchKernelGGL((shrink_kernel<V, R>), dim3(unsigned(blocks)), dim3(256), 0, s, n,
                        (const V *)y, (const V *)g, R(L), R(alpha), (V *)step, (V *)x, beta_tab,
                        counter, R(beta), (V *)y_next);
     FM_CHECK_LAUNCH();
+    // reads y, g (and x_old for the momentum), writes step, x (and y_next)
+    prof_elem(s, &tok, "shrink_kernel", sizeof(V),
+              (y_next ? 6.0 : 4.0) * double(n) * sizeof(V));
     return FM_SUCCESS;
 }
 

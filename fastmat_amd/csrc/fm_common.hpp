@@ -116,6 +116,14 @@ struct ProfToken {
 bool prof_on();
 void prof_begin(hipStream_t s, ProfToken *t);
 void prof_end(hipStream_t s, ProfToken *t, const char *name, double bytes, double flops);
+// prof_end for an elementwise kernel, labelled "<name><<element bytes>B>"
+inline void prof_elem(hipStream_t s, ProfToken *t, const char *name, size_t elem_bytes,
+                      double bytes) {
+    if (!t->active) return;
+    char label[64];
+    snprintf(label, sizeof(label), "%s<%dB>", name, int(elem_bytes));
+    prof_end(s, t, label, bytes, 0.0);
+}
 
 // Per-device cached tables (twiddles), keyed by (device, log2 size, precision).
 const void *twiddle_table(int precision, int logn, hipStream_t stream, fm_status *st);

@@ -127,28 +127,42 @@ __global__ void axpby_kernel(S alpha, const S *__restrict__ x, int64_t ldx, int 
 template <typename S>
 static fm_status diag_t(int conj_d, int64_t n, const void *d, const void *x, int64_t ldx,
                         void *y, int64_t ldy, int64_t k, hipStream_t s) {
+    ProfToken tok;
+    prof_begin(s, &tok);
     hipLaunchKernelGGL(diag_kernel<S>, grid_for(n * k), 256, 0, s, (const S *)d, (const S *)x,
                        ldx, (S *)y, ldy, n, k, conj_d);
     FM_CHECK_LAUNCH();
+    prof_elem(s, &tok, "diag_kernel", sizeof(S), (2.0 * double(n) * double(k) + double(n)) *
+                                                     sizeof(S));
     return FM_SUCCESS;
 }
 
 template <typename S>
 static fm_status gather_t(int64_t n_out, const int64_t *idx, const void *x, int64_t ldx, void *y,
                           int64_t ldy, int64_t k, hipStream_t s) {
+    ProfToken tok;
+    prof_begin(s, &tok);
     hipLaunchKernelGGL(gather_kernel<S>, grid_for(n_out * k), 256, 0, s, idx, (const S *)x, ldx,
                        (S *)y, ldy, n_out, k);
     FM_CHECK_LAUNCH();
+    prof_elem(s, &tok, "gather_kernel", sizeof(S),
+              2.0 * double(n_out) * double(k) * sizeof(S) + 8.0 * double(n_out));
     return FM_SUCCESS;
 }
 
 template <typename S>
 static fm_status scatter_t(int64_t n_in, const int64_t *idx, const void *x, int64_t ldx, void *y,
                            int64_t ldy, int64_t n_out, int64_t k, hipStream_t s) {
+    ProfToken tok;
+    prof_begin(s, &tok);
     hipLaunchKernelGGL(zero_kernel<S>, grid_for(n_out * k), 256, 0, s, (S *)y, ldy, n_out, k);
     hipLaunchKernelGGL(scatter_kernel<S>, grid_for(n_in * k), 256, 0, s, idx, (const S *)x, ldx,
                        (S *)y, ldy, n_in, k);
     FM_CHECK_LAUNCH();
+    // zero fill and scatter together: every output row written, every input
+    // row and index read
+    prof_elem(s, &tok, "scatter_rows", sizeof(S),
+              (double(n_in) + double(n_out)) * double(k) * sizeof(S) + 8.0 * double(n_in));
     return FM_SUCCESS;
 }
 
@@ -156,10 +170,14 @@ template <typename S>
 static fm_status axpby_t(int64_t n, int64_t k, const double *alpha, const void *x, int64_t ldx,
                          int conj_x, const double *beta, void *y, int64_t ldy, hipStream_t s) {
     const int use_beta = (beta[0] != 0.0 || beta[1] != 0.0) ? 1 : 0;
+    ProfToken tok;
+    prof_begin(s, &tok);
     hipLaunchKernelGGL(axpby_kernel<S>, grid_for(n * k), 256, 0, s, ElemOps<S>::from(alpha),
                        (const S *)x, ldx, conj_x, ElemOps<S>::from(beta), use_beta, (S *)y, ldy,
                        n, k);
     FM_CHECK_LAUNCH();
+    prof_elem(s, &tok, "axpby_kernel", sizeof(S),
+              (2.0 + use_beta) * double(n) * double(k) * sizeof(S));
     return FM_SUCCESS;
 }
 

@@ -34,6 +34,10 @@ bool prof_on() { return g_on.load(std::memory_order_relaxed); }
 void prof_begin(hipStream_t s, ProfToken *t) {
     t->active = false;
     if (!prof_on()) return;
+    // launches recorded into a HIP graph are not timed (an event recorded
+    // during capture becomes a graph node, not a timestamp)
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
     if (hipEventCreate(&t->a) != hipSuccess) return;
     if (hipEventCreate(&t->b) != hipSuccess) {
         (void)hipEventDestroy(t->a);

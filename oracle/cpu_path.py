@@ -124,3 +124,60 @@ def toeplitz(vecC, vecR, optimize=True):
 def hadamard_forward(X):
     from . import fwht_c
     return fwht_c.hadamard(X)
+
+
+class CirculantMultilevelCPU:
+    """Multi-level Circulant with power-of-two levels (no embedding):
+    Product(K.H, Diag(fftn(t)/L), K), K = Kron(Fourier(L_1), ..., Fourier(L_d))
+    (Circulant.pyx:138-215), each Kron factor applied along its mode with
+    np.fft.fft (Kron.pyx:244-300, Fourier.pyx:213-214) and F.H as
+    conj(fft(conj(.))) (Fourier.pyx:235-238)."""
+
+    def __init__(self, tenC):
+        t = np.asarray(tenC)
+        self.shape = t.shape
+        self.d = np.fft.fftn(t.astype(np.complex128)) / t.size
+
+    def _k(self, Z, inverse):
+        for ax in range(len(self.shape)):
+            Z = np.conj(np.fft.fft(np.conj(Z), axis=ax)) if inverse else np.fft.fft(Z, axis=ax)
+        return Z
+
+    def _apply(self, X, d):
+        k = X.shape[1]
+        Z = X.astype(np.promote_types(X.dtype, np.complex128)).reshape(self.shape + (k,))
+        Z = self._k(Z, False)
+        Z = d[..., None] * Z
+        return self._k(Z, True).reshape(X.shape)
+
+    def forward(self, X):
+        return self._apply(X, self.d)
+
+    def backward(self, X):
+        return self._apply(X, np.conj(self.d))
+
+
+class LFSRCirculantCPU:
+    """LFSRCirculant._core (LFSRCirculant.pyx:316-396): scatter the rows to
+    the register states (zeros elsewhere), the typed radix-2 Hadamard loop
+    (the C restatement), gather the tap-side states; `states` and `taps` are
+    one period of each sequence."""
+
+    def __init__(self, order, states, taps):
+        self.order = order
+        n = len(states)
+        neg = (-np.arange(n)) % n
+        self.sf, self.sb = np.asarray(states)[neg], np.asarray(states)
+        self.gf, self.gb = np.asarray(taps), np.asarray(taps)[neg]
+
+    def _core(self, X, sc, ga):
+        from . import fwht_c
+        Z = np.zeros((1 << self.order, X.shape[1]), dtype=X.dtype)
+        Z[sc] = X
+        return fwht_c.hadamard(Z)[ga]
+
+    def forward(self, X):
+        return self._core(X, self.sf, self.gf)
+
+    def backward(self, X):
+        return self._core(X, self.sb, self.gb)

@@ -571,3 +571,49 @@ def test_lfsr_circulant_structure():
     i = np.arange(n)
     np.testing.assert_array_equal(C, s[(i[:, None] - i[None, :]) % n])
     np.testing.assert_array_equal(C.T @ C, (n + 1) * np.eye(n, dtype=np.int64) - 1)
+
+
+def test_cpu_path_multilevel_circulant_matches_dense():
+    rng = np.random.default_rng(81)
+    t = rng.standard_normal((4, 8)) + 1j * rng.standard_normal((4, 8))
+    op = cpu_path.CirculantMultilevelCPU(t)
+    M = dense.circulant_multilevel(t)
+    x = rng.standard_normal((32, 3)) + 1j * rng.standard_normal((32, 3))
+    np.testing.assert_allclose(op.forward(x), M @ x, atol=1e-12)
+    np.testing.assert_allclose(op.backward(x), M.conj().T @ x, atol=1e-12)
+
+
+def _lfsr_states_taps(poly, start):
+    order, mask = 0, 1
+    while (~mask & poly) > mask:
+        mask <<= 1
+        order += 1
+    mask = 1 << order
+    s = st0 = start & (mask - 1)
+    states = []
+    while True:
+        states.append(s)
+        if bin(s & poly).count("1") & 1:
+            s |= mask
+        s >>= 1
+        if s == st0:
+            break
+    taps, b = [], 1
+    for _ in states:
+        taps.append(b)
+        b <<= 1
+        if b & mask:
+            b ^= poly | mask
+    return order, states, taps
+
+
+@pytest.mark.parametrize("poly,start", [(0x19, 0xD), (0x17, 0x1), (0x25, 0x1F)])
+def test_cpu_path_lfsr_hadamard_factorisation(poly, start):
+    # scatter -> Hadamard -> gather equals the dense circulant of the
+    # sequence (the GF(2)-linearity identity the reference's _core relies on)
+    order, states, taps = _lfsr_states_taps(poly, start)
+    op = cpu_path.LFSRCirculantCPU(order, states, taps)
+    C = dense.lfsr_circulant(poly, start).astype(np.int64)
+    x = np.random.default_rng(poly).integers(-3, 4, (C.shape[0], 4)).astype(np.int64)
+    np.testing.assert_array_equal(op.forward(x), C @ x)
+    np.testing.assert_array_equal(op.backward(x), C.T @ x)
