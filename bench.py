@@ -64,15 +64,16 @@ def device_block(a, dtype):
     return torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda").to(dtype)
 
 
-def traffic_from_profiles(kernel):
+def traffic_from_profiles(kernel, config):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), if recorded for this
-    kernel in profiles/*traffic*.txt as `<kernel label> | bytes_per_launch`."""
+    kernel on this workload: a line `<kernel label> | bytes_per_launch` in
+    profiles/*<config>*traffic*.md (or .txt)."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None
     for f in sorted(os.listdir(d)):
-        if "traffic" not in f or not f.endswith((".txt", ".md")):
+        if "traffic" not in f or config not in f or not f.endswith((".txt", ".md")):
             continue
         for line in open(os.path.join(d, f)):
             parts = [p.strip() for p in line.split("|")]
@@ -635,7 +636,7 @@ def main():
         tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(tflops, 2), "peak": FP32_MFMA_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(tflops / FP32_MFMA_PEAK_TFLOPS, 4),
-                    "traffic": traffic_from_profiles(name), "kernel": name,
+                    "traffic": traffic_from_profiles(name, args.config), "kernel": name,
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
                     "algorithmic_flops_per_launch": flops_per_launch,
                     "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -643,7 +644,7 @@ def main():
     else:
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic_from_profiles(name), "kernel": name,
+                    "traffic": traffic_from_profiles(name, args.config), "kernel": name,
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
                     "algorithmic_bytes_per_launch": bytes_per_launch, "timing": roof_note}
     kernels = {k: {"launches": v["launches"], "avg_us": round(v["ms"] / v["launches"] * 1e3, 2),

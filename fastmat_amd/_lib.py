@@ -78,6 +78,8 @@ _PROTOTYPES = {
     "fm_counter_advance": (_c_int, [_vp, _vp]),
     "fm_lfsr_sequence": (_c_int, [ctypes.c_uint32, ctypes.c_uint32, _c_int,
                                   ctypes.POINTER(_c_i64), _vp, _vp, _c_i64]),
+    "fm_lfsr_apply": (_c_int, [_c_int, _c_int, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _c_i64,
+                               _vp, _c_i64, _vp]),
     "fm_stela_prox": (_c_int, [_c_int, _c_int, _c_i64, _c_i64, _vp, _vp, _vp, ctypes.c_double,
                                _vp, _vp, _vp, _c_i64, _vp]),
     "fm_stela_line": (_c_int, [_c_int, _c_i64, _c_i64, _vp, _vp, _vp, _c_i64, ctypes.c_double,

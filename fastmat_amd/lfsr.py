@@ -9,7 +9,8 @@ scatter of the input rows to the register states (zeros elsewhere), one
 Hadamard transform of length 2^n in the data's own type (integer data wraps
 like the reference), and a gather of the tap-side states.  The state
 sequences come from the library's host routine (fm_lfsr_sequence); the
-scatter, transform and gather run on the device kernels.
+scatter, transform and gather run in one library call (fm_lfsr_apply) that
+moves whole rows of all columns at once through a row-major work array.
 """
 import ctypes
 import warnings
@@ -104,23 +105,18 @@ class LFSRCirculant(Matrix):
     def _run(self, X, adjoint):
         """forward: y[i] = H[taps[i], :] . z with z[states[t]] = x[-t mod N];
         backward: z[states[t]] = x[t] and y[-i mod N] = (H z)[taps[i]]
-        (LFSRCirculant.pyx:316-407)."""
+        (LFSRCirculant.pyx:316-407), one fm_lfsr_apply call."""
         k, n = X.shape
         dt = D.np_dtype(X.dtype)
-        L = 1 << self.order
-        Z = D.empty(k, L, dt)
         Y = D.empty(k, n, dt)
         if k == 0:
             return Y
-        t = D.fm_type(dt)
         sc = self._index("scatter_bwd" if adjoint else "scatter_fwd")
         ga = self._index("gather_bwd" if adjoint else "gather_fwd")
-        _lib.call("fm_scatter_rows", t, n, _lib.ptr(sc), _lib.ptr(X), n, _lib.ptr(Z), L, L, k,
-                  D.stream())
-        _lib.call("fm_hadamard_apply", t, self.order, _lib.ptr(Z), L, _lib.ptr(Z), L, k,
-                  D.stream())
-        _lib.call("fm_gather_rows", t, n, _lib.ptr(ga), _lib.ptr(Z), L, _lib.ptr(Y), n, k,
-                  D.stream())
+        need = 2 * (1 << self.order) * k * dt.itemsize
+        ws = D.workspace(need)
+        _lib.call("fm_lfsr_apply", D.fm_type(dt), self.order, n, _lib.ptr(sc), _lib.ptr(ga),
+                  _lib.ptr(X), n, _lib.ptr(Y), n, k, _lib.ptr(ws), need, D.stream())
         return Y
 
     def _fwd(self, X):

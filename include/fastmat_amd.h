@@ -250,6 +250,16 @@ fm_status fm_counter_advance(int64_t *step_counter, void *stream);
  * of the Hadamard-based apply (LFSRCirculant.pyx:316-396). */
 fm_status fm_lfsr_sequence(uint32_t polynomial, uint32_t start, int order, int64_t *period,
                            int64_t *states, int64_t *taps, int64_t capacity);
+/* LFSRCirculant apply (LFSRCirculant._core, LFSRCirculant.pyx:316-396):
+ * y[:, c] = H-based circular product of x[:, c] for c < k, n = period rows:
+ * z[scatter_idx[i], c] = x[i, c] (all other rows of z zero), w = H_order z,
+ * y[i, c] = w[gather_idx[i], c].  The index tables hold the generator and
+ * tap-side states with the reference's flips for forward or backward
+ * (values < 2^order).  work: 2 * 2^order * k elements of dtype (the row-major
+ * z and w); every type of the Hadamard transform, integers wrapping. */
+fm_status fm_lfsr_apply(int dtype, int order, int64_t n, const int64_t *scatter_idx,
+                        const int64_t *gather_idx, const void *x, int64_t ldx, void *y,
+                        int64_t ldy, int64_t k, void *work, int64_t work_bytes, void *stream);
 
 /* STELA (algorithms/STELA.py:128-267): soft thresholding with exact line
  * search, one call per phase of a step, all on the stream, no allocation.
