@@ -58,6 +58,11 @@ class Hadamard(Matrix):
         dt = D.promote(X.dtype, self.dtype)
         if D.np_dtype(X.dtype) != dt or D.np_dtype(Y.dtype) != dt:
             return False
+        # with leading dimensions other than 2^order the library transforms
+        # one column per launch sequence; for more than a few columns a
+        # packed copy is cheaper
+        if X.shape[0] > 4 and (D.ld(X) != X.shape[1] or D.ld(Y) != Y.shape[1]):
+            return False
         _lib.call("fm_hadamard_apply", D.fm_type(dt), self._order, _lib.ptr(X), D.ld(X),
                   _lib.ptr(Y), D.ld(Y), X.shape[0], D.stream())
         return True
