@@ -11,6 +11,7 @@ another.  Input is widened to the composition's dtype first, as the
 reference does (BlockDiag.pyx:130, Blocks.pyx:195).
 """
 import numpy as np
+import torch
 
 from . import _device as D
 from . import _lib
@@ -63,9 +64,12 @@ class BlockDiag(Matrix):
                 convs.append((idx, xs, ys))
             else:
                 term._apply_into(xs, ys, adjoint)
-        if len(convs) == 1:
-            idx, xs, ys = convs[0]
-            self._content[idx]._apply_into(xs, ys, adjoint)
+        # binding a batch to new addresses allocates and synchronises, which a
+        # stream under graph capture does not allow: captured applies take the
+        # blocks one by one
+        if len(convs) == 1 or (convs and torch.cuda.is_current_stream_capturing()):
+            for idx, xs, ys in convs:
+                self._content[idx]._apply_into(xs, ys, adjoint)
         elif convs:
             b = self._conv_batch(convs, k, out_dt)
             b.bind_views([c[1] for c in convs], [c[2] for c in convs], adjoint)

@@ -271,3 +271,21 @@ def test_column_norms_and_scipy_operator(fm):
     assert accept.max_rel_error(L.rmatvec(v), Cd.conj().T @ v) < 1e-5
     assert accept.max_rel_error(L.matmat(V), Cd @ V) < 1e-5
     assert C.scipyLinearOperator is L
+
+
+def test_fista_on_blockdiag_captures(fm):
+    """A solver over a BlockDiag of Circulant blocks: the eager first step
+    applies the blocks as one batch, the captured step block by block; both
+    agree with the oracle."""
+    import fastmat_amd.algorithms as fma
+    from oracle import dense
+    rng = np.random.default_rng(66)
+    c1, c2 = rng.standard_normal(256), rng.standard_normal(128)
+    B = fm.BlockDiag(fm.Circulant(c1), fm.Circulant(c2))
+    ref_m = dense.block_diag([dense.circulant(c1), dense.circulant(c2)])
+    x0 = sparse_truth(rng, 384, 5, 2, np.float64)
+    b = ref_m @ x0
+    say("FISTA over BlockDiag")
+    out = fma.FISTA(B, numLambda=0.05, numMaxSteps=30).process(b)
+    ref, _, _ = oalg.fista(ref_m, b, num_lambda=0.05, num_max_steps=30)
+    assert accept.max_rel_error(out, ref) < 1e-8
