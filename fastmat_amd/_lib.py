@@ -46,6 +46,7 @@ _PROTOTYPES = {
     "fm_fourier_plan_destroy": (_c_int, [_vp]),
     "fm_circulant_plan_create": (_c_int, [_c_int, _c_i64, _vp, _vp, _pp]),
     "fm_toeplitz_plan_create": (_c_int, [_c_int, _c_i64, _c_i64, _vp, _vp, _pp]),
+    "fm_circulant2_plan_create": (_c_int, [_c_int, _c_i64, _c_i64, _vp, _vp, _pp]),
     "fm_conv_plan_info": (_c_int, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64),
                                    ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int)]),
     "fm_conv_workspace_size": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_i64)]),

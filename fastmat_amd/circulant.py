@@ -138,9 +138,12 @@ class _ConvBase(Matrix):
             from .product import Product
             Ls, t, rows, cols = self._multi
             Lt = int(np.prod(Ls))
-            spec = (np.fft.fftn(t) / Lt).reshape(-1).astype(cdt)
-            K = Kron(*[Fourier(L, dtype=cdt) for L in Ls])
-            P = Product(K.H, Diag(spec), K)
+            spec = (np.fft.fftn(t) / Lt).reshape(-1)
+            if _two_level_ok(Ls, cdt):
+                P = _TwoLevelConv(Ls, spec, cdt)
+            else:
+                K = Kron(*[Fourier(L, dtype=cdt) for L in Ls])
+                P = Product(K.H, Diag(spec.astype(cdt)), K)
             chain = Partial(P, rows=None if len(rows) == Lt else rows,
                             cols=None if len(cols) == Lt else cols)
             self._multi_chains[cdt] = chain
@@ -367,6 +370,61 @@ class Toeplitz(_ConvBase):
         # the closed form of Toeplitz._normalizeColCore (Toeplitz.pyx:371-470)
         return _multilevel_col_norms(self._tenT, tuple(self._arrDimRows),
                                      tuple(self._arrDimCols))
+
+
+def _two_level_ok(Ls, cdt):
+    """Two levels that the three-pass two-level plan takes: each 2^5 ..
+    2^13 (complex64) or 2^12 (complex128), together beyond one tile."""
+    if len(Ls) != 2:
+        return False
+    single = 8192 if np.dtype(cdt) == np.complex64 else 4096
+    top = 13 if np.dtype(cdt) == np.complex64 else 12
+    logs = [int(L).bit_length() - 1 for L in Ls]
+    return min(logs) >= 5 and max(logs) <= top and Ls[0] * Ls[1] > single
+
+
+class _TwoLevelConv(Matrix):
+    """The (L0 L1) x (L0 L1) two-level circulant with power-of-two levels
+    and spectrum fftn(t)/(L0 L1) (C order), applied by the library's
+    three-pass two-level plan (fm_circulant2_plan_create): level-0 DFT,
+    level-1 DFT / spectrum / inverse, level-0 inverse DFT."""
+
+    def __init__(self, Ls, spectrum, cdt):
+        self._Ls = (int(Ls[0]), int(Ls[1]))
+        self._content = ()
+        n = self._Ls[0] * self._Ls[1]
+        self._initProperties(n, n, cdt)
+        spec = D.as_tensor(np.asarray(spectrum, dtype=np.complex128), np.complex128)
+        h = ctypes.c_void_p()
+        _lib.call("fm_circulant2_plan_create", precision_of(cdt), self._Ls[0], self._Ls[1],
+                  _lib.ptr(spec), D.stream(), ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and _lib._lib is not None:
+                _lib._lib.fm_conv_plan_destroy(self._h)
+        except Exception:
+            pass
+
+    def _run(self, X, adjoint):
+        cdt = self.dtype
+        if D.np_dtype(X.dtype) != cdt:
+            X = X.to(D.torch_dtype(cdt))
+        k, n = X.shape
+        Y = D.empty(k, n, cdt)
+        need = ctypes.c_int64()
+        _lib.call("fm_conv_workspace_size", self._h, k, ctypes.byref(need))
+        ws = D.workspace(need.value)
+        _lib.call("fm_conv_apply", self._h, int(adjoint), _lib.ptr(X), n, _lib.ptr(Y), n, k,
+                  None if ws is None else _lib.ptr(ws), need.value, D.stream())
+        return Y
+
+    def _fwd(self, X):
+        return self._run(X, False)
+
+    def _bwd(self, X):
+        return self._run(X, True)
 
 
 def _grid_selection(dims, Ls):

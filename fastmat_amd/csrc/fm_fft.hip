@@ -836,6 +836,71 @@ extern "C" fm_status fm_toeplitz_plan_create(int precision, int64_t m, int64_t n
     return conv_plan_create(precision, 1, m, n, L, m, n - 1, generator, d, as_stream(stream), plan);
 }
 
+// Two-level circulant of n0 x n1 (powers of two) as the three-pass
+// pipeline of a 1-D convolution with the four-step twiddles left out: pass A
+// transforms level 0 (strided), pass B transforms level 1, multiplies the
+// spectrum and transforms back, pass C transforms level 0 back.  Without
+// the twiddle W_L^{n2 k1} the two passes are exactly the 2-D DFT, and the
+// spectrum, fftn(t)/(n0 n1) in C order, is already in the row pass's order.
+extern "C" fm_status fm_circulant2_plan_create(int precision, int64_t n0, int64_t n1,
+                                               const void *spectrum, void *stream,
+                                               fm_conv_plan **plan) {
+    FM_REQUIRE(plan != nullptr, FM_ERR_VALUE, "plan pointer is NULL");
+    *plan = nullptr;
+    FM_REQUIRE(precision == FM_COMPLEX64 || precision == FM_COMPLEX128, FM_ERR_TYPE,
+               "two-level convolution computes in complex64 or complex128");
+    FM_REQUIRE(spectrum != nullptr, FM_ERR_VALUE, "spectrum is NULL");
+    FM_REQUIRE(n0 >= 2 && n1 >= 2 && is_pow2(n0) && is_pow2(n1), FM_ERR_VALUE,
+               "two-level lengths must be powers of two >= 2 (got %lld x %lld)",
+               (long long)n0, (long long)n1);
+    const int max_log = precision == FM_COMPLEX64 ? 13 : 12;
+    FM_REQUIRE(ilog2(n0) <= max_log && ilog2(n1) <= max_log, FM_ERR_NOT_IMPLEMENTED,
+               "two-level lengths above 2^%d", max_log);
+    hipStream_t s = as_stream(stream);
+    auto *p = new fm_conv_plan();
+    memset(p, 0, sizeof(*p));
+    hipGetDevice(&p->device);
+    const int64_t L = n0 * n1;
+    p->prec = precision; p->kind = 2; p->rows = L; p->cols = L; p->L = L; p->gen_len = L;
+    Geometry &g = p->geo;
+    g.prec = precision;
+    g.logL = ilog2(L);
+    g.passes = 3;
+    g.logN1 = ilog2(n0);
+    g.logN2 = ilog2(n1);
+    fm_status st = FM_SUCCESS;
+    g.tw1 = twiddle_table(FM_COMPLEX128, g.logN1, s, &st);
+    if (!st) g.tw2 = twiddle_table(FM_COMPLEX128, g.logN2, s, &st);
+    p->geo_fft = g;
+    p->geo_fft.passes = 0;  // no generator gradient for this kind
+    if (!st && hipMalloc(&p->D, esize(precision) * size_t(L)) != hipSuccess) {
+        set_error("device allocation for convolution plan failed");
+        st = FM_ERR_ALLOC;
+    }
+    if (!st) {
+        if (precision == FM_COMPLEX64)
+            hipLaunchKernelGGL(scale_cast_kernel<float>, grid1d(L), 256, 0, s, (cpx<float> *)p->D,
+                               (const cpx<double> *)spectrum, L, 1.0, 0, 0);
+        else
+            hipLaunchKernelGGL(scale_cast_kernel<double>, grid1d(L), 256, 0, s,
+                               (cpx<double> *)p->D, (const cpx<double> *)spectrum, L, 1.0, 0, 0);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("spectrum cast launch failed");
+            st = FM_ERR_HIP;
+        }
+    }
+    if (!st && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("stream synchronisation failed");
+        st = FM_ERR_HIP;
+    }
+    if (st) {
+        fm_conv_plan_destroy(p);
+        return st;
+    }
+    *plan = p;
+    return FM_SUCCESS;
+}
+
 extern "C" fm_status fm_conv_plan_info(const fm_conv_plan *p, int64_t *rows, int64_t *cols,
                                        int64_t *fft_length, int *passes) {
     FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
@@ -881,6 +946,8 @@ extern "C" fm_status fm_conv_apply(const fm_conv_plan *p, int adjoint, const voi
 
 extern "C" fm_status fm_conv_apply_strided(const fm_conv_plan *p, int adjoint, const void *x,
                                            void *y, int64_t outer, int64_t inner, void *stream) {
+    FM_REQUIRE(p == nullptr || p->kind != 2, FM_ERR_NOT_IMPLEMENTED,
+               "strided apply of a two-level convolution plan");
     FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
     FM_REQUIRE(p->rows == p->cols && p->L == p->rows, FM_ERR_NOT_IMPLEMENTED,
                "strided application needs a square plan without padding");
@@ -969,6 +1036,8 @@ extern "C" fm_status fm_conv_generator_grad(const fm_conv_plan *p, const void *x
                                             void *workspace, int64_t workspace_bytes,
                                             void *stream) {
     FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->kind != 2, FM_ERR_NOT_IMPLEMENTED,
+               "generator gradient of a two-level convolution plan");
     FM_REQUIRE(k >= 1, FM_ERR_VALUE, "generator gradient needs at least one column");
     FM_REQUIRE(x && g && grad && workspace, FM_ERR_VALUE, "NULL data pointer");
     FM_REQUIRE(ldx >= p->cols && ldg >= p->rows, FM_ERR_VALUE, "leading dimension too small");
@@ -1070,6 +1139,9 @@ extern "C" fm_status fm_batch_plan_create(int count, const fm_conv_plan *const *
     FM_REQUIRE(out != nullptr, FM_ERR_VALUE, "plan pointer is NULL");
     *out = nullptr;
     FM_REQUIRE(count >= 1 && plans && ks, FM_ERR_VALUE, "empty batch");
+    for (int i = 0; i < count; ++i)
+        FM_REQUIRE(plans[i] && plans[i]->kind != 2, FM_ERR_NOT_IMPLEMENTED,
+                   "batch entry %d: only one-level convolution plans batch", i);
     const int prec = plans[0]->prec;
     auto *p = new fm_batch_plan();
     p->prec = prec;

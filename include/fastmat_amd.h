@@ -127,6 +127,16 @@ fm_status fm_circulant_plan_create(int precision, int64_t n,
 fm_status fm_toeplitz_plan_create(int precision, int64_t m, int64_t n,
                                   const void *generator, void *stream,
                                   fm_conv_plan **plan);
+/* Two-level circulant of size (n0 n1) x (n0 n1), n0 and n1 powers of two
+ * (2 .. 2^13 for complex64, 2 .. 2^12 for complex128): the multi-level
+ * Circulant of Circulant.pyx:138-215 with power-of-two levels, here as three
+ * passes (level-0 DFT, level-1 DFT / spectrum / inverse, level-0 inverse)
+ * instead of Kron(Fourier) products.  `spectrum` is device complex128,
+ * fftn(t) / (n0 n1) in C order (n1 fastest).  Applied with fm_conv_apply /
+ * fm_conv_workspace_size; no strided apply, batch or generator gradient
+ * (FM_ERR_NOT_IMPLEMENTED). */
+fm_status fm_circulant2_plan_create(int precision, int64_t n0, int64_t n1, const void *spectrum,
+                                    void *stream, fm_conv_plan **plan);
 fm_status fm_conv_plan_info(const fm_conv_plan *plan, int64_t *rows,
                             int64_t *cols, int64_t *fft_length, int *passes);
 fm_status fm_conv_workspace_size(const fm_conv_plan *plan, int64_t k,

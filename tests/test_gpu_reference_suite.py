@@ -294,3 +294,45 @@ def test_lfsr_circulant_order_20(fm):
     xf = x.astype(np.float32)
     yf = L.forward(torch.from_numpy(np.ascontiguousarray(xf.T)).cuda().t())
     assert accept.max_rel_error(yf.cpu().numpy(), fwd) < 1e-5
+
+
+@pytest.mark.parametrize("cdt", [np.complex64, np.complex128])
+def test_two_level_plan(fm, cdt):
+    """Two-level Circulant / Toeplitz large enough for the three-pass
+    two-level plan: power-of-two levels against the FP64 2-D circular
+    convolution, embedded levels against the oracle's dense matrices."""
+    from fastmat_amd.circulant import _TwoLevelConv
+    rng = np.random.default_rng(91)
+    tol = 1e-5 if cdt == np.complex64 else 1e-12
+    # power-of-two levels 64 x 256: no embedding
+    t = (rng.standard_normal((64, 256)) + 1j * rng.standard_normal((64, 256))).astype(cdt)
+    C = fm.Circulant(t)
+    say("two-level plan pow2", np.dtype(cdt).name)
+    x = rng.standard_normal((64 * 256, 3)) + 1j * rng.standard_normal((64 * 256, 3))
+    x = x.astype(cdt)
+    T = np.fft.fft2(t.astype(np.complex128))
+    X3 = x.astype(np.complex128).T.reshape(3, 64, 256)
+    fwd = np.fft.ifft2(T[None] * np.fft.fft2(X3)).reshape(3, -1).T
+    bwd = np.fft.ifft2(np.conj(T)[None] * np.fft.fft2(X3)).reshape(3, -1).T
+    assert accept.max_rel_error(C.forward(x), fwd) < tol
+    assert accept.max_rel_error(C.backward(x), bwd) < tol
+    assert isinstance(C._multilevel_chain(np.dtype(cdt)).content[0], _TwoLevelConv)
+    # embedded levels: Circulant 40 x 100 -> 128 x 256, Toeplitz (63, 255)
+    # split (32, 128) -> 64 x 256
+    say("two-level plan embedded", np.dtype(cdt).name)
+    t2 = (rng.standard_normal((40, 100)) + 1j * rng.standard_normal((40, 100))).astype(cdt)
+    C2 = fm.Circulant(t2)
+    M2 = dense.circulant_multilevel(t2.astype(np.complex128))
+    x2 = (rng.standard_normal((4000, 2)) + 1j * rng.standard_normal((4000, 2))).astype(cdt)
+    assert accept.max_rel_error(C2.forward(x2), M2 @ x2.astype(np.complex128)) < tol
+    assert accept.max_rel_error(C2.backward(x2), M2.conj().T @ x2.astype(np.complex128)) < tol
+    assert isinstance(C2._multilevel_chain(np.dtype(cdt)).content[0], _TwoLevelConv)
+    t3 = (rng.standard_normal((63, 255)) + 1j * rng.standard_normal((63, 255))).astype(cdt)
+    T3 = fm.Toeplitz(t3, split=np.array([32, 128]))
+    M3 = dense.toeplitz_multilevel(t3.astype(np.complex128), np.array([32, 128]))
+    x3 = (rng.standard_normal((M3.shape[1], 2)) +
+          1j * rng.standard_normal((M3.shape[1], 2))).astype(cdt)
+    assert accept.max_rel_error(T3.forward(x3), M3 @ x3.astype(np.complex128)) < tol
+    g3 = (rng.standard_normal((M3.shape[0], 2)) +
+          1j * rng.standard_normal((M3.shape[0], 2))).astype(cdt)
+    assert accept.max_rel_error(T3.backward(g3), M3.conj().T @ g3.astype(np.complex128)) < tol
