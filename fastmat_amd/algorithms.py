@@ -99,6 +99,15 @@ class Algorithm:
             return callback(self)
         return None
 
+    @property
+    def nbytes(self):
+        """Bytes of the arrays and device buffers the instance holds, its
+        matrix included (Algorithm.pyx's getMemoryFootprint)."""
+        from .matrix import _footprint
+        seen = set()
+        return sum(_footprint(v, seen) for v in vars(self).values()
+                   if not callable(v))
+
 
 class _Shrinkage(Algorithm):
     """Common device loop of ISTA (momentum 0) and FISTA."""

@@ -214,6 +214,10 @@ class Circulant(_ConvBase):
         # Circulant.pyx:229-230: every column holds all of tenC
         return np.full((self._tenC.size,), np.linalg.norm(self._tenC))
 
+    def _getRowNorms(self):
+        # Circulant.pyx:232-233
+        return np.full((self._tenC.size,), np.linalg.norm(self._tenC))
+
     @property
     def tenC(self):
         return self._tenC

@@ -28,6 +28,10 @@ class Diag(Matrix):
         # Diag.pyx:126-127
         return np.abs(self._vecD)
 
+    def _getRowNorms(self):
+        # Diag.pyx:129-130
+        return np.abs(self._vecD)
+
     @property
     def vecD(self):
         return self._vecD

@@ -16,6 +16,10 @@ class Eye(Matrix):
         # Eye.pyx:92-93
         return np.ones((self.numCols,), dtype=self.dtype)
 
+    def _getRowNorms(self):
+        # Eye.pyx:95-96
+        return np.ones((self.numRows,), dtype=self.dtype)
+
     def _fwd(self, X):
         return X
 

@@ -82,6 +82,10 @@ class Fourier(Matrix):
         # Fourier.pyx:176-177: every column has norm sqrt(N)
         return np.full((self.numCols,), np.sqrt(self.numCols))
 
+    def _getRowNorms(self):
+        # Fourier.pyx:179-180
+        return np.full((self.numRows,), np.sqrt(self.numRows))
+
     @property
     def order(self):
         return self._order

@@ -38,6 +38,10 @@ class Hadamard(Matrix):
         # compares against; the test's definition is kept.)
         return np.full((self.numCols,), np.sqrt(self.numCols))
 
+    def _getRowNorms(self):
+        # Hadamard.pyx:146-147: sqrt(2^n)
+        return np.full((self.numRows,), np.sqrt(self.numRows))
+
     @property
     def order(self):
         return self._order

@@ -49,6 +49,13 @@ class Kron(Matrix):
             out = np.kron(out, f.colNorms)
         return out
 
+    def _getRowNorms(self):
+        # Kron.pyx:174-182
+        out = self._content[0].rowNorms
+        for f in self._content[1:]:
+            out = np.kron(out, f.rowNorms)
+        return out
+
     def _run(self, X, adjoint):
         k = X.shape[0]
         dims = self._dims

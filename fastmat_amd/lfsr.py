@@ -130,6 +130,10 @@ class LFSRCirculant(Matrix):
         # LFSRCirculant.pyx:250-251
         return np.full((self.numCols,), np.sqrt(self.numCols))
 
+    def _getRowNorms(self):
+        # LFSRCirculant.pyx:253-254
+        return np.full((self.numRows,), np.sqrt(self.numRows))
+
     def _getLargestSingularValue(self):
         # a circulant matrix is normal: sigma_max = max |DFT(first column)|
         return float(np.max(np.abs(np.fft.fft(self.vecC.astype(np.float64)))))

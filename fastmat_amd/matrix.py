@@ -146,6 +146,121 @@ class Matrix:
         return out
 
     @property
+    def rowNorms(self):
+        """Euclidean norm of every row, float64 (Matrix.pyx:1091-1148)."""
+        if getattr(self, "_rowNorms", None) is None:
+            return self.getRowNorms()
+        return self._rowNorms
+
+    def getRowNorms(self):
+        self._rowNorms = self._getRowNorms()
+        return self._rowNorms
+
+    def _getRowNorms(self):
+        """The adjoint applied to slices of the float64 identity (row i of A
+        has the norm of column i of A^H), on the device; classes with a
+        closed form override this."""
+        m, n = self.numRows, self.numCols
+        out = np.empty(m, dtype=np.float64)
+        if m == 0:
+            return out
+        if n == 0:
+            out[:] = 0.0
+            return out
+        s = max(1, min(1024, (1 << 24) // n))
+        dev = D.device()
+        for i in range(0, m, s):
+            j = min(m, i + s)
+            E = torch.zeros(j - i, m, dtype=torch.float64, device=dev)
+            E[torch.arange(j - i, device=dev), torch.arange(i, j, device=dev)] = 1.0
+            F = self._apply(E, True)
+            out[i:j] = torch.linalg.vector_norm(F.to(torch.complex128)
+                                                if F.is_complex() else F.double(),
+                                                dim=1).cpu().numpy()
+        return out
+
+    @property
+    def colNormalized(self):
+        """A Diag(1 / colNorms) (Matrix.pyx:1151-1178)."""
+        if getattr(self, "_colNormalized", None) is None:
+            return self.getColNormalized()
+        return self._colNormalized
+
+    def getColNormalized(self):
+        self._colNormalized = self._getColNormalized()
+        return self._colNormalized
+
+    def _getColNormalized(self):
+        from .diag import Diag
+        norms = np.asarray(self.colNorms)
+        if np.any(norms == 0):
+            raise ValueError("Normalization: Matrix has zero-norm column.")
+        return self * Diag(1. / norms)
+
+    @property
+    def rowNormalized(self):
+        """Diag(1 / rowNorms) A (Matrix.pyx:1180-1207)."""
+        if getattr(self, "_rowNormalized", None) is None:
+            return self.getRowNormalized()
+        return self._rowNormalized
+
+    def getRowNormalized(self):
+        self._rowNormalized = self._getRowNormalized()
+        return self._rowNormalized
+
+    def _getRowNormalized(self):
+        from .diag import Diag
+        norms = np.asarray(self.rowNorms)
+        if np.any(norms == 0):
+            raise ValueError("Normalization: Matrix has zero-norm row.")
+        return Diag(1. / norms) * self
+
+    @property
+    def gram(self):
+        """A^H A as a fastmat product (Matrix.pyx:1009-1028)."""
+        if getattr(self, "_gram", None) is None:
+            return self.getGram()
+        return self._gram
+
+    def getGram(self):
+        self._gram = self._getGram()
+        return self._gram
+
+    def _getGram(self):
+        from .product import Product
+        return Product(self.H, self)
+
+    @property
+    def largestSV(self):
+        import warnings
+        warnings.warn('largestSV is deprecated. Use largestSingularValue.', FutureWarning)
+        return self.largestSingularValue
+
+    @property
+    def numN(self):
+        import warnings
+        warnings.warn('numN is deprecated. Use numRows.', FutureWarning)
+        return self.numRows
+
+    @property
+    def numM(self):
+        import warnings
+        warnings.warn('numM is deprecated. Use numCols.', FutureWarning)
+        return self.numCols
+
+    @property
+    def normalized(self):
+        import warnings
+        warnings.warn('normalized is deprecated. Use colNormalized.', FutureWarning)
+        return self.colNormalized
+
+    @property
+    def nbytes(self):
+        """Bytes held by this instance and its terms: host arrays and device
+        buffers (the reference's getMemoryFootprint, Matrix.pyx:317-322)."""
+        return _footprint(self, set())
+
+    @property
     def scipyLinearOperator(self):
         """scipy.sparse.linalg.LinearOperator over forward / backward
         (Matrix.pyx:977-1006)."""
@@ -360,6 +475,36 @@ class Matrix:
         e[idx] = 1
         return self.forward(e)
 
+    def getCols(self, indices):
+        """Columns by index: an int gives getCol, a slice or 1-D index array
+        a 2-D array (Matrix.pyx:389-427), all columns in one batched apply."""
+        if np.isscalar(indices):
+            return self.getCol(indices)
+        if getattr(self, "_array", None) is not None:
+            return self._array[:, indices]
+        idx = (np.arange(*indices.indices(self.numCols)) if isinstance(indices, slice)
+               else np.array(indices))
+        if idx.ndim > 1:
+            raise IndexError("Index array must have at most one dimension.")
+        E = np.zeros((self.numCols, idx.size), dtype=self.dtype)
+        E[idx, np.arange(idx.size)] = 1
+        return self.forward(E)
+
+    def getRows(self, indices):
+        """Rows by index (Matrix.pyx:460-499), all rows in one batched
+        adjoint apply."""
+        if np.isscalar(indices):
+            return self.getRow(indices)
+        if getattr(self, "_array", None) is not None:
+            return self._array[indices, :]
+        idx = (np.arange(*indices.indices(self.numRows)) if isinstance(indices, slice)
+               else np.array(indices))
+        if idx.ndim > 1:
+            raise IndexError("Index array must have at most one dimension.")
+        E = np.zeros((self.numRows, idx.size), dtype=self.dtype)
+        E[idx, np.arange(idx.size)] = 1
+        return np.conj(self.backward(E)).T
+
     def getRow(self, idx):
         e = np.zeros(self.numRows, dtype=self.dtype)
         e[idx] = 1
@@ -375,6 +520,25 @@ class Matrix:
 
     def _reference(self):
         return self._array
+
+
+def _footprint(obj, seen):
+    """Bytes of the numpy arrays and torch tensors an instance holds
+    (attributes, dicts and the terms of compositions), each counted once."""
+    if id(obj) in seen:
+        return 0
+    seen.add(id(obj))
+    if isinstance(obj, np.ndarray):
+        return int(obj.nbytes)
+    if isinstance(obj, torch.Tensor):
+        return int(obj.numel() * obj.element_size())
+    if isinstance(obj, dict):
+        return sum(_footprint(v, seen) for v in obj.values())
+    if isinstance(obj, (list, tuple)):
+        return sum(_footprint(v, seen) for v in obj)
+    if isinstance(obj, Matrix):
+        return sum(_footprint(v, seen) for v in vars(obj).values())
+    return 0
 
 
 def _views_direct(X, Y):

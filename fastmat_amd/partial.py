@@ -61,6 +61,13 @@ class Partial(Matrix):
         norms = self._content[0].colNorms
         return norms if self._colSelection is None else norms[self._colSelection]
 
+    def _getRowNorms(self):
+        # Partial.pyx:243-250
+        if self._colSelection is not None:
+            return super()._getRowNorms()
+        norms = self._content[0].rowNorms
+        return norms if self._rowSelection is None else norms[self._rowSelection]
+
     def _idx(self, name, sel):
         t = self._dev.get(name)
         if t is None:

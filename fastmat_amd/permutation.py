@@ -29,6 +29,10 @@ class Permutation(Matrix):
         # Permutation.pyx:104-105
         return np.ones((self.numCols,), dtype=self.dtype)
 
+    def _getRowNorms(self):
+        # Permutation.pyx:107-108
+        return np.ones((self.numRows,), dtype=self.dtype)
+
     @property
     def sigma(self):
         return self._sigma

@@ -289,3 +289,45 @@ def test_fista_on_blockdiag_captures(fm):
     out = fma.FISTA(B, numLambda=0.05, numMaxSteps=30).process(b)
     ref, _, _ = oalg.fista(ref_m, b, num_lambda=0.05, num_max_steps=30)
     assert accept.max_rel_error(out, ref) < 1e-8
+
+
+def test_matrix_properties_rows_normalized_gram(fm):
+    """rowNorms, colNormalized / rowNormalized, gram, getCols / getRows,
+    nbytes and the deprecated aliases (Matrix.pyx:316-1340)."""
+    import warnings
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(67)
+    say("matrix properties")
+    cases = [fm.Fourier(16), fm.Hadamard(4), fm.Circulant(rng.standard_normal(10)),
+             fm.Toeplitz(rng.standard_normal(6), rng.standard_normal(9)),
+             fm.Diag(rng.standard_normal(7) + 0.5), fm.Permutation(rng.permutation(6)),
+             fm.Kron(fm.Fourier(4), fm.Diag(rng.standard_normal(3) + 2)),
+             fm.Partial(fm.Fourier(16), rows=np.array([0, 2, 5])),
+             fm.Partial(fm.Fourier(16), cols=np.array([1, 3])),
+             fm.LFSRCirculant(0x19, 0xD),
+             fm.Product(fm.Matrix(rng.standard_normal((5, 8))), fm.Hadamard(3))]
+    for A in cases:
+        R = A.reference().astype(np.complex128)
+        tol = 1e-6 if A.dtype in (np.float32, np.complex64) else 1e-12
+        np.testing.assert_allclose(np.asarray(A.rowNorms, dtype=np.float64),
+                                   np.linalg.norm(R, axis=1), rtol=tol, err_msg=repr(A))
+        Cn = A.colNormalized.reference().astype(np.complex128)
+        np.testing.assert_allclose(np.linalg.norm(Cn, axis=0), 1.0, rtol=tol, err_msg=repr(A))
+        Rn = A.rowNormalized.reference().astype(np.complex128)
+        np.testing.assert_allclose(np.linalg.norm(Rn, axis=1), 1.0, rtol=tol, err_msg=repr(A))
+        G = A.gram
+        assert accept.max_rel_error(G.forward(np.eye(A.numCols)), R.conj().T @ R) < tol * 10
+        idx = np.array([0, A.numCols - 1])
+        assert accept.max_rel_error(A.getCols(idx), R[:, idx]) < tol
+        ridx = np.array([A.numRows - 1, 0])
+        assert accept.max_rel_error(A.getRows(ridx), R[ridx, :]) < tol
+        assert A.nbytes > 0
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        F = fm.Fourier(8)
+        assert F.numN == 8 and F.numM == 8 and abs(F.largestSV - np.sqrt(8)) < 1e-12
+        assert F.normalized is F.colNormalized
+    with pytest.raises(ValueError, match="zero-norm column"):
+        fm.Diag(np.array([1.0, 0.0])).colNormalized
+    alg = fma.ISTA(fm.Fourier(8))
+    assert alg.nbytes >= 0
