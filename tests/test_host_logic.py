@@ -161,3 +161,59 @@ def test_lfsr_host_sequences_and_errors():
         fm.LFSRCirculant(0x10, 0x3)
     with pytest.warns(FutureWarning):
         assert L.taps == 0b11001
+
+
+def test_block_composition_constructor_errors():
+    """BlockDiag.pyx:110-127 and Blocks.pyx:88-170 argument checks (host
+    only)."""
+    import fastmat_amd as fm
+    M = fm.Matrix(np.eye(2))
+    B = fm.BlockDiag(M, fm.Matrix(np.ones((3, 1), dtype=np.int16)))
+    assert B.shape == (5, 3) and B.dtype == np.float64
+    with pytest.raises(ValueError, match="at least one matrix"):
+        fm.BlockDiag()
+    with pytest.raises(ValueError, match="Only fastmat matrices"):
+        fm.BlockDiag(M, np.eye(2))
+    G = fm.Blocks([[M, fm.Matrix(np.zeros((2, 3)))], [fm.Matrix(np.ones((1, 2))),
+                                                     fm.Matrix(np.ones((1, 3)))]])
+    assert G.shape == (3, 5)
+    with pytest.raises(ValueError, match="Not a nested list"):
+        fm.Blocks(M)
+    with pytest.raises(ValueError, match="Contains no matrices"):
+        fm.Blocks([])
+    with pytest.raises(TypeError, match="row\\(0\\)"):
+        fm.Blocks([M])
+    with pytest.raises(TypeError, match="not a fastmat Matrix"):
+        fm.Blocks([[M, np.eye(2)]])
+    with pytest.raises(ValueError, match="incompatible number of entries"):
+        fm.Blocks([[M, M], [M]])
+    with pytest.raises(ValueError, match="row height"):
+        fm.Blocks([[M, fm.Matrix(np.ones((3, 2)))]])
+    with pytest.raises(ValueError, match="column width"):
+        fm.Blocks([[M, M], [M, fm.Matrix(np.ones((2, 3)))]])
+
+
+def test_algorithm_parameter_handling():
+    """Algorithm.pyx:28-176 / test/algorithm.py:23-90: abstract base,
+    parameter updates, trace type, constructor type checks (host only)."""
+    import fastmat_amd as fm
+    import fastmat_amd.algorithms as fma
+    with pytest.raises(NotImplementedError):
+        fma.Algorithm()
+    with pytest.raises(TypeError):
+        fma.ISTA(np.eye(3))
+    with pytest.raises(TypeError):
+        fma.STELA(None)
+    alg = fma.FISTA(fm.Matrix(np.eye(3)), numLambda=0.5)
+    assert alg.numLambda == 0.5 and alg.numMaxSteps == 100
+    with pytest.raises(AttributeError):
+        alg.updateParameters(numBla=123)
+    alg.updateParameters(numMaxSteps=7)
+    assert alg.numMaxSteps == 7
+    for bad in (None, (), {}, 1):
+        with pytest.raises(TypeError):
+            alg.trace = bad
+    alg.trace = []
+    s = fma.STELA(fm.Matrix(np.eye(3)))
+    assert s.numMaxError == 1e-6
+    np.testing.assert_allclose(s.softThreshold(np.array([2.0, -0.1]), 0.5), [1.5, 0.0])
