@@ -81,6 +81,10 @@ _PROTOTYPES = {
                                   ctypes.POINTER(_c_i64), _vp, _vp, _c_i64]),
     "fm_lfsr_apply": (_c_int, [_c_int, _c_int, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _c_i64, _c_i64,
                                _vp, _c_i64, _vp]),
+    "fm_omp_workspace_size": (_c_int, [_c_int, _c_i64, _c_i64, _c_int, ctypes.POINTER(_c_i64)]),
+    "fm_omp_step": (_c_int, [_c_int, _c_int, _c_i64, _c_i64, _c_i64, _c_int, _c_int, _vp, _vp,
+                             _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp,
+                             _vp]),
     "fm_stela_prox": (_c_int, [_c_int, _c_int, _c_i64, _c_i64, _vp, _vp, _vp, ctypes.c_double,
                                _vp, _vp, _vp, _c_i64, _vp]),
     "fm_stela_line": (_c_int, [_c_int, _c_i64, _c_i64, _vp, _vp, _vp, _c_i64, ctypes.c_double,

@@ -1,4 +1,4 @@
-"""Sparse-recovery solvers ISTA, FISTA and STELA with device-resident
+"""Sparse-recovery solvers ISTA, FISTA, STELA and OMP with device-resident
 iterations.
 
 Mirrors fastmat.algorithms (algorithms/Algorithm.pyx, ISTA.py, FISTA.py,
@@ -22,6 +22,7 @@ Results: numpy in -> numpy out, device tensor in -> device tensor out; the
 result is 2-D (N, k) also for 1-D input, like the reference.
 """
 import copy
+import ctypes
 
 import numpy as np
 import torch
@@ -392,3 +393,92 @@ class STELA(Algorithm):
         else:
             self.arrX = np.ascontiguousarray(s["X"].cpu().numpy().T)
             self.arrGamma = s["gamma"].cpu().numpy()
+
+
+class OMP(Algorithm):
+    """Orthogonal matching pursuit (OMP.pyx:24-240).
+
+    Each step correlates the residual with the normalised columns through the
+    operator's adjoint, picks the largest correlation per problem, forms the
+    new column with one forward apply of a one-hot block and updates the
+    pseudo-inverse of the selected columns by the reference's rank-1 rule;
+    the selection, the update and the result scatter are device kernels
+    (fm_omp_step).  Works in promote(A.dtype, b.dtype, float64) like the
+    reference; numpy in -> numpy (M, L) out, device tensor in -> device."""
+
+    _name = "OMP"
+
+    def __init__(self, fmatA, **kwargs):
+        if not isinstance(fmatA, Matrix):
+            raise TypeError("fmatA must be a fastmat matrix")
+        self.fmatA = fmatA
+        self.numMaxSteps = 0
+        self.cbStep = None
+        self.updateParameters(**kwargs)
+
+    def _process(self, arrB):
+        A = self.fmatA
+        on_device = torch.is_tensor(arrB)
+        if arrB.ndim > 2 or arrB.ndim < 1:
+            raise ValueError("Only n x m arrays are supported for OMP")
+        b = arrB.reshape((-1, 1)) if arrB.ndim == 1 else arrB
+        self.arrB = b
+        if self.numMaxSteps <= 0:
+            raise ValueError("OMP would like to do at least one step for you")
+        steps = int(self.numMaxSteps)
+        self.numN, self.numM = A.numRows, A.numCols
+        n, m = self.numN, self.numM
+        norms = np.asarray(A.colNorms, dtype=np.float64)
+        if np.any(norms == 0):
+            raise ValueError("Normalization: Matrix has zero-norm column.")
+        self.returnType = np.promote_types(
+            np.promote_types(np.promote_types(A.dtype, np.float64), D.np_dtype(b.dtype)),
+            np.float64)
+        vdt = self.returnType
+        if vdt not in (np.dtype(np.float64), np.dtype(np.complex128)):
+            raise TypeError("OMP needs a numeric problem, got %s" % vdt)
+        tv = D.torch_dtype(vdt)
+        Bd, _ = D.to_block(b, n)
+        Bd = Bd.to(tv).contiguous()
+        L = Bd.shape[0]
+        dev = Bd.device
+        R = Bd.clone()
+        P = torch.zeros(L, steps, n, dtype=tv, device=dev)
+        Ac = torch.zeros(L, steps, n, dtype=tv, device=dev)
+        Xt = torch.zeros(L, steps, dtype=tv, device=dev)
+        E = torch.empty(L, m, dtype=tv, device=dev)
+        idx = torch.zeros(L, dtype=torch.int64, device=dev)
+        sup = torch.zeros(L, steps, dtype=torch.int64, device=dev)
+        inv = torch.tensor(1.0 / norms, dtype=torch.float64, device=dev)
+        fm_t = _lib.NP_TO_FM[np.dtype(vdt)]
+        need = ctypes.c_int64()
+        _lib.call("fm_omp_workspace_size", fm_t, L, n, steps, ctypes.byref(need))
+        work = D.workspace(need.value)
+        Xf = torch.empty(L, m, dtype=tv, device=dev)
+        st = D.stream()
+        for self.numStep in range(steps):
+            ii = self.numStep
+            G = A._apply(R, True)
+            G = (G if G.dtype == tv else G.to(tv)).contiguous()
+            _lib.call("fm_omp_step", fm_t, 0, L, m, n, steps, ii, _lib.ptr(G), _lib.ptr(inv),
+                      _lib.ptr(idx), _lib.ptr(sup), _lib.ptr(E), None, None, None, None, None,
+                      _lib.ptr(work), need.value, None, None, st)
+            C = A._apply(E, False)
+            C = (C if C.dtype == tv else C.to(tv)).contiguous()
+            _lib.call("fm_omp_step", fm_t, 1, L, m, n, steps, ii, None, None, None, None, None,
+                      _lib.ptr(C), _lib.ptr(Bd), _lib.ptr(P), _lib.ptr(Ac), _lib.ptr(R),
+                      _lib.ptr(work), need.value, _lib.ptr(Xt), None, st)
+            if self.cbStep is not None or self.cbTrace is not None:
+                self.arrSupport = sup.cpu().numpy().T
+                self.handleCallback(self.cbStep)
+                self.handleCallback(self.cbTrace)
+        _lib.call("fm_omp_step", fm_t, 2, L, m, n, steps, 0, None, None, None, _lib.ptr(sup),
+                  None, None, None, None, None, None, _lib.ptr(work), need.value, _lib.ptr(Xt),
+                  _lib.ptr(Xf), st)
+        if on_device:
+            self.arrSupport = sup.t()
+            self.arrX = Xf.t()
+        else:
+            self.arrSupport = sup.cpu().numpy().T
+            self.arrX = np.ascontiguousarray(Xf.cpu().numpy().T)
+        return self.arrX

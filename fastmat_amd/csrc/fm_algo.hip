@@ -381,3 +381,330 @@ extern "C" fm_status fm_stela_select(int dtype_z, int64_t n, int64_t k, void *z,
     FM_CHECK_LAUNCH();
     return FM_SUCCESS;
 }
+
+// ------------------------------------------------------------------- OMP
+// Orthogonal matching pursuit (algorithms/OMP.pyx:122-240), per problem l of
+// L, in double precision (V = double or complex<double>), the rank-1
+// pseudo-inverse update of the reference kept step for step:
+//   select:  idx = argmax_j |(A^H r)_j| / colnorm_j          (first maximum)
+//   c = A e_idx (the new column, computed by the operator's forward)
+//   v1 = P[:ii] c,  v2 = c - Acols[:ii]^T v1,  nrm2 = ||v2||^2
+//   y = <v2, b> / nrm2,  x[:ii] -= y v1,  x[ii] += y
+//   P[:ii] -= v1 (conj(v2) / nrm2)^T,  P[ii] = conj(v2) / nrm2,  Acols[ii] = c
+//   r -= y v2
+// P and Acols are (L, steps, N) row-major, r / b / c / v2 (L, N), x (L, steps).
+namespace fm {
+
+template <typename V> struct OmpOps;
+template <> struct OmpOps<double> {
+    __device__ static double mul(double a, double b) { return a * b; }
+    __device__ static double conj(double a) { return a; }
+    __device__ static double abs2(double a) { return a * a; }
+    __device__ static double scale(double a, double s) { return a * s; }
+    __device__ static double zero() { return 0.0; }
+    __device__ static double one() { return 1.0; }
+    __device__ static double add(double a, double b) { return a + b; }
+    __device__ static double sub(double a, double b) { return a - b; }
+};
+template <> struct OmpOps<cpx<double>> {
+    using C = cpx<double>;
+    __device__ static C mul(C a, C b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+    __device__ static C conj(C a) { return mk(a.x, -a.y); }
+    __device__ static double abs2(C a) { return a.x * a.x + a.y * a.y; }
+    __device__ static C scale(C a, double s) { return mk(a.x * s, a.y * s); }
+    __device__ static C zero() { return mk(0.0, 0.0); }
+    __device__ static C one() { return mk(1.0, 0.0); }
+    __device__ static C add(C a, C b) { return mk(a.x + b.x, a.y + b.y); }
+    __device__ static C sub(C a, C b) { return mk(a.x - b.x, a.y - b.y); }
+};
+
+constexpr int kOmpNT = 256;
+
+template <typename V>
+__device__ __forceinline__ V omp_block_sum(V v, V *sh) {
+    using O = OmpOps<V>;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kOmpNT / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) sh[threadIdx.x] = O::add(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    V r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// one block per problem: first index of the largest |g_j| inv_norm_j
+template <typename V>
+__global__ __launch_bounds__(kOmpNT) void omp_select_kernel(int64_t m, const V *__restrict__ g,
+                                                            const double *__restrict__ inv_norm,
+                                                            int64_t *__restrict__ idx,
+                                                            int64_t *__restrict__ support,
+                                                            int steps, int ii) {
+    using O = OmpOps<V>;
+    __shared__ double sv[kOmpNT];
+    __shared__ int64_t si[kOmpNT];
+    const int64_t l = blockIdx.x;
+    double best = -1.0;
+    int64_t bi = m;
+    for (int64_t j = threadIdx.x; j < m; j += kOmpNT) {
+        const double v = sqrt(O::abs2(g[l * m + j])) * inv_norm[j];
+        if (v > best || (v != v && best == best)) {  // first NaN wins, as argmax
+            best = v;
+            bi = j;
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int w = kOmpNT / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            const double a = sv[threadIdx.x], b = sv[threadIdx.x + w];
+            const int64_t ia = si[threadIdx.x], ib = si[threadIdx.x + w];
+            const bool a_nan = a != a, b_nan = b != b;
+            bool take_b;
+            if (a_nan || b_nan) take_b = b_nan && (!a_nan || ib < ia);
+            else take_b = b > a || (b == a && ib < ia);
+            if (take_b) {
+                sv[threadIdx.x] = b;
+                si[threadIdx.x] = ib;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int64_t j = si[0] < m ? si[0] : 0;
+        idx[l] = j;
+        support[l * steps + ii] = j;
+    }
+}
+
+template <typename V>
+__global__ void omp_onehot_kernel(int64_t L, int64_t m, const int64_t *__restrict__ idx,
+                                  V *__restrict__ e) {
+    using O = OmpOps<V>;
+    const int64_t total = L * m;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t l = q / m, j = q - l * m;
+        e[q] = (j == idx[l]) ? O::one() : O::zero();
+    }
+}
+
+// v1[l, i] = sum_j P[l, i, j] c[l, j]; grid (ii, L)
+template <typename V>
+__global__ __launch_bounds__(kOmpNT) void omp_proj_kernel(int64_t n, int steps,
+                                                          const V *__restrict__ P,
+                                                          const V *__restrict__ c,
+                                                          V *__restrict__ v1) {
+    using O = OmpOps<V>;
+    __shared__ V sh[kOmpNT];
+    const int64_t i = blockIdx.x, l = blockIdx.y;
+    const V *row = P + (l * steps + i) * n;
+    V acc = O::zero();
+    for (int64_t j = threadIdx.x; j < n; j += kOmpNT) acc = O::add(acc, O::mul(row[j], c[l * n + j]));
+    acc = omp_block_sum<V>(acc, sh);
+    if (threadIdx.x == 0) v1[l * steps + i] = acc;
+}
+
+// v2 = c - Acols[:ii]^T v1 and partial sums of |v2|^2 and conj(v2) b;
+// grid (parts, L)
+template <typename V>
+__global__ __launch_bounds__(kOmpNT) void omp_resid_kernel(
+    int64_t n, int steps, int ii, const V *__restrict__ acols, const V *__restrict__ v1,
+    const V *__restrict__ c, const V *__restrict__ b, V *__restrict__ v2,
+    double *__restrict__ part_n, V *__restrict__ part_d, int64_t parts) {
+    using O = OmpOps<V>;
+    __shared__ V sh[kOmpNT];
+    __shared__ double shn[kOmpNT];
+    const int64_t p = blockIdx.x, l = blockIdx.y;
+    const int64_t chunk = ceil_div(n, parts);
+    const int64_t lo = p * chunk, hi = (lo + chunk < n) ? lo + chunk : n;
+    double nrm = 0.0;
+    V dot = O::zero();
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kOmpNT) {
+        V v = c[l * n + j];
+        for (int i = 0; i < ii; ++i)
+            v = O::sub(v, O::mul(acols[(l * steps + i) * n + j], v1[l * steps + i]));
+        v2[l * n + j] = v;
+        nrm += O::abs2(v);
+        dot = O::add(dot, O::mul(O::conj(v), b[l * n + j]));
+    }
+    shn[threadIdx.x] = nrm;
+    __syncthreads();
+    for (int w = kOmpNT / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) shn[threadIdx.x] += shn[threadIdx.x + w];
+        __syncthreads();
+    }
+    dot = omp_block_sum<V>(dot, sh);
+    if (threadIdx.x == 0) {
+        part_n[l * parts + p] = shn[0];
+        part_d[l * parts + p] = dot;
+    }
+}
+
+// per problem: nrm2, y = <v2, b> / nrm2, coefficient updates
+template <typename V>
+__global__ void omp_coef_kernel(int64_t L, int steps, int ii, const double *__restrict__ part_n,
+                                const V *__restrict__ part_d, int64_t parts,
+                                const V *__restrict__ v1, V *__restrict__ x,
+                                double *__restrict__ nrm2, V *__restrict__ y) {
+    using O = OmpOps<V>;
+    const int64_t l = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (l >= L) return;
+    double s = 0.0;
+    V d = O::zero();
+    for (int64_t p = 0; p < parts; ++p) {
+        s += part_n[l * parts + p];
+        d = O::add(d, part_d[l * parts + p]);
+    }
+    const V yy = O::scale(d, 1.0 / s);
+    nrm2[l] = s;
+    y[l] = yy;
+    for (int i = 0; i < ii; ++i)
+        x[l * steps + i] = O::sub(x[l * steps + i], O::mul(yy, v1[l * steps + i]));
+    x[l * steps + ii] = O::add(x[l * steps + ii], yy);
+}
+
+// rank-1 update of P, new rows of P and Acols, residual; grid-stride over L n
+template <typename V>
+__global__ void omp_rank1_kernel(int64_t L, int64_t n, int steps, int ii,
+                                 const V *__restrict__ v1, const V *__restrict__ v2,
+                                 const V *__restrict__ c, const double *__restrict__ nrm2,
+                                 const V *__restrict__ y, V *__restrict__ P,
+                                 V *__restrict__ acols, V *__restrict__ r) {
+    using O = OmpOps<V>;
+    const int64_t total = L * n;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t l = q / n, j = q - l * n;
+        const V vv = v2[q];
+        const V w = O::conj(O::scale(vv, 1.0 / nrm2[l]));
+        for (int i = 0; i < ii; ++i) {
+            V &pij = P[(l * steps + i) * n + j];
+            pij = O::sub(pij, O::mul(w, v1[l * steps + i]));
+        }
+        P[(l * steps + ii) * n + j] = w;
+        acols[(l * steps + ii) * n + j] = c[q];
+        r[q] = O::sub(r[q], O::mul(y[l], vv));
+    }
+}
+
+// x_full[l, support[l, i]] = x[l, i], later steps overwriting earlier ones;
+// x_full zero-filled first
+template <typename V>
+__global__ void omp_zero_kernel(V *p, int64_t count) {
+    using O = OmpOps<V>;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < count;
+         q += int64_t(gridDim.x) * blockDim.x)
+        p[q] = O::zero();
+}
+
+template <typename V>
+__global__ void omp_result_kernel(int64_t L, int64_t m, int steps,
+                                  const int64_t *__restrict__ support,
+                                  const V *__restrict__ x, V *__restrict__ out) {
+    const int64_t l = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (l >= L) return;
+    for (int i = 0; i < steps; ++i) out[l * m + support[l * steps + i]] = x[l * steps + i];
+}
+
+template <typename V>
+static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int steps, int ii,
+                            const void *g, const double *inv_norm, int64_t *idx,
+                            int64_t *support, void *e, const void *c, const void *b, void *P,
+                            void *acols, void *r, void *work, void *x, void *x_full,
+                            hipStream_t s) {
+    const int64_t parts = (n + 65535) / 65536 < 1 ? 1 : ((n + 65535) / 65536 > 256 ? 256 : (n + 65535) / 65536);
+    // work: v1 (L steps V) | v2 (L n V) | part_d (L parts V) | y (L V) |
+    //       part_n (L parts f64) | nrm2 (L f64)
+    V *v1 = (V *)work;
+    V *v2 = v1 + L * steps;
+    V *part_d = v2 + L * n;
+    V *yv = part_d + L * parts;
+    double *part_n = (double *)(yv + L);
+    double *nrm2 = part_n + L * parts;
+    auto grid_of = [](int64_t total) {
+        int64_t bl = ceil_div(total, 256);
+        if (bl < 1) bl = 1;
+        if (bl > 8192) bl = 8192;
+        return unsigned(bl);
+    };
+    if (phase == 0) {  // select the new index, build the one-hot block
+        hipLaunchKernelGGL((omp_select_kernel<V>), dim3(unsigned(L)), dim3(kOmpNT), 0, s, m,
+                           (const V *)g, inv_norm, idx, support, steps, ii);
+        FM_CHECK_LAUNCH();
+        hipLaunchKernelGGL((omp_onehot_kernel<V>), dim3(grid_of(L * m)), dim3(256), 0, s, L, m,
+                           (const int64_t *)idx, (V *)e);
+        FM_CHECK_LAUNCH();
+        return FM_SUCCESS;
+    }
+    if (phase == 1) {  // pseudo-inverse update with the new column c
+        if (ii > 0) {
+            hipLaunchKernelGGL((omp_proj_kernel<V>), dim3(unsigned(ii), unsigned(L)),
+                               dim3(kOmpNT), 0, s, n, steps, (const V *)P, (const V *)c, v1);
+            FM_CHECK_LAUNCH();
+        }
+        hipLaunchKernelGGL((omp_resid_kernel<V>), dim3(unsigned(parts), unsigned(L)),
+                           dim3(kOmpNT), 0, s, n, steps, ii, (const V *)acols,
+                           (const V *)v1, (const V *)c, (const V *)b, v2, part_n, part_d,
+                           parts);
+        FM_CHECK_LAUNCH();
+        hipLaunchKernelGGL((omp_coef_kernel<V>), dim3(unsigned(ceil_div(L, 64))), dim3(64), 0, s,
+                           L, steps, ii, (const double *)part_n, (const V *)part_d, parts,
+                           (const V *)v1, (V *)x, nrm2, yv);
+        FM_CHECK_LAUNCH();
+        hipLaunchKernelGGL((omp_rank1_kernel<V>), dim3(grid_of(L * n)), dim3(256), 0, s, L, n,
+                           steps, ii, (const V *)v1, (const V *)v2, (const V *)c,
+                           (const double *)nrm2, (const V *)yv, (V *)P, (V *)acols, (V *)r);
+        FM_CHECK_LAUNCH();
+        return FM_SUCCESS;
+    }
+    // phase 2: scatter the coefficients into the (L, m) result
+    hipLaunchKernelGGL((omp_zero_kernel<V>), dim3(grid_of(L * m)), dim3(256), 0, s, (V *)x_full,
+                       L * m);
+    FM_CHECK_LAUNCH();
+    hipLaunchKernelGGL((omp_result_kernel<V>), dim3(unsigned(ceil_div(L, 64))), dim3(64), 0, s, L,
+                       m, steps, (const int64_t *)support, (const V *)x, (V *)x_full);
+    FM_CHECK_LAUNCH();
+    return FM_SUCCESS;
+}
+
+}  // namespace fm
+
+extern "C" fm_status fm_omp_workspace_size(int dtype, int64_t L, int64_t n, int steps,
+                                           int64_t *bytes) {
+    FM_REQUIRE(bytes != nullptr, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(dtype == FM_FLOAT64 || dtype == FM_COMPLEX128, FM_ERR_TYPE,
+               "OMP runs in float64 / complex128 (type %d)", dtype);
+    FM_REQUIRE(L >= 1 && n >= 1 && steps >= 1, FM_ERR_VALUE, "OMP: bad sizes");
+    const int64_t vs = dtype == FM_FLOAT64 ? 8 : 16;
+    const int64_t parts = (n + 65535) / 65536 < 1 ? 1 : ((n + 65535) / 65536 > 256 ? 256 : (n + 65535) / 65536);
+    *bytes = vs * (L * steps + L * n + L * parts + L) + 8 * (L * parts + L);
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_omp_step(int dtype, int phase, int64_t L, int64_t m, int64_t n, int steps,
+                                 int ii, const void *g, const double *inv_norm, int64_t *idx,
+                                 int64_t *support, void *e, const void *c, const void *b, void *P,
+                                 void *acols, void *r, void *work, int64_t work_bytes, void *x,
+                                 void *x_full, void *stream) {
+    FM_REQUIRE(dtype == FM_FLOAT64 || dtype == FM_COMPLEX128, FM_ERR_TYPE,
+               "OMP runs in float64 / complex128 (type %d)", dtype);
+    FM_REQUIRE(phase >= 0 && phase <= 2, FM_ERR_VALUE, "OMP: phase %d", phase);
+    FM_REQUIRE(L >= 1 && L <= 65535 && m >= 1 && n >= 1 && steps >= 1 && ii >= 0 && ii < steps,
+               FM_ERR_VALUE, "OMP: bad sizes or step");
+    int64_t need = 0;
+    fm_omp_workspace_size(dtype, L, n, steps, &need);
+    FM_REQUIRE(work && work_bytes >= need, FM_ERR_VALUE, "OMP workspace of %lld < %lld bytes",
+               (long long)work_bytes, (long long)need);
+    if (phase == 0) FM_REQUIRE(g && inv_norm && idx && support && e, FM_ERR_VALUE, "NULL argument");
+    if (phase == 1) FM_REQUIRE(c && b && P && acols && r && x, FM_ERR_VALUE, "NULL argument");
+    if (phase == 2) FM_REQUIRE(support && x && x_full, FM_ERR_VALUE, "NULL argument");
+    hipStream_t s = as_stream(stream);
+    if (dtype == FM_FLOAT64)
+        return omp_step_t<double>(phase, L, m, n, steps, ii, g, inv_norm, idx, support, e, c, b,
+                                  P, acols, r, work, x, x_full, s);
+    return omp_step_t<cpx<double>>(phase, L, m, n, steps, ii, g, inv_norm, idx, support, e, c,
+                                   b, P, acols, r, work, x, x_full, s);
+}

@@ -271,6 +271,25 @@ fm_status fm_lfsr_apply(int dtype, int order, int64_t n, const int64_t *scatter_
                         const int64_t *gather_idx, const void *x, int64_t ldx, void *y,
                         int64_t ldy, int64_t k, void *work, int64_t work_bytes, void *stream);
 
+/* OMP (algorithms/OMP.pyx:122-240): orthogonal matching pursuit for L
+ * problems in float64 / complex128 (dtype), the reference's rank-1
+ * pseudo-inverse update kept step for step.  Per step ii (0 <= ii < steps):
+ *   phase 0: idx[l] = first argmax_j |g[l, j]| inv_norm[j] (g = A^H r, an
+ *            (L, m) block), support[l, ii] = idx[l], e = the one-hot (L, m)
+ *            block of idx (the caller then forms c = A e, an (L, n) block)
+ *   phase 1: with the new columns c: v1 = P[:ii] c, v2 = c - Acols[:ii]^T v1,
+ *            y = <v2, b> / ||v2||^2, x[:ii] -= y v1, x[ii] += y,
+ *            P[:ii] -= v1 conj(v2)^T / ||v2||^2, P[ii] = conj(v2) / ||v2||^2,
+ *            Acols[ii] = c, r -= y v2
+ *   phase 2: x_full (L, m) = 0, x_full[l, support[l, i]] = x[l, i]
+ * P and Acols are (L, steps, n) row-major, b / r / c (L, n), x (L, steps),
+ * support (L, steps) int64; work as fm_omp_workspace_size reports. */
+fm_status fm_omp_workspace_size(int dtype, int64_t L, int64_t n, int steps, int64_t *bytes);
+fm_status fm_omp_step(int dtype, int phase, int64_t L, int64_t m, int64_t n, int steps, int ii,
+                      const void *g, const double *inv_norm, int64_t *idx, int64_t *support,
+                      void *e, const void *c, const void *b, void *P, void *acols, void *r,
+                      void *work, int64_t work_bytes, void *x, void *x_full, void *stream);
+
 /* STELA (algorithms/STELA.py:128-267): soft thresholding with exact line
  * search, one call per phase of a step, all on the stream, no allocation.
  * The k x n iterate x, bx, dlt and the k x m residual res / forward f are of

@@ -137,3 +137,49 @@ def stela_objective(A, b, x, lam):
     A = np.asarray(A)
     r = A.dot(x) - np.asarray(b).reshape(A.shape[0], -1)
     return 0.5 * np.sum(np.abs(r) ** 2, axis=0) + lam * np.sum(np.abs(x), axis=0)
+
+
+def omp(A, b, num_max_steps):
+    """OMP (algorithms/OMP.pyx:122-240) on a dense matrix, the reference's
+    rank-1 pseudo-inverse update step by step.  Returns (x, support)."""
+    A = np.asarray(A)
+    b = np.asarray(b)
+    if b.ndim > 2 or b.ndim < 1:
+        raise ValueError("Only n x m arrays are supported for OMP")
+    b2 = b.reshape((-1, 1)) if b.ndim == 1 else b
+    if num_max_steps <= 0:
+        raise ValueError("OMP would like to do at least one step for you")
+    s = int(num_max_steps)
+    n, m, L = A.shape[0], A.shape[1], b2.shape[1]
+    Cn = A * (1.0 / col_norms(A))[None, :]
+    rt = np.promote_types(np.promote_types(Cn.dtype, b2.dtype), np.float64)
+    xt = np.zeros((s, L), dtype=rt)
+    res = b2.astype(rt, copy=True)
+    sup = np.empty((s, L), dtype=np.intp)
+    pinv = np.zeros((s, n, L), dtype=rt)
+    acols = np.zeros((n, s, L), dtype=rt)
+    for ii in range(s):
+        c = np.abs(Cn.conj().T.dot(res))
+        new = np.argmax(c, axis=0)
+        sup[ii, :] = new
+        cols = A[:, new]
+        acols[:, ii, :] = cols
+        if ii == 0:
+            v2 = cols
+            v2n = (v2 / np.linalg.norm(v2, axis=0) ** 2).conj()
+            v2y = np.einsum('ji,ji->i', v2n, b2)
+            xt[0, :] = v2y
+            pinv[0, :, :] = v2n
+        else:
+            v1 = np.einsum('ijk,jk->ik', pinv[:ii, :, :], cols)
+            v2 = cols - np.einsum('ijk,jk->ik', acols[:, :ii, :], v1)
+            v2n = (v2 / np.linalg.norm(v2, axis=0) ** 2).conj()
+            v2y = np.einsum('ji,ji->i', v2n, b2)
+            xt[:ii, :] -= v2y * v1
+            xt[ii, :] += v2y
+            pinv[:ii, :, :] -= np.einsum('ik,jk->jik', v2n, v1)
+            pinv[ii, :, :] = v2n
+        res -= v2y * v2
+    x = np.zeros((m, L), dtype=rt)
+    x[sup, np.arange(L)] = xt
+    return x, sup

@@ -331,3 +331,78 @@ def test_matrix_properties_rows_normalized_gram(fm):
         fm.Diag(np.array([1.0, 0.0])).colNormalized
     alg = fma.ISTA(fm.Fourier(8))
     assert alg.nbytes >= 0
+
+
+# ------------------------------------------------------------------- OMP
+ALLTYPES = [np.int8, np.int16, np.int32, np.int64, np.float32, np.float64, np.complex64,
+            np.complex128]
+
+
+@pytest.mark.parametrize("typeA", ALLTYPES)
+def test_omp_reference_configuration(fm, typeA):
+    """OMP.pyx:241-290: Product(Matrix(U(-100, 100)) 8 x 8, Hadamard(3),
+    typeExpansion=typeA), 3-sparse truth, 3 steps, against the oracle on the
+    exact dense product."""
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(71 + ALLTYPES.index(typeA))
+    say("OMP reference config", np.dtype(typeA).name)
+    M = rng.uniform(-100, 100, (8, 8)).astype(typeA)
+    A = fm.Product(fm.Matrix(M), fm.Hadamard(3), typeExpansion=typeA)
+    Ad = A.reference()
+    x0 = np.zeros((8, 2), dtype=np.promote_types(typeA, np.float64))
+    for j in range(2):
+        x0[rng.choice(8, 3, replace=False), j] = rng.uniform(1, 2, 3)
+    b = Ad @ x0
+    out = fma.OMP(A, numMaxSteps=3).process(b)
+    ref, sup = oalg.omp(Ad, b, 3)
+    assert out.shape == ref.shape and out.dtype == ref.dtype, (out.dtype, ref.dtype)
+    assert accept.max_rel_error(out, ref) < 1e-9
+
+
+def test_omp_reference_unit_test(fm):
+    """test/algorithm.py:23-90: 40 x 100 Gaussian, 6-sparse: support and
+    values recovered; argument errors."""
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(97)
+    say("OMP recovery")
+    A = rng.standard_normal((40, 100))
+    x = np.zeros(100)
+    supp = rng.choice(100, 6, replace=False)
+    x[supp] = rng.standard_normal(6)
+    alg = fma.OMP(fm.Matrix(A), numMaxSteps=6)
+    y = alg.process(A @ x)
+    np.testing.assert_array_equal(np.sort(np.nonzero(y[:, 0])[0]), np.sort(supp))
+    np.testing.assert_allclose(y[:, 0], x, rtol=1e-10, atol=1e-13)
+    alg.process(rng.standard_normal((40, 7)))
+    with pytest.raises(ValueError):
+        alg.process(rng.standard_normal((40, 40, 7)))
+    with pytest.raises(ValueError):
+        alg.process(np.zeros(()))
+    alg.updateParameters(numMaxSteps=0)
+    with pytest.raises(ValueError):
+        alg.process(rng.standard_normal(40))
+
+
+def test_omp_structured_operators(fm):
+    """Complex Circulant deconvolution (the docstring's example shape) and a
+    Fourier-Hadamard product with several columns, against the oracle, and a
+    device-resident call."""
+    import torch
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(98)
+    n = 512
+    t = np.linspace(0, 20 * np.pi, n)
+    c = np.cos(2 * t) * np.exp(-t ** 2 / 50) + 0.3j * np.sin(3 * t) * np.exp(-t ** 2 / 40)
+    C = fm.Circulant(c)
+    x = np.zeros((n, 3), dtype=np.complex128)
+    for j in range(3):
+        x[rng.choice(n, 3, replace=False), j] = rng.uniform(1, 2, 3)
+    b = C.reference() @ x
+    say("OMP circulant")
+    out = fma.OMP(C, numMaxSteps=3).process(b)
+    ref, _ = oalg.omp(C.reference(), b, 3)
+    assert accept.max_rel_error(out, ref) < 1e-9
+    bd = torch.from_numpy(np.ascontiguousarray(b.T)).cuda().t()
+    dev = fma.OMP(C, numMaxSteps=3).process(bd)
+    assert torch.is_tensor(dev) and dev.is_cuda
+    np.testing.assert_array_equal(dev.cpu().numpy(), out)

@@ -617,3 +617,37 @@ def test_cpu_path_lfsr_hadamard_factorisation(poly, start):
     x = np.random.default_rng(poly).integers(-3, 4, (C.shape[0], 4)).astype(np.int64)
     np.testing.assert_array_equal(op.forward(x), C @ x)
     np.testing.assert_array_equal(op.backward(x), C.T @ x)
+
+
+def test_omp_exact_recovery_reference_test():
+    # test/algorithm.py:23-70: 40 x 100 Gaussian, 6-sparse: support and
+    # values recovered (rtol 1e-12 as the reference's own test states)
+    rng = np.random.default_rng(95)
+    A = rng.standard_normal((40, 100))
+    x = np.zeros(100)
+    supp = rng.choice(100, 6, replace=False)
+    x[supp] = rng.standard_normal(6)
+    y, sup = oalg.omp(A, A @ x, 6)
+    assert sorted(sup[:, 0]) == sorted(supp)
+    np.testing.assert_allclose(y[:, 0], x, rtol=1e-12, atol=1e-15)
+
+
+def test_omp_least_squares_and_orthogonal_residual():
+    # after s steps the coefficients are the least-squares fit on the
+    # selected columns and the residual is orthogonal to them; the first
+    # pick is the largest normalised correlation
+    rng = np.random.default_rng(96)
+    for cplx in (False, True):
+        A = rng.standard_normal((30, 50)) * rng.uniform(0.5, 3, 50)
+        b = rng.standard_normal((30, 2))
+        if cplx:
+            A = A + 1j * rng.standard_normal((30, 50))
+            b = b + 1j * rng.standard_normal((30, 2))
+        y, sup = oalg.omp(A, b, 5)
+        for l in range(2):
+            S = sup[:, l]
+            assert sup[0, l] == np.argmax(np.abs(A.conj().T @ b[:, l]) / np.linalg.norm(A, axis=0))
+            xs = np.linalg.lstsq(A[:, S], b[:, l], rcond=None)[0]
+            np.testing.assert_allclose(y[S, l], xs, rtol=1e-10, atol=1e-12)
+            r = b[:, l] - A @ y[:, l]
+            assert np.max(np.abs(A[:, S].conj().T @ r)) < 1e-10 * np.linalg.norm(b[:, l])
