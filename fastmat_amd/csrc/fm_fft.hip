@@ -396,6 +396,8 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         return FM_SUCCESS;
     }
     FM_REQUIRE(io.work != nullptr, FM_ERR_VALUE, "multi-pass transform needs a workspace");
+    // a.B below only sets the vector-fast addressing flags; the sequences
+    // per tile are fixed by the pass kernel's tile shape (normalize_pass)
     const int64_t N1 = int64_t(1) << g.logN1, N2 = int64_t(1) << g.logN2;
     cpx<T> *tmp = (cpx<T> *)io.work;
     // pass A: strided N1-point FFTs over n1 for every n2, twiddle W_L^{n2 k1}
@@ -403,7 +405,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN1;
         a.O = io.k; a.J = N2;
-        a.B = pick_batch(g.logN1, N2, max_tile, knob("FASTMAT_AMD_TILE_A", 16));
+        a.B = pick_batch(g.logN1, N2, max_tile, 16);
         a.ntiles = ceil_div(io.k * N2, a.B);
         a.x = (const cpx<T> *)io.x;
         a.in_os = io.ldx; a.in_js = 1; a.in_ns = N2;
@@ -424,7 +426,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN2;
         a.O = io.k; a.J = N1;
-        a.B = pick_batch(g.logN2, N1, max_tile, knob("FASTMAT_AMD_TILE_B", 16));
+        a.B = pick_batch(g.logN2, N1, max_tile, 16);
         a.ntiles = ceil_div(io.k * N1, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = N2; a.in_ns = 1;
@@ -444,7 +446,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN2;
         a.O = io.k; a.J = N1;
-        a.B = pick_batch(g.logN2, N1, max_tile, knob("FASTMAT_AMD_TILE_CONV", 2));
+        a.B = pick_batch(g.logN2, N1, max_tile, 2);
         a.ntiles = ceil_div(io.k * N1, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = N2; a.in_ns = 1;
@@ -463,7 +465,7 @@ static fm_status build_pipeline(const Geometry &g, const PipeIO &io,
         TileArgs<T> a = blank_args<T>();
         a.logN = g.logN1;
         a.O = io.k; a.J = N2;
-        a.B = pick_batch(g.logN1, N2, max_tile, knob("FASTMAT_AMD_TILE_A", 16));
+        a.B = pick_batch(g.logN1, N2, max_tile, 16);
         a.ntiles = ceil_div(io.k * N2, a.B);
         a.x = tmp;
         a.in_os = L; a.in_js = 1; a.in_ns = N2;
