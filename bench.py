@@ -47,7 +47,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="fourier",
                    choices=["fourier", "hadamard", "circulant", "kron", "batch", "copy",
-                            "multilevel", "fista", "blockdiag", "lfsr"])
+                            "multilevel", "fista", "blockdiag", "lfsr", "batch_grad"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cols", type=int, default=None, help="override the column count")
     return p.parse_args()
@@ -544,10 +544,66 @@ def workload_lfsr(args, rng):
                 dtype="f32", metric=METRIC)
 
 
+def workload_batch_grad(args, rng):
+    """The uneven batch through the autograd Function: loss = sum_i
+    ||A_i x_i - y_i||^2, then its gradients with respect to every x_i and
+    every generator (the batch's forward, the batch's adjoint and one
+    generator-gradient FFT correlation per operator)."""
+    import torch
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    spec = make_uneven_batch()
+    ops, ks = build_batch_ops(spec)
+    batch = ConvBatch(ops, ks)
+    gens = [torch.from_numpy(np.asarray(e[4], dtype=np.complex64)).cuda().requires_grad_(True)
+            for e in spec]
+    xs = [torch.from_numpy(crandn(rng, (op.numCols, k))).cuda().requires_grad_(True)
+          for op, k in zip(ops, ks)]
+    ys = [torch.from_numpy(crandn(rng, (op.numRows, k))).cuda() for op, k in zip(ops, ks)]
+    # forward + adjoint inputs/outputs, and per operator the generator
+    # gradient's reads of x and the residual and its write of the generator
+    bytes_step = (batch.algorithmic_bytes(False) + batch.algorithmic_bytes(True) +
+                  sum((op.numRows + op.numCols) * k * 8 + (op.numRows + op.numCols - 1) * 8
+                      for op, k in zip(ops, ks)))
+
+    def step():
+        for t in gens + xs:
+            t.grad = None
+        outs = fa.batch_conv_matvec(batch, gens, xs)
+        loss = sum(((o - y).abs() ** 2).sum() for o, y in zip(outs, ys))
+        loss.backward()
+        return loss
+
+    def check():
+        from oracle import accept
+        step()
+        torch.cuda.synchronize()
+        errs = []
+        from oracle import fast64
+        for i in range(0, len(ops), 51):
+            kind, m, n, k, gen = spec[i]
+            x = xs[i].detach().cpu().numpy().astype(np.complex128)
+            y = ys[i].cpu().numpy().astype(np.complex128)
+            if kind == "circulant":
+                r = fast64.circulant(gen, x) - y
+                ref = 2 * fast64.circulant(gen, r, adjoint=True)
+            else:
+                r = fast64.toeplitz(gen, m, x) - y
+                ref = 2 * fast64.toeplitz(gen, m, r, adjoint=True)
+            errs.append(accept.max_rel_error(xs[i].grad.cpu().numpy(), ref))
+        return {"x_grad_sampled_max": max(errs)}
+
+    cfg = {"workload": "uneven_batch_256_autograd_loss_grad_x_and_generators_c64",
+           "operators": len(ops), "total_columns": int(sum(ks))}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=None, cfg=cfg,
+                dtype="c64", metric=METRIC)
+
+
 WORKLOADS = {"copy": workload_copy, "fourier": workload_fourier, "hadamard": workload_hadamard,
              "circulant": workload_circulant, "kron": workload_kron, "batch": workload_batch,
              "multilevel": workload_multilevel, "fista": workload_fista,
-             "blockdiag": workload_blockdiag, "lfsr": workload_lfsr}
+             "blockdiag": workload_blockdiag, "lfsr": workload_lfsr,
+             "batch_grad": workload_batch_grad}
 
 
 def time_cpu(legs):
