@@ -123,9 +123,9 @@ class _BatchConvMatvec(torch.autograd.Function):
             gx = [t.t() for t in batch.backward(G)]
         ggen = [None] * n
         if need_gen:
-            ggen = [op.generator_grad(x, g).to(dt) if cplx else op.generator_grad(x, g).real.to(dt)
-                    for op, x, g, cplx, dt in zip(batch.ops, X, G, ctx.gen_complex,
-                                                   ctx.gen_dtype)]
+            raw = batch.generator_grads(list(X), G)
+            ggen = [r.to(dt) if cplx else r.real.to(dt)
+                    for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
         return (None, *ggen, *gx)
 
 

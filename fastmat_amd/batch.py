@@ -202,6 +202,48 @@ class ConvBatch:
         # while the same addresses are handed in again
         self._bound[int(adjoint)] = key
 
+    def generator_grads(self, xs, gs):
+        """Gradients of a real loss with respect to every operator's
+        generator, given the forward inputs xs[i] (k_i, cols_i) and the
+        output gradients gs[i] (k_i, rows_i) as contiguous blocks: what
+        ``op.generator_grad(x, g)`` returns per operator, for the whole batch
+        in one batched program (fm_batch_grad_*)."""
+        n = len(self.ops)
+        for i, (op, k) in enumerate(zip(self.ops, self.ks)):
+            if tuple(xs[i].shape) != (k, op.numCols) or tuple(gs[i].shape) != (k, op.numRows):
+                raise ValueError("ConvBatch: gradient block %d has the wrong shape" % i)
+            if not (xs[i].is_contiguous() and gs[i].is_contiguous()):
+                raise ValueError("ConvBatch: gradient blocks must be contiguous")
+            if D.np_dtype(xs[i].dtype) != self.dtype or D.np_dtype(gs[i].dtype) != self.dtype:
+                raise TypeError("ConvBatch: gradient blocks must be %s" % self.dtype)
+        # the program writes into one persistent buffer (stable addresses, so
+        # the binding is reused); the caller gets views of one copy of it
+        if getattr(self, "_gout", None) is None:
+            lens = [int(op._gen128.size) for op in self.ops]
+            self._goff = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+            self._gout = D.empty(1, int(self._goff[-1]), self.dtype)[0]
+        off = self._goff
+        grads = [self._gout[int(off[i]):int(off[i + 1])] for i in range(n)]
+        if getattr(self, "_gws", None) is None:
+            need = ctypes.c_int64()
+            _lib.call("fm_batch_grad_workspace_size", self._h, ctypes.byref(need))
+            self._gws = D.workspace(need.value)
+            self._gws_bytes = need.value
+        key = tuple(t.data_ptr() for t in xs) + tuple(t.data_ptr() for t in gs) + \
+            tuple(t.data_ptr() for t in grads)
+        if getattr(self, "_gbound", None) != key:
+            px = (ctypes.c_void_p * n)(*[t.data_ptr() for t in xs])
+            pg = (ctypes.c_void_p * n)(*[t.data_ptr() for t in gs])
+            po = (ctypes.c_void_p * n)(*[t.data_ptr() for t in grads])
+            lx = (ctypes.c_int64 * n)(*[t.shape[1] for t in xs])
+            lg = (ctypes.c_int64 * n)(*[t.shape[1] for t in gs])
+            _lib.call("fm_batch_grad_bind", self._h, px, lx, pg, lg, po, _lib.ptr(self._gws),
+                      self._gws_bytes, D.stream())
+            self._gbound = key
+        _lib.call("fm_batch_grad_apply", self._h, D.stream())
+        out = self._gout.clone()
+        return [out[int(off[i]):int(off[i + 1])] for i in range(n)]
+
     def set_grid(self, workgroups_per_cu):
         _lib.call("fm_batch_set_grid", self._h, int(workgroups_per_cu))
         self.grid_per_cu = int(workgroups_per_cu)

@@ -1121,6 +1121,11 @@ struct fm_batch_plan {
     int grid_per_cu;
     int ncu;
     BatchDir dir[2];
+    // generator-gradient program: forward FFTs of the inputs and residuals,
+    // the per-operator correlation sum, inverse FFTs, the fold
+    BatchDir gdir[2];
+    void *gtab;
+    int64_t g_maxL, g_maxgen;
     // side streams (default priority) for the independent length classes of
     // one phase, forked from and joined back into the caller's stream
     static constexpr int kSide = 3;
@@ -1208,46 +1213,26 @@ extern "C" fm_status fm_batch_set_grid(fm_batch_plan *p, int workgroups_per_cu) 
     return FM_SUCCESS;
 }
 
+// Descriptors and work-item groups of one batched program: every pipeline's
+// passes, pass q of a multi-pass pipeline in phase q and the one-pass
+// pipelines in the last phase (they depend on nothing; measured on the
+// uneven batch: 2.75 ms per step in the first phase, 2.68 in the last).
 template <typename T>
-static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *ins,
-                              const int64_t *ld_in, void *const *outs, const int64_t *ld_out,
-                              void *workspace, hipStream_t s) {
-    BatchDir &d = p->dir[adjoint ? 1 : 0];
+static fm_status batch_build_dir(BatchDir &d, std::vector<std::vector<PassSpec<T>>> &pipes,
+                                 hipStream_t s) {
     batch_free_dir(d);
     std::vector<BatchDesc<T>> descs;
     std::vector<int> phase_of;
     int last_phase = 0;
-    for (int i = 0; i < p->count; ++i) {
-        const int np = p->plans[i]->geo.passes;
-        if (np - 1 > last_phase) last_phase = np - 1;
-    }
-    for (int i = 0; i < p->count; ++i) {
-        const fm_conv_plan *c = p->plans[i];
-        const int64_t nin = adjoint ? c->rows : c->cols;
-        const int64_t mout = adjoint ? c->cols : c->rows;
-        FM_REQUIRE(ins[i] && outs[i], FM_ERR_VALUE, "batch entry %d: NULL buffer", i);
-        FM_REQUIRE(ld_in[i] >= nin && ld_out[i] >= mout, FM_ERR_VALUE,
-                   "batch entry %d: leading dimension too small", i);
-        PipeIO io;
-        memset(&io, 0, sizeof(io));
-        io.x = ins[i]; io.ldx = ld_in[i]; io.y = outs[i]; io.ldy = ld_out[i]; io.k = p->ks[i];
-        io.n_in = nin; io.m_out = mout; io.D = c->D; io.conj_d = adjoint ? 1 : 0;
-        io.work = c->geo.passes > 1 ? (char *)workspace + p->ws_off[i] : nullptr;
-        FM_REQUIRE(c->geo.passes == 1 || workspace, FM_ERR_VALUE, "batch needs a workspace");
-        std::vector<PassSpec<T>> passes;
-        fm_status st = build_pipeline<T>(c->geo, io, &passes);
-        if (st) return st;
-        // a one-pass operator depends on nothing: it joins the last phase,
-        // alongside the final passes of the multi-pass operators (measured
-        // on the uneven batch: 2.75 ms per step in the first phase, 2.68 in
-        // the last)
-        const int single_phase = last_phase;
-        for (size_t q = 0; q < passes.size(); ++q) {
-            TileArgs<T> a = passes[q].a;
-            st = normalize_pass(a);
+    for (auto &pl : pipes)
+        if (int(pl.size()) - 1 > last_phase) last_phase = int(pl.size()) - 1;
+    for (auto &pl : pipes) {
+        for (size_t q = 0; q < pl.size(); ++q) {
+            TileArgs<T> a = pl[q].a;
+            fm_status st = normalize_pass(a);
             if (st) return st;
-            descs.push_back(BatchDesc<T>{a, passes[q].conv ? 1 : 0});
-            phase_of.push_back(passes.size() == 1 ? single_phase : int(q));
+            descs.push_back(BatchDesc<T>{a, pl[q].conv ? 1 : 0});
+            phase_of.push_back(pl.size() == 1 ? last_phase : int(q));
         }
     }
     // work items grouped by (phase, length, conv); groups of a phase run
@@ -1294,6 +1279,31 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
     FM_CHECK_HIP(hipStreamSynchronize(s));
     d.bound = true;
     return FM_SUCCESS;
+}
+
+template <typename T>
+static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *ins,
+                              const int64_t *ld_in, void *const *outs, const int64_t *ld_out,
+                              void *workspace, hipStream_t s) {
+    BatchDir &d = p->dir[adjoint ? 1 : 0];
+    std::vector<std::vector<PassSpec<T>>> pipes(p->count);
+    for (int i = 0; i < p->count; ++i) {
+        const fm_conv_plan *c = p->plans[i];
+        const int64_t nin = adjoint ? c->rows : c->cols;
+        const int64_t mout = adjoint ? c->cols : c->rows;
+        FM_REQUIRE(ins[i] && outs[i], FM_ERR_VALUE, "batch entry %d: NULL buffer", i);
+        FM_REQUIRE(ld_in[i] >= nin && ld_out[i] >= mout, FM_ERR_VALUE,
+                   "batch entry %d: leading dimension too small", i);
+        PipeIO io;
+        memset(&io, 0, sizeof(io));
+        io.x = ins[i]; io.ldx = ld_in[i]; io.y = outs[i]; io.ldy = ld_out[i]; io.k = p->ks[i];
+        io.n_in = nin; io.m_out = mout; io.D = c->D; io.conj_d = adjoint ? 1 : 0;
+        io.work = c->geo.passes > 1 ? (char *)workspace + p->ws_off[i] : nullptr;
+        FM_REQUIRE(c->geo.passes == 1 || workspace, FM_ERR_VALUE, "batch needs a workspace");
+        fm_status st = build_pipeline<T>(c->geo, io, &pipes[i]);
+        if (st) return st;
+    }
+    return batch_build_dir<T>(d, pipes, s);
 }
 
 extern "C" fm_status fm_batch_bind(fm_batch_plan *p, int adjoint, const void *const *ins,
@@ -1434,10 +1444,177 @@ extern "C" fm_status fm_batch_apply(const fm_batch_plan *p, int adjoint, void *s
     return batch_apply_t<double>(p, d, as_stream(stream));
 }
 
+// ------------------------------------------------- batched generator gradient
+// For every operator i of the batch (the per-operator fm_conv_generator_grad
+// at once): xh = FFT(x'), gh = FFT(g') over its k_i columns, acc = sum_c
+// gh conj(xh), r = unnormalised inverse FFT of acc, folded onto the generator
+// and scaled by 1/L.  The FFTs run as two batched programs on the batch
+// engine (descriptor table, persistent kernels per length class); the sum
+// and the fold are one launch each over all operators.
+namespace fm {
+template <typename T> struct GradEntry {
+    const cpx<T> *gh, *xh, *r;
+    cpx<T> *acc, *out;
+    int64_t L, k, head, tail, gen_len;
+    int kind;
+};
+
+template <typename T>
+__global__ void batch_corr_reduce_kernel(const GradEntry<T> *__restrict__ tab) {
+    const GradEntry<T> e = tab[blockIdx.y];
+    const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (q >= e.L) return;
+    cpx<T> acc = mk(T(0), T(0));
+    for (int64_t c = 0; c < e.k; ++c) acc = acc + cmulc(e.gh[c * e.L + q], e.xh[c * e.L + q]);
+    e.acc[q] = acc;
+}
+
+template <typename T>
+__global__ void batch_fold_kernel(const GradEntry<T> *__restrict__ tab) {
+    const GradEntry<T> e = tab[blockIdx.y];
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= e.gen_len) return;
+    cpx<T> v;
+    if (e.kind == 0) {
+        v = e.r[i];
+        if (e.tail > 0 && i >= 1) v = v + e.r[e.L - e.tail + i - 1];
+    } else {
+        v = i < e.head ? e.r[i] : e.r[e.L - e.tail + (i - e.head)];
+    }
+    e.out[i] = scale(v, T(1.0 / double(e.L)));
+}
+
+static int64_t grad_ws_entry(const fm_conv_plan *c, int64_t k, int64_t *parts) {
+    const int64_t e = int64_t(esize(c->prec)), L = c->L;
+    const bool multi = c->geo_fft.passes > 1;
+    auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+    parts[0] = up(e * L * k);                 // xh
+    parts[1] = up(e * L * k);                 // gh
+    parts[2] = multi ? up(e * L * k) : 0;     // four-step intermediate of FFT(x')
+    parts[3] = multi ? up(e * L * k) : 0;     // ... of FFT(g')
+    parts[4] = up(e * L);                     // acc
+    parts[5] = up(e * L);                     // r
+    parts[6] = multi ? up(e * L) : 0;         // intermediate of the inverse
+    int64_t t = 0;
+    for (int q = 0; q < 7; ++q) t += parts[q];
+    return t;
+}
+
+template <typename T>
+static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, const int64_t *ldx,
+                                   const void *const *gs, const int64_t *ldg,
+                                   void *const *grads, char *ws, hipStream_t s) {
+    std::vector<std::vector<PassSpec<T>>> fwd(2 * size_t(p->count)), inv(p->count);
+    std::vector<GradEntry<T>> tab(p->count);
+    int64_t off = 0, maxL = 1, maxgen = 1;
+    for (int i = 0; i < p->count; ++i) {
+        const fm_conv_plan *c = p->plans[i];
+        const int64_t k = p->ks[i], L = c->L;
+        FM_REQUIRE(xs[i] && gs[i] && grads[i], FM_ERR_VALUE, "batch entry %d: NULL buffer", i);
+        FM_REQUIRE(ldx[i] >= c->cols && ldg[i] >= c->rows, FM_ERR_VALUE,
+                   "batch entry %d: leading dimension too small", i);
+        int64_t part[7];
+        const int64_t tot = grad_ws_entry(c, k, part);
+        char *b = ws + off;
+        cpx<T> *xh = (cpx<T> *)b, *gh = (cpx<T> *)(b + part[0]);
+        char *tx = b + part[0] + part[1], *tg = tx + part[2];
+        cpx<T> *acc = (cpx<T> *)(tg + part[3]), *r = (cpx<T> *)((char *)acc + part[4]);
+        char *tr = (char *)r + part[5];
+        off += tot;
+        PipeIO io;
+        memset(&io, 0, sizeof(io));
+        io.x = xs[i]; io.ldx = ldx[i]; io.y = xh; io.ldy = L; io.k = k;
+        io.n_in = c->cols; io.m_out = L; io.work = part[2] ? tx : nullptr;
+        fm_status st = build_pipeline<T>(c->geo_fft, io, &fwd[2 * size_t(i)]);
+        if (st) return st;
+        io.x = gs[i]; io.ldx = ldg[i]; io.y = gh; io.n_in = c->rows; io.work = part[3] ? tg : nullptr;
+        st = build_pipeline<T>(c->geo_fft, io, &fwd[2 * size_t(i) + 1]);
+        if (st) return st;
+        PipeIO iv;
+        memset(&iv, 0, sizeof(iv));
+        iv.x = acc; iv.ldx = L; iv.y = r; iv.ldy = L; iv.k = 1; iv.n_in = L; iv.m_out = L;
+        iv.conj_io = 1; iv.work = part[6] ? tr : nullptr;
+        st = build_pipeline<T>(c->geo_fft, iv, &inv[i]);
+        if (st) return st;
+        tab[i] = GradEntry<T>{gh, xh, r, acc, (cpx<T> *)grads[i], L, k, c->head, c->tail,
+                              c->gen_len, c->kind};
+        if (L > maxL) maxL = L;
+        if (c->gen_len > maxgen) maxgen = c->gen_len;
+    }
+    fm_status st = batch_build_dir<T>(p->gdir[0], fwd, s);
+    if (st) return st;
+    st = batch_build_dir<T>(p->gdir[1], inv, s);
+    if (st) return st;
+    if (p->gtab) (void)hipFree(p->gtab);
+    p->gtab = nullptr;
+    FM_CHECK_HIP(hipMalloc(&p->gtab, sizeof(GradEntry<T>) * tab.size()));
+    FM_CHECK_HIP(hipMemcpyAsync(p->gtab, tab.data(), sizeof(GradEntry<T>) * tab.size(),
+                                hipMemcpyHostToDevice, s));
+    FM_CHECK_HIP(hipStreamSynchronize(s));
+    p->g_maxL = maxL;
+    p->g_maxgen = maxgen;
+    return FM_SUCCESS;
+}
+
+template <typename T>
+static fm_status batch_grad_apply_t(const fm_batch_plan *p, hipStream_t s) {
+    fm_status st = batch_apply_t<T>(p, p->gdir[0], s);
+    if (st) return st;
+    hipLaunchKernelGGL(batch_corr_reduce_kernel<T>,
+                       dim3(unsigned(ceil_div(p->g_maxL, 256)), unsigned(p->count)), dim3(256), 0,
+                       s, (const GradEntry<T> *)p->gtab);
+    FM_CHECK_LAUNCH();
+    st = batch_apply_t<T>(p, p->gdir[1], s);
+    if (st) return st;
+    hipLaunchKernelGGL(batch_fold_kernel<T>,
+                       dim3(unsigned(ceil_div(p->g_maxgen, 256)), unsigned(p->count)), dim3(256),
+                       0, s, (const GradEntry<T> *)p->gtab);
+    FM_CHECK_LAUNCH();
+    return FM_SUCCESS;
+}
+}  // namespace fm
+
+extern "C" fm_status fm_batch_grad_workspace_size(const fm_batch_plan *p, int64_t *bytes) {
+    FM_REQUIRE(p && bytes, FM_ERR_VALUE, "NULL argument");
+    int64_t t = 0, part[7];
+    for (int i = 0; i < p->count; ++i) t += grad_ws_entry(p->plans[i], p->ks[i], part);
+    *bytes = t;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_grad_bind(fm_batch_plan *p, const void *const *xs,
+                                        const int64_t *ldx, const void *const *gs,
+                                        const int64_t *ldg, void *const *grads, void *workspace,
+                                        int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p && xs && ldx && gs && ldg && grads && workspace, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(p->count <= 65535, FM_ERR_VALUE, "batch gradient: more than 65535 operators");
+    int64_t need = 0;
+    fm_batch_grad_workspace_size(p, &need);
+    FM_REQUIRE(workspace_bytes >= need, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)need);
+    if (p->prec == FM_COMPLEX64)
+        return batch_grad_bind_t<float>(p, xs, ldx, gs, ldg, grads, (char *)workspace,
+                                        as_stream(stream));
+    return batch_grad_bind_t<double>(p, xs, ldx, gs, ldg, grads, (char *)workspace,
+                                     as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_grad_apply(const fm_batch_plan *p, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->gdir[0].bound && p->gdir[1].bound && p->gtab, FM_ERR_VALUE,
+               "batch gradient used before fm_batch_grad_bind");
+    if (p->prec == FM_COMPLEX64) return batch_grad_apply_t<float>(p, as_stream(stream));
+    return batch_grad_apply_t<double>(p, as_stream(stream));
+}
+
 extern "C" fm_status fm_batch_plan_destroy(fm_batch_plan *p) {
     if (!p) return FM_SUCCESS;
     batch_free_dir(p->dir[0]);
     batch_free_dir(p->dir[1]);
+    batch_free_dir(p->gdir[0]);
+    batch_free_dir(p->gdir[1]);
+    if (p->gtab) (void)hipFree(p->gtab);
     for (int q = 0; q < fm_batch_plan::kSide; ++q) {
         if (p->side[q]) (void)hipStreamDestroy(p->side[q]);
         if (p->join_ev[q]) (void)hipEventDestroy(p->join_ev[q]);

@@ -185,6 +185,20 @@ fm_status fm_batch_bind(fm_batch_plan *plan, int adjoint, const void *const *inp
                         const int64_t *ld_out, void *workspace,
                         int64_t workspace_bytes, void *stream);
 fm_status fm_batch_apply(const fm_batch_plan *plan, int adjoint, void *stream);
+/* Batched generator gradient (pattern (a) over the uneven batch): for every
+ * operator i, with x_i its forward input block (k_i x cols_i, leading
+ * dimension ldx[i]) and g_i the gradient of the loss with respect to its
+ * output (k_i x rows_i, ldg[i]), grads[i] (gen_len_i elements, the plan's
+ * generator layout) receives what fm_conv_generator_grad returns for that
+ * operator alone.  fm_batch_grad_bind builds the program for one set of
+ * device pointers (allocates its tables, synchronises its stream);
+ * fm_batch_grad_apply only launches kernels (capturable).  The workspace
+ * holds the operators' spectra and intermediates. */
+fm_status fm_batch_grad_workspace_size(const fm_batch_plan *plan, int64_t *bytes);
+fm_status fm_batch_grad_bind(fm_batch_plan *plan, const void *const *xs, const int64_t *ldx,
+                             const void *const *gs, const int64_t *ldg, void *const *grads,
+                             void *workspace, int64_t workspace_bytes, void *stream);
+fm_status fm_batch_grad_apply(const fm_batch_plan *plan, void *stream);
 fm_status fm_batch_plan_destroy(fm_batch_plan *plan);
 
 /* --------------------------------------------------------------- dense

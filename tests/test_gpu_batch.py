@@ -248,3 +248,25 @@ def test_autograd_batch_and_matvec(torch_mod):
     Pd = np.fft.fft(np.eye(64), axis=0) @ np.diag(P.content[1].vecD)
     ref = 2 * Pd.conj().T @ (Pd @ x.detach().cpu().numpy())
     assert accept.max_rel_error(x.grad.cpu().numpy(), ref) < 1e-5
+
+
+def test_batched_generator_gradients_match_per_operator(torch_mod):
+    """ConvBatch.generator_grads (one batched program) against each
+    operator's own generator_grad, on a mixed batch of one- and multi-pass
+    Circulant / Toeplitz operators, called twice (the binding is reused)."""
+    torch = torch_mod
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(12)
+    spec = small_spec(12, count=10, hi=20000, kmax=4)
+    ops, ks = build(spec)
+    batch = ConvBatch(ops, ks)
+    say("batched generator gradients", len(ops), "ops")
+    for rep in range(2):
+        xs = [torch.complex(torch.randn(k, op.numCols, device="cuda"),
+                            torch.randn(k, op.numCols, device="cuda")) for op, k in zip(ops, ks)]
+        gs = [torch.complex(torch.randn(k, op.numRows, device="cuda"),
+                            torch.randn(k, op.numRows, device="cuda")) for op, k in zip(ops, ks)]
+        got = batch.generator_grads(xs, gs)
+        for op, x, g, r in zip(ops, xs, gs, got):
+            ref = op.generator_grad(x, g)
+            assert accept.max_rel_error(r.cpu().numpy(), ref.cpu().numpy()) < 1e-5, (rep, op)
