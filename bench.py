@@ -557,9 +557,11 @@ def workload_batch_grad(args, rng):
     batch = ConvBatch(ops, ks)
     gens = [torch.from_numpy(np.asarray(e[4], dtype=np.complex64)).cuda().requires_grad_(True)
             for e in spec]
-    xs = [torch.from_numpy(crandn(rng, (op.numCols, k))).cuda().requires_grad_(True)
+    # column-major data (fastmat's F-ordered layout): leaves of shape (k, n),
+    # the operators see their (n, k) transposed views
+    xl = [torch.from_numpy(crandn(rng, (k, op.numCols))).cuda().requires_grad_(True)
           for op, k in zip(ops, ks)]
-    ys = [torch.from_numpy(crandn(rng, (op.numRows, k))).cuda() for op, k in zip(ops, ks)]
+    ys = [torch.from_numpy(crandn(rng, (k, op.numRows))).cuda().t() for op, k in zip(ops, ks)]
     # forward + adjoint inputs/outputs, and per operator the generator
     # gradient's reads of x and the residual and its write of the generator
     bytes_step = (batch.algorithmic_bytes(False) + batch.algorithmic_bytes(True) +
@@ -567,9 +569,9 @@ def workload_batch_grad(args, rng):
                       for op, k in zip(ops, ks)))
 
     def step():
-        for t in gens + xs:
+        for t in gens + xl:
             t.grad = None
-        outs = fa.batch_conv_matvec(batch, gens, xs)
+        outs = fa.batch_conv_matvec(batch, gens, [x.t() for x in xl])
         loss = sum(((o - y).abs() ** 2).sum() for o, y in zip(outs, ys))
         loss.backward()
         return loss
@@ -582,7 +584,7 @@ def workload_batch_grad(args, rng):
         from oracle import fast64
         for i in range(0, len(ops), 51):
             kind, m, n, k, gen = spec[i]
-            x = xs[i].detach().cpu().numpy().astype(np.complex128)
+            x = xl[i].detach().cpu().numpy().T.astype(np.complex128)
             y = ys[i].cpu().numpy().astype(np.complex128)
             if kind == "circulant":
                 r = fast64.circulant(gen, x) - y
@@ -590,7 +592,7 @@ def workload_batch_grad(args, rng):
             else:
                 r = fast64.toeplitz(gen, m, x) - y
                 ref = 2 * fast64.toeplitz(gen, m, r, adjoint=True)
-            errs.append(accept.max_rel_error(xs[i].grad.cpu().numpy(), ref))
+            errs.append(accept.max_rel_error(xl[i].grad.cpu().numpy().T, ref))
         return {"x_grad_sampled_max": max(errs)}
 
     cfg = {"workload": "uneven_batch_256_autograd_loss_grad_x_and_generators_c64",
