@@ -557,11 +557,11 @@ def workload_batch_grad(args, rng):
     batch = ConvBatch(ops, ks)
     gens = [torch.from_numpy(np.asarray(e[4], dtype=np.complex64)).cuda().requires_grad_(True)
             for e in spec]
-    # column-major data (fastmat's F-ordered layout): leaves of shape (k, n),
-    # the operators see their (n, k) transposed views
-    xl = [torch.from_numpy(crandn(rng, (k, op.numCols))).cuda().requires_grad_(True)
-          for op, k in zip(ops, ks)]
-    ys = [torch.from_numpy(crandn(rng, (k, op.numRows))).cuda().t() for op, k in zip(ops, ks)]
+    # packed column-major data: every operator's (n_i, k_i) input (and
+    # target) stored one after another in one flat tensor
+    n_in, n_out = batch.packed_sizes()
+    xp = torch.from_numpy(crandn(rng, (n_in,))).cuda().requires_grad_(True)
+    yp = torch.from_numpy(crandn(rng, (n_out,))).cuda()
     # forward + adjoint inputs/outputs, and per operator the generator
     # gradient's reads of x and the residual and its write of the generator
     bytes_step = (batch.algorithmic_bytes(False) + batch.algorithmic_bytes(True) +
@@ -569,10 +569,10 @@ def workload_batch_grad(args, rng):
                       for op, k in zip(ops, ks)))
 
     def step():
-        for t in gens + xl:
+        for t in gens + [xp]:
             t.grad = None
-        outs = fa.batch_conv_matvec(batch, gens, [x.t() for x in xl])
-        loss = sum(((o - y).abs() ** 2).sum() for o, y in zip(outs, ys))
+        out = fa.batch_conv_matvec_packed(batch, gens, xp)
+        loss = ((out - yp).abs() ** 2).sum()
         loss.backward()
         return loss
 
@@ -582,21 +582,24 @@ def workload_batch_grad(args, rng):
         torch.cuda.synchronize()
         errs = []
         from oracle import fast64
+        xv = batch.packed_views(xp.detach(), inputs=True)
+        yv = batch.packed_views(yp, inputs=False)
+        gv = batch.packed_views(xp.grad, inputs=True)
         for i in range(0, len(ops), 51):
             kind, m, n, k, gen = spec[i]
-            x = xl[i].detach().cpu().numpy().T.astype(np.complex128)
-            y = ys[i].cpu().numpy().astype(np.complex128)
+            x = xv[i].cpu().numpy().T.astype(np.complex128)
+            y = yv[i].cpu().numpy().T.astype(np.complex128)
             if kind == "circulant":
                 r = fast64.circulant(gen, x) - y
                 ref = 2 * fast64.circulant(gen, r, adjoint=True)
             else:
                 r = fast64.toeplitz(gen, m, x) - y
                 ref = 2 * fast64.toeplitz(gen, m, r, adjoint=True)
-            errs.append(accept.max_rel_error(xl[i].grad.cpu().numpy().T, ref))
+            errs.append(accept.max_rel_error(gv[i].cpu().numpy().T, ref))
         return {"x_grad_sampled_max": max(errs)}
 
     cfg = {"workload": "uneven_batch_256_autograd_loss_grad_x_and_generators_c64",
-           "operators": len(ops), "total_columns": int(sum(ks))}
+           "operators": len(ops), "total_columns": int(sum(ks)), "data": "packed"}
     return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=None, cfg=cfg,
                 dtype="c64", metric=METRIC)
 

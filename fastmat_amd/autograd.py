@@ -11,7 +11,14 @@ back onto c (Circulant) or [vecC, vecR] (Toeplitz); see
 fm_conv_generator_grad in include/fastmat_amd.h.
 
 ``batch_conv_matvec(batch, gens, xs)``: the uneven batch (ConvBatch) with
-per-operator generator gradients.
+the generator gradients of all operators in one batched program.
+
+``batch_conv_matvec_packed(batch, gens, xp)``: the same on packed data: the
+input blocks of all operators (operator i's (k_i, n_i) block, i.e. its
+column-major (n_i, k_i) data) one after another in one flat tensor ``xp``,
+the result packed the same way (``batch.packed_sizes()``), so that a loss
+over the whole batch is a handful of elementwise operations on two flat
+tensors instead of several per operator.
 
 Tensors are user-facing (n,) or (n, k) device tensors; complex gradients
 follow PyTorch's convention (grad = dL/dRe + i dL/dIm).
@@ -132,3 +139,59 @@ class _BatchConvMatvec(torch.autograd.Function):
 def batch_conv_matvec(batch, gens, xs):
     """Differentiable uneven batch: returns [A_i(gen_i) x_i]."""
     return list(_BatchConvMatvec.apply(batch, *gens, *xs))
+
+
+def _packed_buffers(batch, dtype, device):
+    """Persistent packed buffers of a batch (forward output, output gradient,
+    input gradient): fixed addresses, so the batch's bindings are reused from
+    call to call; callers receive copies."""
+    bufs = getattr(batch, "_packed_bufs", None)
+    if bufs is None or bufs[0].dtype != dtype:
+        n_in, n_out = batch.packed_sizes()
+        bufs = (torch.empty(n_out, dtype=dtype, device=device),
+                torch.empty(n_out, dtype=dtype, device=device),
+                torch.empty(n_in, dtype=dtype, device=device))
+        batch._packed_bufs = bufs
+    return bufs
+
+
+class _BatchConvPacked(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, batch, xp, *gens):
+        y_buf, _, _ = _packed_buffers(batch, xp.dtype, xp.device)
+        batch.apply(batch.packed_views(xp, adjoint=False, inputs=True),
+                    batch.packed_views(y_buf, adjoint=False, inputs=False), adjoint=False)
+        ctx.batch = batch
+        ctx.gen_complex = [g.is_complex() for g in gens]
+        ctx.gen_dtype = [g.dtype for g in gens]
+        ctx.save_for_backward(xp)
+        return y_buf.clone()
+
+    @staticmethod
+    def backward(ctx, gyp):
+        batch = ctx.batch
+        n = len(batch.ops)
+        (xp,) = ctx.saved_tensors
+        _, g_buf, gx_buf = _packed_buffers(batch, xp.dtype, xp.device)
+        g_buf.copy_(gyp)
+        G = batch.packed_views(g_buf, adjoint=False, inputs=False)
+        gxp = None
+        if ctx.needs_input_grad[1]:
+            batch.apply(G, batch.packed_views(gx_buf, adjoint=False, inputs=True), adjoint=True)
+            gxp = gx_buf.clone()
+        ggen = [None] * n
+        if any(ctx.needs_input_grad[2:]):
+            X = batch.packed_views(xp, adjoint=False, inputs=True)
+            raw = batch.generator_grads(X, G)
+            ggen = [r.to(dt) if cplx else r.real.to(dt)
+                    for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
+        return (None, gxp, *ggen)
+
+
+def batch_conv_matvec_packed(batch, gens, xp):
+    """Differentiable uneven batch on packed data: y_p = [A_i(gen_i) x_i]
+    packed; xp and the result are flat device tensors of the batch dtype."""
+    if xp.dim() != 1 or xp.numel() != batch.packed_sizes()[0]:
+        raise ValueError("packed input must be a flat tensor of %d elements"
+                         % batch.packed_sizes()[0])
+    return _BatchConvPacked.apply(batch, xp, *gens)

@@ -132,6 +132,27 @@ class ConvBatch:
         return tot
 
     # ----------------------------------------------------------- buffers
+    def packed_sizes(self):
+        """(elements of all forward inputs, elements of all forward outputs)
+        when the (k_i, n_i) blocks are stored one after another."""
+        return (int(sum(op.numCols * k for op, k in zip(self.ops, self.ks))),
+                int(sum(op.numRows * k for op, k in zip(self.ops, self.ks))))
+
+    def packed_views(self, flat, adjoint=False, inputs=True):
+        """(k_i, len_i) block views into a flat tensor holding every
+        operator's block in turn: the forward inputs (len = cols) or outputs
+        (len = rows); with adjoint=True the roles swap."""
+        use_cols = inputs != adjoint
+        out, off = [], 0
+        for op, k in zip(self.ops, self.ks):
+            w = op.numCols if use_cols else op.numRows
+            out.append(flat[off:off + k * w].view(k, w))
+            off += k * w
+        if off != flat.numel():
+            raise ValueError("ConvBatch: packed tensor has %d elements, the blocks %d"
+                             % (flat.numel(), off))
+        return out
+
     def new_inputs(self, adjoint=False, fill=None):
         """Allocate input blocks (k_i, n_i) for a direction."""
         out = []

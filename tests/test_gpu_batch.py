@@ -270,3 +270,36 @@ def test_batched_generator_gradients_match_per_operator(torch_mod):
         for op, x, g, r in zip(ops, xs, gs, got):
             ref = op.generator_grad(x, g)
             assert accept.max_rel_error(r.cpu().numpy(), ref.cpu().numpy()) < 1e-5, (rep, op)
+
+
+def test_autograd_packed_batch_matches_per_tensor(torch_mod):
+    """batch_conv_matvec_packed (all blocks in two flat tensors) gives the
+    outputs and gradients of batch_conv_matvec, twice in a row."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(13)
+    spec = small_spec(13, count=8, hi=15000, kmax=3)
+    ops, ks = build(spec)
+    batch = ConvBatch(ops, ks)
+    n_in, n_out = batch.packed_sizes()
+    say("packed autograd batch", len(ops))
+    for rep in range(2):
+        gens = [torch.from_numpy(e[4]).cuda().requires_grad_(True) for e in spec]
+        xp = torch.from_numpy(crandn(rng, n_in)).cuda().requires_grad_(True)
+        yp = torch.from_numpy(crandn(rng, n_out)).cuda()
+        out = fa.batch_conv_matvec_packed(batch, gens, xp)
+        ((out - yp).abs() ** 2).sum().backward()
+        # the per-tensor Function on the same data
+        gens2 = [g.detach().clone().requires_grad_(True) for g in gens]
+        xb = [v.detach().clone().t().requires_grad_(True)
+              for v in batch.packed_views(xp.detach(), inputs=True)]
+        yb = [v.t() for v in batch.packed_views(yp, inputs=False)]
+        outs2 = fa.batch_conv_matvec(batch, gens2, xb)
+        sum(((o - y).abs() ** 2).sum() for o, y in zip(outs2, yb)).backward()
+        ref_out = torch.cat([o.t().reshape(-1) for o in outs2])
+        assert accept.max_rel_error(out.detach().cpu().numpy(), ref_out.detach().cpu().numpy()) < 1e-6
+        ref_gx = torch.cat([x.grad.t().reshape(-1) for x in xb])
+        assert accept.max_rel_error(xp.grad.cpu().numpy(), ref_gx.cpu().numpy()) < 1e-6
+        for g, g2 in zip(gens, gens2):
+            assert accept.max_rel_error(g.grad.cpu().numpy(), g2.grad.cpu().numpy()) < 1e-6
