@@ -568,7 +568,7 @@ def workload_batch_grad(args, rng):
                   sum((op.numRows + op.numCols) * k * 8 + (op.numRows + op.numCols - 1) * 8
                       for op, k in zip(ops, ks)))
 
-    def step():
+    def eager_step():
         for t in gens + [xp]:
             t.grad = None
         out = fa.batch_conv_matvec_packed(batch, gens, xp)
@@ -576,10 +576,42 @@ def workload_batch_grad(args, rng):
         loss.backward()
         return loss
 
+    # whole-step capture (forward, loss, backward) into one HIP graph after
+    # eager warm-up on a side stream (plans, bindings and buffers settle)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            eager_step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    for t in gens + [xp]:
+        t.grad = None
+    with torch.cuda.graph(graph):
+        out_s = fa.batch_conv_matvec_packed(batch, gens, xp)
+        loss_s = ((out_s - yp).abs() ** 2).sum()
+        loss_s.backward()
+
+    def step():
+        graph.replay()
+        return loss_s
+
     def check():
         from oracle import accept
         step()
         torch.cuda.synchronize()
+        g_graph = [g.grad.clone() for g in gens[:8]]
+        x_graph = xp.grad.clone()
+        l_graph = float(loss_s.detach())
+        l_eager = float(eager_step().detach())
+        torch.cuda.synchronize()
+        # the replayed graph reproduces the eager step (same kernels, same
+        # order; the loss's reduction may order its sums differently)
+        same = (accept.max_rel_error(x_graph.cpu().numpy(), xp.grad.cpu().numpy()) < 1e-6 and
+                all(accept.max_rel_error(a.cpu().numpy(), g.grad.cpu().numpy()) < 1e-6
+                    for a, g in zip(g_graph, gens[:8])) and
+                abs(l_graph - l_eager) <= 1e-5 * abs(l_eager))
         errs = []
         from oracle import fast64
         xv = batch.packed_views(xp.detach(), inputs=True)
@@ -596,12 +628,13 @@ def workload_batch_grad(args, rng):
                 r = fast64.toeplitz(gen, m, x) - y
                 ref = 2 * fast64.toeplitz(gen, m, r, adjoint=True)
             errs.append(accept.max_rel_error(gv[i].cpu().numpy().T, ref))
-        return {"x_grad_sampled_max": max(errs)}
+        return {"x_grad_sampled_max": max(errs), "graph_matches_eager": bool(same)}
 
     cfg = {"workload": "uneven_batch_256_autograd_loss_grad_x_and_generators_c64",
            "operators": len(ops), "total_columns": int(sum(ks)), "data": "packed"}
-    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=None, cfg=cfg,
-                dtype="c64", metric=METRIC)
+    cfg["hipgraph"] = True
+    return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check, plain=None,
+                cpu=None, cfg=cfg, dtype="c64", metric=METRIC)
 
 
 WORKLOADS = {"copy": workload_copy, "fourier": workload_fourier, "hadamard": workload_hadamard,
