@@ -303,3 +303,52 @@ def test_autograd_packed_batch_matches_per_tensor(torch_mod):
         assert accept.max_rel_error(xp.grad.cpu().numpy(), ref_gx.cpu().numpy()) < 1e-6
         for g, g2 in zip(gens, gens2):
             assert accept.max_rel_error(g.grad.cpu().numpy(), g2.grad.cpu().numpy()) < 1e-6
+
+
+def test_packed_autograd_step_replays_in_a_graph(torch_mod):
+    """The packed batch autograd step (forward, loss, backward) captured in
+    one HIP graph: every replay reproduces the eager step's gradients, with
+    eager work and synchronisations between the replays."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(14)
+    spec = small_spec(14, count=6, hi=12000, kmax=3)
+    ops, ks = build(spec)
+    batch = ConvBatch(ops, ks)
+    n_in, n_out = batch.packed_sizes()
+    gens = [torch.from_numpy(e[4]).cuda().requires_grad_(True) for e in spec]
+    xp = torch.from_numpy(crandn(rng, n_in)).cuda().requires_grad_(True)
+    yp = torch.from_numpy(crandn(rng, n_out)).cuda()
+
+    def eager():
+        for t in gens + [xp]:
+            t.grad = None
+        out = fa.batch_conv_matvec_packed(batch, gens, xp)
+        ((out - yp).abs() ** 2).sum().backward()
+
+    say("packed autograd graph capture")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            eager()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    ref_x = xp.grad.clone()
+    ref_g = [g.grad.clone() for g in gens]
+    graph = torch.cuda.CUDAGraph()
+    for t in gens + [xp]:
+        t.grad = None
+    with torch.cuda.graph(graph):
+        out = fa.batch_conv_matvec_packed(batch, gens, xp)
+        ((out - yp).abs() ** 2).sum().backward()
+    gx, gg = xp.grad, [g.grad for g in gens]
+    for rep in range(3):
+        gx.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert accept.max_rel_error(gx.cpu().numpy(), ref_x.cpu().numpy()) < 1e-6, rep
+        for a, b in zip(gg, ref_g):
+            assert accept.max_rel_error(a.cpu().numpy(), b.cpu().numpy()) < 1e-6, rep
+        _ = (yp * 2).sum().item()  # eager work between replays
