@@ -23,10 +23,7 @@ tensors instead of several per operator.
 Tensors are user-facing (n,) or (n, k) device tensors; complex gradients
 follow PyTorch's convention (grad = dL/dRe + i dL/dIm).
 """
-import numpy as np
 import torch
-
-from . import _device as D
 
 
 def _to_block(x):

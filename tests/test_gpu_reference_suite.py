@@ -69,7 +69,6 @@ def layouts(x):
 def check_transforms(fm_mat, ref, tol_power, min_eps=0.0, name=""):
     """Forward and backward of fm_mat against the dense `ref` for every input
     dtype, column count and layout; returns the number of queries run."""
-    from fastmat_amd import _device as D
     n_rows, n_cols = ref.shape
     ref64 = ref.astype(np.complex128)
     queries = 0
