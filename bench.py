@@ -47,7 +47,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--config", default="fourier",
                    choices=["fourier", "hadamard", "circulant", "kron", "batch", "copy",
-                            "multilevel", "fista", "blockdiag", "lfsr", "batch_grad"])
+                            "multilevel", "fista", "blockdiag", "lfsr", "batch_grad",
+                            "omp"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cols", type=int, default=None, help="override the column count")
     return p.parse_args()
@@ -637,11 +638,46 @@ def workload_batch_grad(args, rng):
                 cpu=None, cfg=cfg, dtype="c64", metric=METRIC)
 
 
+def workload_omp(args, rng):
+    """OMP (32 steps per run) on a Circulant of N = 2^16 with a windowed
+    cosine kernel (the docstring's problem, scaled up), 8 columns of 16-sparse
+    truths, complex128 arithmetic.  One bench step is one solver run; value
+    counts the operator applies (one adjoint and one forward per OMP step)."""
+    import torch
+    import fastmat_amd as fm
+    from fastmat_amd import algorithms as fma
+    k = args.cols or 8
+    N = 1 << 16
+    steps = 32
+    t = np.linspace(0, 40 * np.pi, N)
+    c = np.cos(2 * t) * np.exp(-((t - 20 * np.pi) / 3) ** 2)
+    C = fm.Circulant(c)
+    x0 = np.zeros((N, k))
+    for j in range(k):
+        x0[rng.choice(N, 16, replace=False), j] = rng.uniform(1, 2, 16)
+    B = C._apply(device_block(x0, torch.float64), False).contiguous()  # (k, N) complex128
+    Bv = B.t()
+    _ = C.colNorms
+    bytes_step = steps * 2 * (2 * N * k * 16 + N * 16)
+
+    def step():
+        return fma.OMP(C, numMaxSteps=steps).process(Bv)
+
+    def check():
+        y = step()
+        r = C._apply(y.t().contiguous(), False) - B
+        return {"rel_residual": float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(B))}
+
+    cfg = {"workload": "omp_circulant_2p16_c128_32steps", "N": N, "cols": k, "steps": steps}
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=None, cfg=cfg,
+                dtype="c128", metric=METRIC)
+
+
 WORKLOADS = {"copy": workload_copy, "fourier": workload_fourier, "hadamard": workload_hadamard,
              "circulant": workload_circulant, "kron": workload_kron, "batch": workload_batch,
              "multilevel": workload_multilevel, "fista": workload_fista,
              "blockdiag": workload_blockdiag, "lfsr": workload_lfsr,
-             "batch_grad": workload_batch_grad}
+             "batch_grad": workload_batch_grad, "omp": workload_omp}
 
 
 def time_cpu(legs):

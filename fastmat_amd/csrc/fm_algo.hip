@@ -435,19 +435,20 @@ __device__ __forceinline__ V omp_block_sum(V v, V *sh) {
 }
 
 // one block per problem: first index of the largest |g_j| inv_norm_j
+constexpr int kSelNT = 1024;
 template <typename V>
-__global__ __launch_bounds__(kOmpNT) void omp_select_kernel(int64_t m, const V *__restrict__ g,
+__global__ __launch_bounds__(kSelNT) void omp_select_kernel(int64_t m, const V *__restrict__ g,
                                                             const double *__restrict__ inv_norm,
                                                             int64_t *__restrict__ idx,
                                                             int64_t *__restrict__ support,
                                                             int steps, int ii) {
     using O = OmpOps<V>;
-    __shared__ double sv[kOmpNT];
-    __shared__ int64_t si[kOmpNT];
+    __shared__ double sv[kSelNT];
+    __shared__ int64_t si[kSelNT];
     const int64_t l = blockIdx.x;
     double best = -1.0;
     int64_t bi = m;
-    for (int64_t j = threadIdx.x; j < m; j += kOmpNT) {
+    for (int64_t j = threadIdx.x; j < m; j += kSelNT) {
         const double v = sqrt(O::abs2(g[l * m + j])) * inv_norm[j];
         if (v > best || (v != v && best == best)) {  // first NaN wins, as argmax
             best = v;
@@ -457,7 +458,7 @@ __global__ __launch_bounds__(kOmpNT) void omp_select_kernel(int64_t m, const V *
     sv[threadIdx.x] = best;
     si[threadIdx.x] = bi;
     __syncthreads();
-    for (int w = kOmpNT / 2; w > 0; w >>= 1) {
+    for (int w = kSelNT / 2; w > 0; w >>= 1) {
         if (int(threadIdx.x) < w) {
             const double a = sv[threadIdx.x], b = sv[threadIdx.x + w];
             const int64_t ia = si[threadIdx.x], ib = si[threadIdx.x + w];
@@ -615,7 +616,7 @@ static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int step
                             int64_t *support, void *e, const void *c, const void *b, void *P,
                             void *acols, void *r, void *work, void *x, void *x_full,
                             hipStream_t s) {
-    const int64_t parts = (n + 65535) / 65536 < 1 ? 1 : ((n + 65535) / 65536 > 256 ? 256 : (n + 65535) / 65536);
+    const int64_t parts = (n + 4095) / 4096 < 1 ? 1 : ((n + 4095) / 4096 > 256 ? 256 : (n + 4095) / 4096);
     // work: v1 (L steps V) | v2 (L n V) | part_d (L parts V) | y (L V) |
     //       part_n (L parts f64) | nrm2 (L f64)
     V *v1 = (V *)work;
@@ -631,7 +632,7 @@ static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int step
         return unsigned(bl);
     };
     if (phase == 0) {  // select the new index, build the one-hot block
-        hipLaunchKernelGGL((omp_select_kernel<V>), dim3(unsigned(L)), dim3(kOmpNT), 0, s, m,
+        hipLaunchKernelGGL((omp_select_kernel<V>), dim3(unsigned(L)), dim3(kSelNT), 0, s, m,
                            (const V *)g, inv_norm, idx, support, steps, ii);
         FM_CHECK_LAUNCH();
         hipLaunchKernelGGL((omp_onehot_kernel<V>), dim3(grid_of(L * m)), dim3(256), 0, s, L, m,
@@ -679,7 +680,7 @@ extern "C" fm_status fm_omp_workspace_size(int dtype, int64_t L, int64_t n, int 
                "OMP runs in float64 / complex128 (type %d)", dtype);
     FM_REQUIRE(L >= 1 && n >= 1 && steps >= 1, FM_ERR_VALUE, "OMP: bad sizes");
     const int64_t vs = dtype == FM_FLOAT64 ? 8 : 16;
-    const int64_t parts = (n + 65535) / 65536 < 1 ? 1 : ((n + 65535) / 65536 > 256 ? 256 : (n + 65535) / 65536);
+    const int64_t parts = (n + 4095) / 4096 < 1 ? 1 : ((n + 4095) / 4096 > 256 ? 256 : (n + 4095) / 4096);
     *bytes = vs * (L * steps + L * n + L * parts + L) + 8 * (L * parts + L);
     return FM_SUCCESS;
 }
