@@ -492,20 +492,38 @@ __global__ void omp_onehot_kernel(int64_t L, int64_t m, const int64_t *__restric
     }
 }
 
-// v1[l, i] = sum_j P[l, i, j] c[l, j]; grid (ii, L)
+// partial sums v1p[l, i, p] = sum_{j in chunk p} P[l, i, j] c[l, j];
+// grid (parts, ii, L)
 template <typename V>
-__global__ __launch_bounds__(kOmpNT) void omp_proj_kernel(int64_t n, int steps,
+__global__ __launch_bounds__(kOmpNT) void omp_proj_kernel(int64_t n, int steps, int64_t parts,
                                                           const V *__restrict__ P,
                                                           const V *__restrict__ c,
-                                                          V *__restrict__ v1) {
+                                                          V *__restrict__ v1p) {
     using O = OmpOps<V>;
     __shared__ V sh[kOmpNT];
-    const int64_t i = blockIdx.x, l = blockIdx.y;
+    const int64_t p = blockIdx.x, i = blockIdx.y, l = blockIdx.z;
+    const int64_t chunk = ceil_div(n, parts);
+    const int64_t lo = p * chunk, hi = (lo + chunk < n) ? lo + chunk : n;
     const V *row = P + (l * steps + i) * n;
     V acc = O::zero();
-    for (int64_t j = threadIdx.x; j < n; j += kOmpNT) acc = O::add(acc, O::mul(row[j], c[l * n + j]));
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kOmpNT)
+        acc = O::add(acc, O::mul(row[j], c[l * n + j]));
     acc = omp_block_sum<V>(acc, sh);
-    if (threadIdx.x == 0) v1[l * steps + i] = acc;
+    if (threadIdx.x == 0) v1p[(l * steps + i) * parts + p] = acc;
+}
+
+// v1[l, i] = sum_p v1p[l, i, p], chunks in order; one thread per (l, i < ii)
+template <typename V>
+__global__ void omp_proj_sum_kernel(int64_t L, int steps, int ii, int64_t parts,
+                                    const V *__restrict__ v1p, V *__restrict__ v1) {
+    using O = OmpOps<V>;
+    const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (q >= L * ii) return;
+    const int64_t l = q / ii, i = q - l * ii;
+    const V *src = v1p + (l * steps + i) * parts;
+    V acc = O::zero();
+    for (int64_t p = 0; p < parts; ++p) acc = O::add(acc, src[p]);
+    v1[l * steps + i] = acc;
 }
 
 // v2 = c - Acols[:ii]^T v1 and partial sums of |v2|^2 and conj(v2) b;
@@ -610,20 +628,28 @@ __global__ void omp_result_kernel(int64_t L, int64_t m, int steps,
     for (int i = 0; i < steps; ++i) out[l * m + support[l * steps + i]] = x[l * steps + i];
 }
 
+// column chunks of the OMP reductions, about 512 elements each, so that a
+// step's reductions spread over many workgroups even for a few problems
+static inline int64_t omp_parts(int64_t n) {
+    const int64_t p = (n + 511) / 512;
+    return p < 1 ? 1 : (p > 1024 ? 1024 : p);
+}
+
 template <typename V>
 static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int steps, int ii,
                             const void *g, const double *inv_norm, int64_t *idx,
                             int64_t *support, void *e, const void *c, const void *b, void *P,
                             void *acols, void *r, void *work, void *x, void *x_full,
                             hipStream_t s) {
-    const int64_t parts = (n + 4095) / 4096 < 1 ? 1 : ((n + 4095) / 4096 > 256 ? 256 : (n + 4095) / 4096);
+    const int64_t parts = omp_parts(n);
     // work: v1 (L steps V) | v2 (L n V) | part_d (L parts V) | y (L V) |
-    //       part_n (L parts f64) | nrm2 (L f64)
+    //       v1p (L steps parts V) | part_n (L parts f64) | nrm2 (L f64)
     V *v1 = (V *)work;
     V *v2 = v1 + L * steps;
     V *part_d = v2 + L * n;
     V *yv = part_d + L * parts;
-    double *part_n = (double *)(yv + L);
+    V *v1p = yv + L;
+    double *part_n = (double *)(v1p + L * steps * parts);
     double *nrm2 = part_n + L * parts;
     auto grid_of = [](int64_t total) {
         int64_t bl = ceil_div(total, 256);
@@ -642,8 +668,12 @@ static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int step
     }
     if (phase == 1) {  // pseudo-inverse update with the new column c
         if (ii > 0) {
-            hipLaunchKernelGGL((omp_proj_kernel<V>), dim3(unsigned(ii), unsigned(L)),
-                               dim3(kOmpNT), 0, s, n, steps, (const V *)P, (const V *)c, v1);
+            hipLaunchKernelGGL((omp_proj_kernel<V>),
+                               dim3(unsigned(parts), unsigned(ii), unsigned(L)), dim3(kOmpNT), 0,
+                               s, n, steps, parts, (const V *)P, (const V *)c, v1p);
+            FM_CHECK_LAUNCH();
+            hipLaunchKernelGGL((omp_proj_sum_kernel<V>), dim3(unsigned(ceil_div(L * ii, 256))),
+                               dim3(256), 0, s, L, steps, ii, parts, (const V *)v1p, v1);
             FM_CHECK_LAUNCH();
         }
         hipLaunchKernelGGL((omp_resid_kernel<V>), dim3(unsigned(parts), unsigned(L)),
@@ -680,8 +710,8 @@ extern "C" fm_status fm_omp_workspace_size(int dtype, int64_t L, int64_t n, int 
                "OMP runs in float64 / complex128 (type %d)", dtype);
     FM_REQUIRE(L >= 1 && n >= 1 && steps >= 1, FM_ERR_VALUE, "OMP: bad sizes");
     const int64_t vs = dtype == FM_FLOAT64 ? 8 : 16;
-    const int64_t parts = (n + 4095) / 4096 < 1 ? 1 : ((n + 4095) / 4096 > 256 ? 256 : (n + 4095) / 4096);
-    *bytes = vs * (L * steps + L * n + L * parts + L) + 8 * (L * parts + L);
+    const int64_t parts = fm::omp_parts(n);
+    *bytes = vs * (L * steps + L * n + L * parts + L + L * steps * parts) + 8 * (L * parts + L);
     return FM_SUCCESS;
 }
 
@@ -693,7 +723,8 @@ extern "C" fm_status fm_omp_step(int dtype, int phase, int64_t L, int64_t m, int
     FM_REQUIRE(dtype == FM_FLOAT64 || dtype == FM_COMPLEX128, FM_ERR_TYPE,
                "OMP runs in float64 / complex128 (type %d)", dtype);
     FM_REQUIRE(phase >= 0 && phase <= 2, FM_ERR_VALUE, "OMP: phase %d", phase);
-    FM_REQUIRE(L >= 1 && L <= 65535 && m >= 1 && n >= 1 && steps >= 1 && ii >= 0 && ii < steps,
+    FM_REQUIRE(L >= 1 && L <= 65535 && m >= 1 && n >= 1 && steps >= 1 && steps <= 65535 &&
+                   ii >= 0 && ii < steps,
                FM_ERR_VALUE, "OMP: bad sizes or step");
     int64_t need = 0;
     fm_omp_workspace_size(dtype, L, n, steps, &need);
