@@ -492,6 +492,82 @@ __global__ void omp_onehot_kernel(int64_t L, int64_t m, const int64_t *__restric
     }
 }
 
+// (a, ia) against (b, ib): true when b wins -- a NaN first (the lower index
+// among NaNs), then the larger value, then the lower index
+__device__ __forceinline__ bool omp_sel_takes(double a, int64_t ia, double b, int64_t ib) {
+    const bool a_nan = a != a, b_nan = b != b;
+    if (a_nan || b_nan) return b_nan && (!a_nan || ib < ia);
+    return b > a || (b == a && ib < ia);
+}
+
+// tree over the block's (value, index) pairs; the winner ends in sv[0], si[0]
+template <int NT>
+__device__ __forceinline__ void omp_sel_reduce(double best, int64_t bi, double *sv, int64_t *si) {
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w &&
+            omp_sel_takes(sv[threadIdx.x], si[threadIdx.x], sv[threadIdx.x + w], si[threadIdx.x + w])) {
+            sv[threadIdx.x] = sv[threadIdx.x + w];
+            si[threadIdx.x] = si[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+}
+
+// selection, first stage: the winner of each chunk of columns; grid (sp, L)
+template <typename V>
+__global__ __launch_bounds__(kOmpNT) void omp_select_part_kernel(
+    int64_t m, int64_t sp, const V *__restrict__ g, const double *__restrict__ inv_norm,
+    double *__restrict__ part_v, int64_t *__restrict__ part_i) {
+    using O = OmpOps<V>;
+    __shared__ double sv[kOmpNT];
+    __shared__ int64_t si[kOmpNT];
+    const int64_t p = blockIdx.x, l = blockIdx.y;
+    const int64_t chunk = ceil_div(m, sp);
+    const int64_t lo = p * chunk, hi = (lo + chunk < m) ? lo + chunk : m;
+    double best = -1.0;
+    int64_t bi = m;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kOmpNT) {
+        const double v = sqrt(O::abs2(g[l * m + j])) * inv_norm[j];
+        if (omp_sel_takes(best, bi, v, j)) {
+            best = v;
+            bi = j;
+        }
+    }
+    omp_sel_reduce<kOmpNT>(best, bi, sv, si);
+    if (threadIdx.x == 0) {
+        part_v[l * sp + p] = sv[0];
+        part_i[l * sp + p] = si[0];
+    }
+}
+
+// selection, second stage: merge the chunk winners of each problem; grid (L)
+__global__ __launch_bounds__(kOmpNT) void omp_select_merge_kernel(
+    int64_t m, int64_t sp, const double *__restrict__ part_v, const int64_t *__restrict__ part_i,
+    int64_t *__restrict__ idx, int64_t *__restrict__ support, int steps, int ii) {
+    __shared__ double sv[kOmpNT];
+    __shared__ int64_t si[kOmpNT];
+    const int64_t l = blockIdx.x;
+    double best = -1.0;
+    int64_t bi = m;
+    for (int64_t p = threadIdx.x; p < sp; p += kOmpNT) {
+        const double v = part_v[l * sp + p];
+        const int64_t j = part_i[l * sp + p];
+        if (omp_sel_takes(best, bi, v, j)) {
+            best = v;
+            bi = j;
+        }
+    }
+    omp_sel_reduce<kOmpNT>(best, bi, sv, si);
+    if (threadIdx.x == 0) {
+        const int64_t j = si[0] < m ? si[0] : 0;
+        idx[l] = j;
+        support[l * steps + ii] = j;
+    }
+}
+
 // partial sums v1p[l, i, p] = sum_{j in chunk p} P[l, i, j] c[l, j];
 // grid (parts, ii, L)
 template <typename V>
@@ -512,18 +588,19 @@ __global__ __launch_bounds__(kOmpNT) void omp_proj_kernel(int64_t n, int steps, 
     if (threadIdx.x == 0) v1p[(l * steps + i) * parts + p] = acc;
 }
 
-// v1[l, i] = sum_p v1p[l, i, p], chunks in order; one thread per (l, i < ii)
+// v1[l, i] = sum_p v1p[l, i, p]; grid (ii, L)
 template <typename V>
-__global__ void omp_proj_sum_kernel(int64_t L, int steps, int ii, int64_t parts,
-                                    const V *__restrict__ v1p, V *__restrict__ v1) {
+__global__ __launch_bounds__(kOmpNT) void omp_proj_sum_kernel(int steps, int64_t parts,
+                                                              const V *__restrict__ v1p,
+                                                              V *__restrict__ v1) {
     using O = OmpOps<V>;
-    const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (q >= L * ii) return;
-    const int64_t l = q / ii, i = q - l * ii;
+    __shared__ V sh[kOmpNT];
+    const int64_t i = blockIdx.x, l = blockIdx.y;
     const V *src = v1p + (l * steps + i) * parts;
     V acc = O::zero();
-    for (int64_t p = 0; p < parts; ++p) acc = O::add(acc, src[p]);
-    v1[l * steps + i] = acc;
+    for (int64_t p = threadIdx.x; p < parts; p += kOmpNT) acc = O::add(acc, src[p]);
+    acc = omp_block_sum<V>(acc, sh);
+    if (threadIdx.x == 0) v1[l * steps + i] = acc;
 }
 
 // v2 = c - Acols[:ii]^T v1 and partial sums of |v2|^2 and conj(v2) b;
@@ -562,27 +639,41 @@ __global__ __launch_bounds__(kOmpNT) void omp_resid_kernel(
     }
 }
 
-// per problem: nrm2, y = <v2, b> / nrm2, coefficient updates
+// per problem: nrm2, y = <v2, b> / nrm2, coefficient updates; grid (L)
 template <typename V>
-__global__ void omp_coef_kernel(int64_t L, int steps, int ii, const double *__restrict__ part_n,
-                                const V *__restrict__ part_d, int64_t parts,
-                                const V *__restrict__ v1, V *__restrict__ x,
-                                double *__restrict__ nrm2, V *__restrict__ y) {
+__global__ __launch_bounds__(kOmpNT) void omp_coef_kernel(int steps, int ii,
+                                                          const double *__restrict__ part_n,
+                                                          const V *__restrict__ part_d,
+                                                          int64_t parts, const V *__restrict__ v1,
+                                                          V *__restrict__ x,
+                                                          double *__restrict__ nrm2,
+                                                          V *__restrict__ y) {
     using O = OmpOps<V>;
-    const int64_t l = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (l >= L) return;
+    __shared__ V sh[kOmpNT];
+    __shared__ double shn[kOmpNT];
+    const int64_t l = blockIdx.x;
     double s = 0.0;
     V d = O::zero();
-    for (int64_t p = 0; p < parts; ++p) {
+    for (int64_t p = threadIdx.x; p < parts; p += kOmpNT) {
         s += part_n[l * parts + p];
         d = O::add(d, part_d[l * parts + p]);
     }
-    const V yy = O::scale(d, 1.0 / s);
-    nrm2[l] = s;
-    y[l] = yy;
-    for (int i = 0; i < ii; ++i)
+    shn[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kOmpNT / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) shn[threadIdx.x] += shn[threadIdx.x + w];
+        __syncthreads();
+    }
+    d = omp_block_sum<V>(d, sh);
+    const double tot = shn[0];
+    const V yy = O::scale(d, 1.0 / tot);
+    if (threadIdx.x == 0) {
+        nrm2[l] = tot;
+        y[l] = yy;
+        x[l * steps + ii] = O::add(x[l * steps + ii], yy);
+    }
+    for (int i = threadIdx.x; i < ii; i += kOmpNT)
         x[l * steps + i] = O::sub(x[l * steps + i], O::mul(yy, v1[l * steps + i]));
-    x[l * steps + ii] = O::add(x[l * steps + ii], yy);
 }
 
 // rank-1 update of P, new rows of P and Acols, residual; grid-stride over L n
@@ -658,9 +749,24 @@ static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int step
         return unsigned(bl);
     };
     if (phase == 0) {  // select the new index, build the one-hot block
-        hipLaunchKernelGGL((omp_select_kernel<V>), dim3(unsigned(L)), dim3(kSelNT), 0, s, m,
-                           (const V *)g, inv_norm, idx, support, steps, ii);
-        FM_CHECK_LAUNCH();
+        // chunk winners go to the v2 region, which phase 0 does not use
+        const int64_t sp_cap = n * int64_t(sizeof(V)) / 16;
+        const int64_t sp = omp_parts(m) < sp_cap ? omp_parts(m) : sp_cap;
+        if (sp > 1) {
+            double *sel_v = (double *)v2;
+            int64_t *sel_i = (int64_t *)(sel_v + L * sp);
+            hipLaunchKernelGGL((omp_select_part_kernel<V>), dim3(unsigned(sp), unsigned(L)),
+                               dim3(kOmpNT), 0, s, m, sp, (const V *)g, inv_norm, sel_v, sel_i);
+            FM_CHECK_LAUNCH();
+            hipLaunchKernelGGL(omp_select_merge_kernel, dim3(unsigned(L)), dim3(kOmpNT), 0, s, m,
+                               sp, (const double *)sel_v, (const int64_t *)sel_i, idx, support,
+                               steps, ii);
+            FM_CHECK_LAUNCH();
+        } else {
+            hipLaunchKernelGGL((omp_select_kernel<V>), dim3(unsigned(L)), dim3(kSelNT), 0, s, m,
+                               (const V *)g, inv_norm, idx, support, steps, ii);
+            FM_CHECK_LAUNCH();
+        }
         hipLaunchKernelGGL((omp_onehot_kernel<V>), dim3(grid_of(L * m)), dim3(256), 0, s, L, m,
                            (const int64_t *)idx, (V *)e);
         FM_CHECK_LAUNCH();
@@ -672,8 +778,8 @@ static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int step
                                dim3(unsigned(parts), unsigned(ii), unsigned(L)), dim3(kOmpNT), 0,
                                s, n, steps, parts, (const V *)P, (const V *)c, v1p);
             FM_CHECK_LAUNCH();
-            hipLaunchKernelGGL((omp_proj_sum_kernel<V>), dim3(unsigned(ceil_div(L * ii, 256))),
-                               dim3(256), 0, s, L, steps, ii, parts, (const V *)v1p, v1);
+            hipLaunchKernelGGL((omp_proj_sum_kernel<V>), dim3(unsigned(ii), unsigned(L)),
+                               dim3(kOmpNT), 0, s, steps, parts, (const V *)v1p, v1);
             FM_CHECK_LAUNCH();
         }
         hipLaunchKernelGGL((omp_resid_kernel<V>), dim3(unsigned(parts), unsigned(L)),
@@ -681,9 +787,9 @@ static fm_status omp_step_t(int phase, int64_t L, int64_t m, int64_t n, int step
                            (const V *)v1, (const V *)c, (const V *)b, v2, part_n, part_d,
                            parts);
         FM_CHECK_LAUNCH();
-        hipLaunchKernelGGL((omp_coef_kernel<V>), dim3(unsigned(ceil_div(L, 64))), dim3(64), 0, s,
-                           L, steps, ii, (const double *)part_n, (const V *)part_d, parts,
-                           (const V *)v1, (V *)x, nrm2, yv);
+        hipLaunchKernelGGL((omp_coef_kernel<V>), dim3(unsigned(L)), dim3(kOmpNT), 0, s, steps, ii,
+                           (const double *)part_n, (const V *)part_d, parts, (const V *)v1,
+                           (V *)x, nrm2, yv);
         FM_CHECK_LAUNCH();
         hipLaunchKernelGGL((omp_rank1_kernel<V>), dim3(grid_of(L * n)), dim3(256), 0, s, L, n,
                            steps, ii, (const V *)v1, (const V *)v2, (const V *)c,

@@ -406,3 +406,34 @@ def test_omp_structured_operators(fm):
     dev = fma.OMP(C, numMaxSteps=3).process(bd)
     assert torch.is_tensor(dev) and dev.is_cuda
     np.testing.assert_array_equal(dev.cpu().numpy(), out)
+
+
+def test_omp_chunked_selection(fm):
+    """More than 512 columns, so the selection runs per column chunk and then
+    merges: a complex Circulant 4096 against the oracle, and an exact tie
+    between columns of different chunks (Hadamard, integer correlations),
+    where the first index must win as in np.argmax."""
+    import fastmat_amd.algorithms as fma
+    rng = np.random.default_rng(99)
+    n = 4096
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    C = fm.Circulant(c)
+    Cd = C.reference()
+    x = np.zeros((n, 3), dtype=np.complex128)
+    for j in range(3):
+        x[rng.choice(n, 4, replace=False), j] = rng.uniform(1, 2, 4)
+    b = Cd @ x
+    say("OMP circulant 4096")
+    out = fma.OMP(C, numMaxSteps=4).process(b)
+    ref, _ = oalg.omp(Cd, b, 4)
+    assert accept.max_rel_error(out, ref) < 1e-9
+    H = fm.Hadamard(12)
+    Hd = H.reference().astype(np.float64)
+    bt = Hd[:, 3000] + Hd[:, 700] + 0.5 * Hd[:, 1500]
+    say("OMP hadamard tie")
+    alg = fma.OMP(H, numMaxSteps=3)
+    got = alg.process(bt)
+    ref, sup = oalg.omp(Hd, bt, 3)
+    np.testing.assert_array_equal(alg.arrSupport[:, 0], [700, 3000, 1500])
+    np.testing.assert_array_equal(np.asarray(sup).reshape(-1)[:3], [700, 3000, 1500])
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
