@@ -506,11 +506,11 @@ __device__ __forceinline__ void omp_sel_reduce(double best, int64_t bi, double *
     sv[threadIdx.x] = best;
     si[threadIdx.x] = bi;
     __syncthreads();
+    const int t = threadIdx.x;
     for (int w = NT / 2; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w &&
-            omp_sel_takes(sv[threadIdx.x], si[threadIdx.x], sv[threadIdx.x + w], si[threadIdx.x + w])) {
-            sv[threadIdx.x] = sv[threadIdx.x + w];
-            si[threadIdx.x] = si[threadIdx.x + w];
+        if (t < w && omp_sel_takes(sv[t], si[t], sv[t + w], si[t + w])) {
+            sv[t] = sv[t + w];
+            si[t] = si[t + w];
         }
         __syncthreads();
     }
