@@ -28,6 +28,19 @@ def _check_selection(sel, size, name):
     return sel.astype(np.int64)
 
 
+def _device_index(sel, size):
+    """Row indices for the scatter / gather kernels with numpy's indexing
+    rules: an index in [-size, -1] counts from the end, anything outside
+    [-size, size) raises IndexError when the selection is used (the
+    reference's check above lets e.g. [-1, 1, 3] of size 3 through, and
+    numpy then raises at apply time, Partial.pyx:272-279)."""
+    bad = (sel < -size) | (sel >= size)
+    if np.any(bad):
+        raise IndexError("index %d is out of bounds for axis 0 with size %d"
+                         % (int(sel[np.argmax(bad)]), size))
+    return np.where(sel < 0, sel + size, sel).astype(np.int64)
+
+
 class Partial(Matrix):
     def __init__(self, mat, **options):
         if not isinstance(mat, Matrix):
@@ -71,7 +84,9 @@ class Partial(Matrix):
     def _idx(self, name, sel):
         t = self._dev.get(name)
         if t is None:
-            t = self._dev[name] = D.as_tensor(sel, np.int64)
+            M = self._content[0]
+            size = M.numRows if name == "r" else M.numCols
+            t = self._dev[name] = D.as_tensor(_device_index(sel, size), np.int64)
         return t
 
     def _scatter(self, X, sel, name, n_full):

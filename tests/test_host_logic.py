@@ -217,3 +217,21 @@ def test_algorithm_parameter_handling():
     s = fma.STELA(fm.Matrix(np.eye(3)))
     assert s.numMaxError == 1e-6
     np.testing.assert_allclose(s.softThreshold(np.array([2.0, -0.1]), 0.5), [1.5, 0.0])
+
+
+def test_partial_selection_follows_numpy_indexing(fm):
+    """The reference's selection check (Partial.pyx:141-166) accepts
+    [-1, 1, 3] for 3 columns (length 3, index sum equal to arange's); numpy
+    then raises IndexError when the selection is applied.  In-range negative
+    indices wrap as numpy's do."""
+    from fastmat_amd.partial import _device_index
+    P = fm.Partial(fm.Eye(3), cols=[-1, 1, 3])
+    assert list(P.colSelection) == [-1, 1, 3]
+    with pytest.raises(IndexError):
+        _device_index(P.colSelection, 3)
+    # [2, -1, 2] also passes the reference's check; numpy reads -1 as 2
+    P2 = fm.Partial(fm.Eye(3), rows=[2, -1, 2])
+    assert list(_device_index(P2.rowSelection, 3)) == [2, 2, 2]
+    np.testing.assert_array_equal(P2._reference(), np.eye(3)[[2, -1, 2], :])
+    with pytest.raises(ValueError):
+        fm.Partial(fm.Eye(3), cols=[0, 5])
