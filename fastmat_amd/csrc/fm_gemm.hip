@@ -18,6 +18,8 @@
 // Tiling: 256 threads (4 wave64) own a 64 x 64 output tile; each wave owns a
 // 32 x 32 quarter as 2 x 2 MFMA 16x16 blocks; K advances in slabs of 16
 // staged through LDS.
+#include <type_traits>
+
 #include "fm_common.hpp"
 
 namespace fm {
@@ -255,6 +257,127 @@ static int64_t max_offset(const GemmArgs &g, int64_t os, int64_t rs, int64_t js,
     return (g.cplx ? 2 : 1) * (o_max * os + j_max * js + (rows - 1) * rs) + 1;
 }
 
+// ----------------------------------------------- other element types
+// Y = A X (adjoint: A^H X) for int8/16/32/64, float64 and complex128 data of
+// one type (ndarray.dot of the promoted operands, Matrix.pyx:1842, 1935).
+// Integers accumulate in 64-bit unsigned arithmetic and are truncated to the
+// element width at the store: the same residue modulo 2^bits as numpy's
+// integer dot, which sums in a 64-bit integer and casts the sum to the
+// output type.  float64 / complex128 accumulate in FP64.  Tiles of 64 rows x
+// 16 columns of Y per 256-thread workgroup, K in LDS slabs of 32.
+template <typename T> struct GenAcc { typedef T type; };
+template <> struct GenAcc<int8_t> { typedef uint64_t type; };
+template <> struct GenAcc<int16_t> { typedef uint64_t type; };
+template <> struct GenAcc<int32_t> { typedef uint64_t type; };
+template <> struct GenAcc<int64_t> { typedef uint64_t type; };
+
+template <typename T> __device__ __forceinline__ typename GenAcc<T>::type gen_widen(T v) {
+    if constexpr (std::is_integral<T>::value) return uint64_t(int64_t(v));
+    else return v;
+}
+template <typename T> __device__ __forceinline__ T gen_conj(T v) { return v; }
+template <> __device__ __forceinline__ cpx<double> gen_conj(cpx<double> v) { return conj(v); }
+template <typename A> __device__ __forceinline__ A gen_mac(A acc, A a, A b) { return acc + a * b; }
+template <>
+__device__ __forceinline__ cpx<double> gen_mac(cpx<double> acc, cpx<double> a, cpx<double> b) {
+    return acc + cmul(a, b);
+}
+template <typename T> __device__ __forceinline__ T gen_narrow(typename GenAcc<T>::type v) {
+    if constexpr (std::is_integral<T>::value) return T(v);  // modulo 2^bits
+    else return v;
+}
+template <typename T> __device__ __forceinline__ typename GenAcc<T>::type gen_zero() {
+    if constexpr (std::is_same<T, cpx<double>>::value) return mk(0.0, 0.0);
+    else return typename GenAcc<T>::type(0);
+}
+
+constexpr int GG_M = 64, GG_J = 16, GG_K = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_generic_kernel(int64_t M, int64_t K, int64_t k,
+                                                           const T *__restrict__ A, int64_t lda,
+                                                           int adjoint, const T *__restrict__ X,
+                                                           int64_t ldx, T *__restrict__ Y,
+                                                           int64_t ldy) {
+    typedef typename GenAcc<T>::type Acc;
+    __shared__ Acc sa[GG_K][GG_M + 1];
+    __shared__ Acc sx[GG_J][GG_K + 1];
+    const int t = threadIdx.x;
+    const int64_t i0 = int64_t(blockIdx.x) * GG_M, j0 = int64_t(blockIdx.y) * GG_J;
+    const int il = t & (GG_M - 1);
+    const int jl = (t >> 6) * 4;  // four columns per thread
+    Acc acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = gen_zero<T>();
+    for (int64_t l0 = 0; l0 < K; l0 += GG_K) {
+        // A slab: rows i0.., columns l0.. of A_eff (A or A^H)
+        for (int e = t; e < GG_M * GG_K; e += 256) {
+            const int ii = e % GG_M, ll = e / GG_M;
+            const int64_t i = i0 + ii, l = l0 + ll;
+            Acc v = gen_zero<T>();
+            if (i < M && l < K)
+                v = adjoint ? gen_widen(gen_conj(A[l * lda + i])) : gen_widen(A[i * lda + l]);
+            sa[ll][ii] = v;
+        }
+        for (int e = t; e < GG_J * GG_K; e += 256) {
+            const int ll = e % GG_K, jj = e / GG_K;
+            const int64_t l = l0 + ll, j = j0 + jj;
+            sx[jj][ll] = (l < K && j < k) ? gen_widen(X[j * ldx + l]) : gen_zero<T>();
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ll = 0; ll < GG_K; ++ll) {
+            const Acc a = sa[ll][il];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = gen_mac(acc[q], a, sx[jl + q][ll]);
+        }
+        __syncthreads();
+    }
+    const int64_t i = i0 + il;
+    if (i >= M) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t j = j0 + jl + q;
+        if (j < k) Y[j * ldy + i] = gen_narrow<T>(acc[q]);
+    }
+}
+
+template <typename T>
+static fm_status gemm_generic(int adjoint, int64_t m, int64_t n, const void *A, int64_t lda,
+                              const void *x, int64_t ldx, void *y, int64_t ldy, int64_t k,
+                              hipStream_t s) {
+    const int64_t M = adjoint ? n : m, K = adjoint ? m : n;
+    FM_REQUIRE(ceil_div(M, GG_M) < (int64_t(1) << 31) && ceil_div(k, GG_J) <= 65535,
+               FM_ERR_VALUE, "dense product too large for one launch");
+    const dim3 grid(unsigned(ceil_div(M, GG_M)), unsigned(ceil_div(k, GG_J)));
+    ProfToken tok;
+    prof_begin(s, &tok);
+    hipLaunchKernelGGL(gemm_generic_kernel<T>, grid, dim3(256), 0, s, M, K, k, (const T *)A, lda,
+                       adjoint, (const T *)x, ldx, (T *)y, ldy);
+    FM_CHECK_LAUNCH();
+    if (tok.active)
+        prof_end(s, &tok, "gemm_generic_kernel",
+                 double(sizeof(T)) * double(M * K + K * k + M * k), 2.0 * double(M) * K * k);
+    return FM_SUCCESS;
+}
+
+static fm_status dense_generic(int dtype, int adjoint, int64_t m, int64_t n, const void *A,
+                               int64_t lda, const void *x, int64_t ldx, void *y, int64_t ldy,
+                               int64_t k, hipStream_t s) {
+    switch (dtype) {
+    case FM_INT8: return gemm_generic<int8_t>(adjoint, m, n, A, lda, x, ldx, y, ldy, k, s);
+    case FM_INT16: return gemm_generic<int16_t>(adjoint, m, n, A, lda, x, ldx, y, ldy, k, s);
+    case FM_INT32: return gemm_generic<int32_t>(adjoint, m, n, A, lda, x, ldx, y, ldy, k, s);
+    case FM_INT64: return gemm_generic<int64_t>(adjoint, m, n, A, lda, x, ldx, y, ldy, k, s);
+    case FM_FLOAT64: return gemm_generic<double>(adjoint, m, n, A, lda, x, ldx, y, ldy, k, s);
+    case FM_COMPLEX128:
+        return gemm_generic<cpx<double>>(adjoint, m, n, A, lda, x, ldx, y, ldy, k, s);
+    default: break;
+    }
+    set_error("dense apply: unsupported element type %d", dtype);
+    return FM_ERR_TYPE;
+}
+
 static fm_status gemm_launch(const GemmArgs &g, hipStream_t stream) {
     const bool small = g.M <= SK_K && g.K <= SK_K && g.NC > 0 &&
                        max_offset(g, g.bos, g.brs, g.bjs, g.K) < (int64_t(1) << 31) &&
@@ -305,16 +428,21 @@ using namespace fm;
 extern "C" fm_status fm_dense_apply(int a_dtype, int x_dtype, int adjoint, int64_t m, int64_t n,
                                     const void *A, int64_t lda, const void *x, int64_t ldx,
                                     void *y, int64_t ldy, int64_t k, void *stream) {
-    FM_REQUIRE(a_dtype == FM_FLOAT32, FM_ERR_TYPE,
-               "dense apply supports float32 matrices (complex ones via their real form)");
-    FM_REQUIRE(x_dtype == FM_FLOAT32 || x_dtype == FM_COMPLEX64, FM_ERR_TYPE,
-               "dense apply supports float32 or complex64 data");
+    const bool generic = a_dtype == x_dtype && a_dtype != FM_FLOAT32 && a_dtype != FM_COMPLEX64 &&
+                         a_dtype >= FM_INT8 && a_dtype <= FM_COMPLEX128;
+    FM_REQUIRE(generic || a_dtype == FM_FLOAT32, FM_ERR_TYPE,
+               "dense apply supports float32 matrices (complex ones via their real form) and "
+               "int8/16/32/64, float64, complex128 matrices with data of the same type");
+    FM_REQUIRE(generic || x_dtype == FM_FLOAT32 || x_dtype == FM_COMPLEX64, FM_ERR_TYPE,
+               "dense apply of a float32 matrix supports float32 or complex64 data");
     FM_REQUIRE(m >= 0 && n >= 0 && k >= 0, FM_ERR_VALUE, "negative size");
-    if (m == 0 || k == 0) return FM_SUCCESS;
-    FM_REQUIRE(A && x && y, FM_ERR_VALUE, "NULL data pointer");
     const int64_t out_rows = adjoint ? n : m, in_rows = adjoint ? m : n;
+    if (out_rows == 0 || k == 0) return FM_SUCCESS;
+    FM_REQUIRE(A && x && y, FM_ERR_VALUE, "NULL data pointer");
     FM_REQUIRE(ldx >= in_rows && ldy >= out_rows && lda >= n, FM_ERR_VALUE,
                "leading dimension too small");
+    if (generic)
+        return dense_generic(a_dtype, adjoint, m, n, A, lda, x, ldx, y, ldy, k, as_stream(stream));
     GemmArgs g;
     memset(&g, 0, sizeof(g));
     g.M = out_rows;

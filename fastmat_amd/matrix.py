@@ -563,51 +563,47 @@ def _realified(A):
 
 
 def _dense_apply(M, X, adjoint):
-    """Y = A X or A^H X for a dense Matrix on the device."""
+    """Y = A X or A^H X for a dense Matrix on the device, in the promoted
+    type of the operands like ndarray.dot (Matrix.pyx:1842, 1935): FP32
+    MFMA for float32 / complex64 results (a real matrix multiplies complex
+    data as interleaved real columns, a complex one as its real 2m x 2n
+    form), the library's typed GEMM otherwise (integers wrapping modulo
+    2^bits like numpy's integer dot, float64 / complex128 in FP64)."""
     A = M._array
     m, n = A.shape
     fout = D.promote(X.dtype, M.dtype) if not M._widenInputDatatype else D.np_dtype(X.dtype)
+    fout = np.dtype(fout)
     k = X.shape[0]
     rows = n if adjoint else m
-    adt = A.dtype
-    xdt = D.np_dtype(X.dtype)
-    if adt == np.float32 and xdt in (np.float32, np.complex64):
+    if D.np_dtype(X.dtype) != fout:
+        X = X.to(D.torch_dtype(fout)).contiguous()
+    Y = D.empty(k, rows, fout)
+    if fout == np.float32 or (fout == np.complex64 and A.dtype.kind != "c"):
         A_d = M._dev.get("f32")
         if A_d is None:
-            A_d = M._dev["f32"] = D.as_tensor(A, np.float32)
-        Y = D.empty(k, rows, xdt)
-        _lib.call("fm_dense_apply", D.fm_type(np.float32), D.fm_type(xdt), int(adjoint), m, n,
+            A_d = M._dev["f32"] = D.as_tensor(A.real if A.dtype.kind == "c" else A, np.float32)
+        _lib.call("fm_dense_apply", D.fm_type(np.float32), D.fm_type(fout), int(adjoint), m, n,
                   _lib.ptr(A_d), n, _lib.ptr(X), X.shape[1], _lib.ptr(Y), rows, k, D.stream())
         return Y
-    if adt == np.complex64 and xdt == np.complex64:
+    if fout == np.complex64:
         key = "re_h" if adjoint else "re"
         A_d = M._dev.get(key)
         if A_d is None:
-            A_d = M._dev[key] = D.as_tensor(_realified(A.conj().T if adjoint else A), np.float32)
-        Y = D.empty(k, rows, np.complex64)
+            Ac = A.astype(np.complex64)
+            A_d = M._dev[key] = D.as_tensor(_realified(Ac.conj().T if adjoint else Ac),
+                                            np.float32)
         mm, nn = (n, m) if adjoint else (m, n)
         _lib.call("fm_dense_apply", D.fm_type(np.float32), D.fm_type(np.float32), 0, 2 * mm,
                   2 * nn, _lib.ptr(A_d), 2 * nn, _lib.ptr(X), 2 * X.shape[1], _lib.ptr(Y),
                   2 * rows, k, D.stream())
         return Y
-    return _generic_dense(M, X, adjoint, fout)
-
-
-def _generic_dense(M, X, adjoint, fout):
-    """Typed GEMM for the remaining dtype combinations (plain library GEMM
-    of the dense matrix on the device, like ndarray.dot)."""
     key = ("t", fout)
     A_d = M._dev.get(key)
     if A_d is None:
-        A_d = M._dev[key] = D.as_tensor(M._array, fout)
-    Xc = X.to(D.torch_dtype(fout))
-    if np.issubdtype(fout, np.integer):
-        # exact integer products: float64 is exact while |values| < 2^53
-        Af = A_d.to(torch.float64)
-        Y = (Af.conj().t() if adjoint else Af).matmul(Xc.to(torch.float64).t()).t()
-        return Y.round().to(D.torch_dtype(fout)).contiguous()
-    A_use = A_d.conj().t() if adjoint else A_d
-    return A_use.matmul(Xc.t()).t().contiguous()
+        A_d = M._dev[key] = D.as_tensor(A, fout)
+    _lib.call("fm_dense_apply", D.fm_type(fout), D.fm_type(fout), int(adjoint), m, n,
+              _lib.ptr(A_d), n, _lib.ptr(X), X.shape[1], _lib.ptr(Y), rows, k, D.stream())
+    return Y
 
 
 # ------------------------------------------------------------------ wrappers

@@ -236,6 +236,53 @@ def test_dense_mfma(fm, m, n, k):
     assert relerr(Mc.backward(xb), Ac.conj().T.astype(np.complex128) @ xb) < 1e-5
 
 
+@pytest.mark.parametrize("dtype", [np.int8, np.int16, np.int32, np.int64])
+def test_dense_integer_wraps_like_numpy_dot(fm, dtype):
+    """Integer dense products wrap modulo 2^bits exactly as numpy's integer
+    ndarray.dot (Matrix.pyx:1842) does, overflow included."""
+    rng = np.random.default_rng(np.dtype(dtype).itemsize)
+    info = np.iinfo(dtype)
+    m, n, k = 70, 129, 5
+    A = rng.integers(info.min, info.max, size=(m, n), endpoint=True).astype(dtype)
+    x = rng.integers(info.min, info.max, size=(n, k), endpoint=True).astype(dtype)
+    xb = rng.integers(info.min, info.max, size=(m, k), endpoint=True).astype(dtype)
+    M = fm.Matrix(A)
+    say("dense integer", np.dtype(dtype).name)
+    y = M.forward(x)
+    assert y.dtype == dtype
+    with np.errstate(over="ignore"):
+        ref = A.dot(x)
+        refb = A.T.dot(xb)
+    np.testing.assert_array_equal(y, ref)
+    np.testing.assert_array_equal(M.backward(xb), refb)
+    # the products do overflow the width (the test exercises the wrap)
+    big = np.abs(A.astype(object).dot(x.astype(object))).max()
+    assert big > info.max
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_dense_fp64(fm, dtype):
+    rng = np.random.default_rng(9)
+    m, n, k = 97, 61, 7
+    A = rng.standard_normal((m, n)).astype(dtype)
+    x = rng.standard_normal((n, k)).astype(dtype)
+    if np.dtype(dtype).kind == "c":
+        A = A + 1j * rng.standard_normal((m, n))
+        x = x + 1j * rng.standard_normal((n, k))
+    xb = x[:, :3].copy()
+    xb = np.resize(xb, (m, 3))
+    M = fm.Matrix(A)
+    say("dense fp64", np.dtype(dtype).name)
+    assert relerr(M.forward(x), A @ x) < 1e-13
+    assert relerr(M.backward(xb), A.conj().T @ xb) < 1e-13
+    # mixed: int32 matrix with complex64 data promotes to complex128
+    Ai = rng.integers(-5, 6, size=(m, n)).astype(np.int32)
+    xc = crandn(rng, n, 2, dtype=np.complex64)
+    yi = fm.Matrix(Ai).forward(xc)
+    assert yi.dtype == np.complex128
+    assert relerr(yi, Ai @ xc.astype(np.complex128)) < 1e-13
+
+
 # ------------------------------------------------------------------- Kron
 def test_kron_small_mixed(fm):
     rng = np.random.default_rng(11)
