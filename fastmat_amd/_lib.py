@@ -55,6 +55,9 @@ _PROTOTYPES = {
     "fm_conv_generator_grad_workspace_size": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_i64)]),
     "fm_conv_generator_grad": (_c_int, [_vp, _vp, _c_i64, _vp, _c_i64, _c_i64, _vp, _vp, _c_i64,
                                         _vp]),
+    "fm_conv_set_generator_workspace_size": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
+    "fm_conv_set_generator": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp]),
+    "fm_conv_plan_clone": (_c_int, [_vp, _vp, _pp]),
     "fm_conv_plan_destroy": (_c_int, [_vp]),
     "fm_dense_apply": (_c_int, [_c_int, _c_int, _c_int, _c_i64, _c_i64, _vp, _c_i64, _vp, _c_i64,
                                 _vp, _c_i64, _c_i64, _vp]),
@@ -70,6 +73,7 @@ _PROTOTYPES = {
     "fm_batch_plan_create": (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _pp]),
     "fm_batch_workspace_size": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
     "fm_batch_set_grid": (_c_int, [_vp, _c_int]),
+    "fm_batch_set_schedule": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "fm_batch_bind": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
                                ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _vp, _c_i64, _vp]),
     "fm_batch_apply": (_c_int, [_vp, _c_int, _vp]),
@@ -78,6 +82,10 @@ _PROTOTYPES = {
                                     ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), ctypes.POINTER(_vp),
                                     _vp, _c_i64, _vp]),
     "fm_batch_grad_apply": (_c_int, [_vp, _vp]),
+    "fm_batch_grad_apply_reuse": (_c_int, [_vp, _vp]),
+    "fm_batch_gen_workspace_size": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
+    "fm_batch_gen_bind": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), _vp, _c_i64, _vp]),
+    "fm_batch_gen_apply": (_c_int, [_vp, _vp]),
     "fm_batch_plan_destroy": (_c_int, [_vp]),
     "fm_shrink_step": (_c_int, [_c_int, _c_i64, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
                                 _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
@@ -101,6 +109,7 @@ _PROTOTYPES = {
     "fm_profiler_reset": (_c_int, []),
     "fm_profiler_count": (_c_int, [ctypes.POINTER(_c_int)]),
     "fm_profiler_entry": (_c_int, [_c_int, ctypes.c_char_p, _c_int, ctypes.POINTER(_c_i64),
+                                   ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)]),
@@ -183,7 +192,7 @@ class Profiler:
 
     >>> with Profiler() as p:
     ...     op.forward(x)
-    >>> p.results  # {kernel label: dict(launches, ms, bytes, flops)}
+    >>> p.results  # {kernel label: dict(launches, ms, bytes, flops, pass_bytes)}
     """
 
     def __init__(self):
@@ -202,10 +211,12 @@ class Profiler:
         check(lib.fm_profiler_count(ctypes.byref(n)))
         buf = ctypes.create_string_buffer(128)
         for i in range(n.value):
-            la, ms, by, fl = _c_i64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            la, ms = _c_i64(), ctypes.c_double()
+            by, fl, pb = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
             check(lib.fm_profiler_entry(i, buf, 128, ctypes.byref(la), ctypes.byref(ms),
-                                        ctypes.byref(by), ctypes.byref(fl)))
+                                        ctypes.byref(by), ctypes.byref(fl), ctypes.byref(pb)))
             self.results[buf.value.decode()] = dict(launches=la.value, ms=ms.value,
-                                                    bytes=by.value, flops=fl.value)
+                                                    bytes=by.value, flops=fl.value,
+                                                    pass_bytes=pb.value)
         check(lib.fm_profiler_reset())
         return False

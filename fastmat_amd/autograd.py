@@ -25,6 +25,8 @@ follow PyTorch's convention (grad = dL/dRe + i dL/dIm).
 """
 import torch
 
+from . import _device as D
+
 
 def _to_block(x):
     was1d = x.dim() == 1
@@ -60,22 +62,38 @@ class _ConvMatvec(torch.autograd.Function):
     @staticmethod
     def forward(ctx, gen, x, A):
         X, was1d = _to_block(x)
-        ctx.A = A
+        cdt = A.compute_dtype(X.dtype)
+        Xc = X.to(D.torch_dtype(cdt))
+        # the forward is the operator of gen's current values: the spectrum
+        # is recomputed from gen on the device (a private plan of A's)
+        h, tok = A.load_generator(gen, cdt)
+        ctx.A, ctx.cdt, ctx.tok = A, cdt, tok
         ctx.gen_complex = gen.is_complex()
         ctx.gen_dtype = gen.dtype
+        ctx.x_dtype = x.dtype
         ctx.was1d = was1d
-        ctx.save_for_backward(X)
-        return _from_block(A._apply(X, False), was1d)
+        ctx.save_for_backward(Xc, gen)
+        return _from_block(A._run_plan(h, Xc, False), was1d)
 
     @staticmethod
     def backward(ctx, g):
-        (X,) = ctx.saved_tensors
+        Xc, gen = ctx.saved_tensors
+        A, cdt = ctx.A, ctx.cdt
         G, _ = _to_block(g)
-        A = ctx.A
-        gx = _from_block(A._apply(G, True), ctx.was1d) if ctx.needs_input_grad[1] else None
+        G = G.to(D.torch_dtype(cdt)).contiguous()
+        gx = None
+        if ctx.needs_input_grad[1]:
+            h = A._gen_plan(cdt)
+            if A.generator_token(cdt) != ctx.tok:
+                # another forward loaded a different generator since
+                h, _ = A.load_generator(gen, cdt)
+            gx = A._run_plan(h, G, True)
+            if not ctx.x_dtype.is_complex:
+                gx = gx.real
+            gx = _from_block(gx.to(ctx.x_dtype), ctx.was1d)
         ggen = None
         if ctx.needs_input_grad[0]:
-            ggen = A.generator_grad(X, G)
+            ggen = A.generator_grad(Xc, G)
             if not ctx.gen_complex:
                 ggen = ggen.real
             ggen = ggen.to(ctx.gen_dtype)
@@ -83,8 +101,11 @@ class _ConvMatvec(torch.autograd.Function):
 
 
 def conv_matvec(A, gen, x):
-    """Differentiable y = A(gen) x for a Circulant/Toeplitz ``A`` built from
-    the values of ``gen`` (c, or the Toeplitz defining vector [vecC, vecR])."""
+    """Differentiable y = A(gen) x: the Circulant/Toeplitz of the structure of
+    ``A`` (sizes, kind) whose generator is the device tensor ``gen`` (c, or the
+    Toeplitz defining vector [vecC, vecR]).  Every call computes the operator
+    from gen's current values on the device, so updates of gen (an optimizer
+    step) take effect; ``A`` itself keeps its constructor's generator."""
     return _ConvMatvec.apply(gen, x, A)
 
 
@@ -108,22 +129,27 @@ class _BatchConvMatvec(torch.autograd.Function):
         gens, xs = args[:n], args[n:]
         ctx.gen_complex = [g.is_complex() for g in gens]
         ctx.gen_dtype = [g.dtype for g in gens]
+        ctx.tok = batch.load_generators(gens)
         X = [x.t().contiguous() for x in xs]
         Y = batch.forward(X)
         ctx.batch = batch
-        ctx.save_for_backward(*X)
+        ctx.n = n
+        ctx.save_for_backward(*X, *gens)
         return tuple(y.t() for y in Y)
 
     @staticmethod
     def backward(ctx, *gs):
         batch = ctx.batch
-        n = len(batch.ops)
-        X = ctx.saved_tensors
+        n = ctx.n
+        saved = ctx.saved_tensors
+        X, gens = saved[:n], saved[n:]
         G = [g.t().contiguous() for g in gs]
         need_gen = any(ctx.needs_input_grad[1:1 + n])
         need_x = any(ctx.needs_input_grad[1 + n:])
         gx = [None] * n
         if need_x:
+            if batch.generator_token != ctx.tok:
+                batch.load_generators(gens)
             gx = [t.t() for t in batch.backward(G)]
         ggen = [None] * n
         if need_gen:
@@ -134,7 +160,8 @@ class _BatchConvMatvec(torch.autograd.Function):
 
 
 def batch_conv_matvec(batch, gens, xs):
-    """Differentiable uneven batch: returns [A_i(gen_i) x_i]."""
+    """Differentiable uneven batch: returns [A_i(gen_i) x_i], every operator
+    computed from its generator's current values on the device."""
     return list(_BatchConvMatvec.apply(batch, *gens, *xs))
 
 
@@ -156,30 +183,40 @@ class _BatchConvPacked(torch.autograd.Function):
     @staticmethod
     def forward(ctx, batch, xp, *gens):
         y_buf, _, _ = _packed_buffers(batch, xp.dtype, xp.device)
+        ctx.tok = batch.load_generators(gens)
         batch.apply(batch.packed_views(xp, adjoint=False, inputs=True),
                     batch.packed_views(y_buf, adjoint=False, inputs=False), adjoint=False)
+        ctx.fwd_count = batch.apply_count[0]
         ctx.batch = batch
         ctx.gen_complex = [g.is_complex() for g in gens]
         ctx.gen_dtype = [g.dtype for g in gens]
-        ctx.save_for_backward(xp)
+        ctx.save_for_backward(xp, *gens)
         return y_buf.clone()
 
     @staticmethod
     def backward(ctx, gyp):
         batch = ctx.batch
         n = len(batch.ops)
-        (xp,) = ctx.saved_tensors
+        saved = ctx.saved_tensors
+        xp, gens = saved[0], saved[1:]
         _, g_buf, gx_buf = _packed_buffers(batch, xp.dtype, xp.device)
         g_buf.copy_(gyp)
         G = batch.packed_views(g_buf, adjoint=False, inputs=False)
         gxp = None
         if ctx.needs_input_grad[1]:
+            if batch.generator_token != ctx.tok:
+                batch.load_generators(gens)
             batch.apply(G, batch.packed_views(gx_buf, adjoint=False, inputs=True), adjoint=True)
             gxp = gx_buf.clone()
         ggen = [None] * n
         if any(ctx.needs_input_grad[2:]):
             X = batch.packed_views(xp, adjoint=False, inputs=True)
-            raw = batch.generator_grads(X, G)
+            # the forward of this step left x's column transforms in the
+            # batch workspace and the adjoint apply above g's: reuse them
+            # unless another forward ran in between
+            reuse = (ctx.needs_input_grad[1] and
+                     batch.apply_count[0] == ctx.fwd_count)
+            raw = batch.generator_grads(X, G, reuse=reuse)
             ggen = [r.to(dt) if cplx else r.real.to(dt)
                     for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
         return (None, gxp, *ggen)

@@ -42,8 +42,11 @@ class Autotuner:
 
     def best(self, key, candidates, run, reps=5):
         """Return the candidate with the lowest mean time of run(candidate)."""
-        if key in self.cache and self.cache[key]["best"] in candidates:
-            return self.cache[key]["best"]
+        if key in self.cache:
+            cached = self.cache[key]["best"]
+            cached = tuple(cached) if isinstance(cached, list) else cached
+            if cached in candidates:
+                return cached
         times = {}
         for c in candidates:
             run(c)  # warm-up / first-touch
@@ -96,8 +99,16 @@ class ConvBatch:
         self.ks = [int(k) for k in ks]
         self.dtype = np.dtype(dtype)
         prec = precision_of(self.dtype)
-        handles = [op.plan_handle(self.dtype) for op in ops]
-        arr = (ctypes.c_void_p * len(ops))(*[h.value for h in handles])
+        # private copies of the operators' plans: the batch's spectra can be
+        # rewritten from device generators (load_generators) while every
+        # operator keeps its own
+        self._own = []
+        for op in ops:
+            h = ctypes.c_void_p()
+            _lib.call("fm_conv_plan_clone", op.plan_handle(self.dtype), D.stream(),
+                      ctypes.byref(h))
+            self._own.append(h)
+        arr = (ctypes.c_void_p * len(ops))(*[h.value for h in self._own])
         karr = (ctypes.c_int64 * len(ops))(*self.ks)
         self._h = ctypes.c_void_p()
         _lib.call("fm_batch_plan_create", len(ops), arr, karr, ctypes.byref(self._h))
@@ -107,14 +118,64 @@ class ConvBatch:
         self._ws_bytes = need.value
         self._bound = [None, None]
         self.grid_per_cu = 2
+        self.group_mb = 96
+        self.lanes = 2
         self._prec = prec
 
     def __del__(self):
         try:
-            if self._h and _lib._lib is not None:
-                _lib._lib.fm_batch_plan_destroy(self._h)
+            if _lib._lib is not None:
+                if self._h:
+                    _lib._lib.fm_batch_plan_destroy(self._h)
+                for h in getattr(self, "_own", []):
+                    _lib._lib.fm_conv_plan_destroy(h)
         except Exception:
             pass
+
+    # ------------------------------------------------- device generators
+    @property
+    def generator_token(self):
+        """Changes with every load_generators call."""
+        return getattr(self, "_spec_tok", 0)
+
+    def load_generators(self, gens):
+        """Set every operator's spectrum from its device generator gens[i]
+        (c, or [vecC, vecR]; one real or complex float type for all): the
+        batch then applies the operators of the generators' current values
+        (Circulant.pyx:130, Toeplitz.pyx:253-255), as one embedding launch and
+        one batched FFT program (fm_batch_gen_*).  Returns a token that
+        changes with every load."""
+        if len(gens) != len(self.ops):
+            raise ValueError("ConvBatch: need one generator per operator")
+        gts = []
+        for op, g in zip(self.ops, gens):
+            if g.dim() != 1 or g.numel() != op._gen128.size:
+                raise ValueError("ConvBatch: generator of %d elements expected, got shape %s"
+                                 % (op._gen128.size, tuple(g.shape)))
+            gts.append(g.detach())
+        dts = {D.np_dtype(g.dtype) for g in gts}
+        ok = (np.dtype(np.float32), np.dtype(np.float64), np.dtype(np.complex64),
+              np.dtype(np.complex128))
+        if len(dts) != 1 or next(iter(dts)) not in ok:
+            gts = [g.to(D.torch_dtype(self.dtype)) for g in gts]
+        gts = [g.contiguous() for g in gts]
+        gdt = D.fm_type(gts[0].dtype)
+        key = (gdt,) + tuple(g.data_ptr() for g in gts)
+        if getattr(self, "_sbound", None) != key:
+            if getattr(self, "_sws", None) is None:
+                need = ctypes.c_int64()
+                _lib.call("fm_batch_gen_workspace_size", self._h, ctypes.byref(need))
+                self._sws = D.workspace(need.value)
+                self._sws_bytes = need.value
+            n = len(gts)
+            pg = (ctypes.c_void_p * n)(*[g.data_ptr() for g in gts])
+            _lib.call("fm_batch_gen_bind", self._h, gdt, pg, _lib.ptr(self._sws), self._sws_bytes,
+                      D.stream())
+            self._sbound = key
+        self._sgens = gts  # the bound generators stay alive with the binding
+        _lib.call("fm_batch_gen_apply", self._h, D.stream())
+        self._spec_tok = self.generator_token + 1
+        return self._spec_tok
 
     @property
     def signature(self):
@@ -123,12 +184,12 @@ class ConvBatch:
         return "batch-%s-%s" % (self.dtype.name, hashlib.sha1(desc.encode()).hexdigest()[:16])
 
     def algorithmic_bytes(self, adjoint=False):
-        """Bytes one apply must move: read every input, write every output,
-        read every spectrum once."""
+        """Bytes one apply must move: read every input block and generator,
+        write every output block."""
         esz = self.dtype.itemsize
         tot = 0
         for op, k in zip(self.ops, self.ks):
-            tot += (op.numRows + op.numCols) * k * esz + op.fftLength * esz
+            tot += ((op.numRows + op.numCols) * k + op._gen128.size) * esz
         return tot
 
     # ----------------------------------------------------------- buffers
@@ -223,12 +284,14 @@ class ConvBatch:
         # while the same addresses are handed in again
         self._bound[int(adjoint)] = key
 
-    def generator_grads(self, xs, gs):
+    def generator_grads(self, xs, gs, reuse=False):
         """Gradients of a real loss with respect to every operator's
         generator, given the forward inputs xs[i] (k_i, cols_i) and the
         output gradients gs[i] (k_i, rows_i) as contiguous blocks: what
         ``op.generator_grad(x, g)`` returns per operator, for the whole batch
-        in one batched program (fm_batch_grad_*)."""
+        in one batched program (fm_batch_grad_*).  reuse=True reads the
+        column transforms the last forward apply (of xs) and adjoint apply
+        (of gs) left in the workspace instead of recomputing them."""
         n = len(self.ops)
         for i, (op, k) in enumerate(zip(self.ops, self.ks)):
             if tuple(xs[i].shape) != (k, op.numCols) or tuple(gs[i].shape) != (k, op.numRows):
@@ -250,6 +313,12 @@ class ConvBatch:
             _lib.call("fm_batch_grad_workspace_size", self._h, ctypes.byref(need))
             self._gws = D.workspace(need.value)
             self._gws_bytes = need.value
+        # the gradient reads and writes the directions' pass-A intermediates:
+        # both directions need a binding (any buffers)
+        if self._bound[0] is None:
+            self.bind(xs, self.new_outputs(False), False)
+        if self._bound[1] is None:
+            self.bind(gs, self.new_outputs(True), True)
         key = tuple(t.data_ptr() for t in xs) + tuple(t.data_ptr() for t in gs) + \
             tuple(t.data_ptr() for t in grads)
         if getattr(self, "_gbound", None) != key:
@@ -261,7 +330,8 @@ class ConvBatch:
             _lib.call("fm_batch_grad_bind", self._h, px, lx, pg, lg, po, _lib.ptr(self._gws),
                       self._gws_bytes, D.stream())
             self._gbound = key
-        _lib.call("fm_batch_grad_apply", self._h, D.stream())
+        _lib.call("fm_batch_grad_apply_reuse" if reuse else "fm_batch_grad_apply", self._h,
+                  D.stream())
         out = self._gout.clone()
         return [out[int(off[i]):int(off[i + 1])] for i in range(n)]
 
@@ -269,9 +339,34 @@ class ConvBatch:
         _lib.call("fm_batch_set_grid", self._h, int(workgroups_per_cu))
         self.grid_per_cu = int(workgroups_per_cu)
 
+    @property
+    def schedule(self):
+        """(workgroups per CU, group budget in MB, lanes)."""
+        return (self.grid_per_cu, self.group_mb, self.lanes)
+
+    def set_schedule(self, workgroups_per_cu, group_mb, lanes):
+        """Persistent grid, budget of one group's four-step intermediates
+        (0: all multi-pass operators in one group) and lanes (streams) the
+        groups alternate between.  A new grouping rebinds on the next apply."""
+        _lib.call("fm_batch_set_schedule", self._h, int(workgroups_per_cu), int(group_mb),
+                  int(lanes))
+        if (int(group_mb), int(lanes)) != (self.group_mb, self.lanes):
+            self._bound = [None, None]
+            self._gbound = None
+        self.grid_per_cu, self.group_mb, self.lanes = (int(workgroups_per_cu), int(group_mb),
+                                                       int(lanes))
+
+    @property
+    def apply_count(self):
+        """Launches issued per direction (forward, adjoint)."""
+        if not hasattr(self, "_napply"):
+            self._napply = [0, 0]
+        return self._napply
+
     def launch(self, adjoint=False):
         """Launch the bound direction (kernels only; graph-capturable)."""
         _lib.call("fm_batch_apply", self._h, int(adjoint), D.stream())
+        self.apply_count[int(adjoint)] += 1
 
     def apply(self, ins, outs=None, adjoint=False):
         if outs is None:
@@ -287,7 +382,11 @@ class ConvBatch:
         return self.apply(ins, outs, True)
 
     # ---------------------------------------------------------- tuning
+    DEFAULT_SCHEDULES = ((2, 96, 2), (2, 64, 2), (2, 128, 1), (2, 192, 1), (2, 0, 1),
+                         (1, 96, 2), (3, 96, 2))
+
     def tune(self, ins, outs, adjoint=False, tuner=None, candidates=(1, 2, 3, 4)):
+        """Pick the persistent grid (workgroups per CU) of one direction."""
         tuner = tuner or default_tuner()
         self.bind(ins, outs, adjoint)
 
@@ -297,6 +396,27 @@ class ConvBatch:
 
         best = tuner.best(self.signature + ("-adj" if adjoint else "-fwd"), list(candidates), run)
         self.set_grid(best)
+        return best
+
+    def tune_schedule(self, xs, ys, gs, zs, tuner=None, candidates=None):
+        """Pick the schedule (grid, group budget, lanes) that minimises one
+        forward plus one adjoint apply on the given buffers, timed on the
+        device; cached per batch signature."""
+        tuner = tuner or default_tuner()
+        cands = [tuple(c) for c in (candidates or self.DEFAULT_SCHEDULES)]
+
+        def run(c):
+            self.set_schedule(*c)
+            self.bind(xs, ys, False)
+            self.bind(gs, zs, True)
+            self.launch(False)
+            self.launch(True)
+
+        best = tuner.best(self.signature + "-schedule", cands, run)
+        best = tuple(best)
+        self.set_schedule(*best)
+        self.bind(xs, ys, False)
+        self.bind(gs, zs, True)
         return best
 
     # ----------------------------------------------------------- graphs

@@ -44,6 +44,8 @@ class _ConvBase(Matrix):
 
     def __del__(self):
         try:
+            if getattr(self, "_gplans", None) is not None:
+                self._gplans.drop()
             self._plans.drop()
         except Exception:
             pass
@@ -71,10 +73,13 @@ class _ConvBase(Matrix):
             return self._multilevel_chain(cdt)._apply(X, adjoint)
         if D.np_dtype(X.dtype) != cdt:
             X = X.to(D.torch_dtype(cdt))
-        h = self._plans.get(precision_of(cdt))
+        return self._run_plan(self._plans.get(precision_of(cdt)), X, adjoint)
+
+    def _run_plan(self, h, X, adjoint):
+        """Apply the native plan `h` to the (k, n) block X of its precision."""
         k, n = X.shape
         m = self.numCols if adjoint else self.numRows
-        Y = D.empty(k, m, cdt)
+        Y = D.empty(k, m, X.dtype)
         need = ctypes.c_int64()
         _lib.call("fm_conv_workspace_size", h, k, ctypes.byref(need))
         ws = D.workspace(need.value)
@@ -153,6 +158,50 @@ class _ConvBase(Matrix):
         if self._multi is not None:
             raise NotImplementedError("generator gradient of a multi-level matrix")
         return self._generator_grad_1d(X, G)
+
+    # ------------------------------------------- spectrum from a device generator
+    def _gen_plan(self, cdt):
+        """Private copy of the native plan of precision `cdt` whose spectrum is
+        set from device generators (fm_conv_set_generator); the matrix's own
+        plan keeps the spectrum of the constructor's generator."""
+        if getattr(self, "_gplans", None) is None:
+            def clone(prec):
+                h = ctypes.c_void_p()
+                _lib.call("fm_conv_plan_clone", self._plans.get(prec), D.stream(),
+                          ctypes.byref(h))
+                return h
+            self._gplans = _PlanCache(clone, "fm_conv_plan_destroy")
+            self._gtokens = {}
+        return self._gplans.get(precision_of(cdt))
+
+    def load_generator(self, gen, cdt):
+        """Make the private plan of precision `cdt` the operator of the device
+        generator `gen` (c, or [vecC, vecR], any real or complex float type):
+        its spectrum becomes fft(t')/L of gen's current values (the operator
+        *is* its generator's spectrum, Circulant.pyx:130, Toeplitz.pyx:253-255).
+        Returns (plan handle, token); the token changes with every load."""
+        if self._multi is not None:
+            raise NotImplementedError("device generators of a multi-level matrix")
+        if gen.dim() != 1 or gen.numel() != self._gen128.size:
+            raise ValueError("generator must be a vector of %d elements" % self._gen128.size)
+        if D.np_dtype(gen.dtype) not in (np.dtype(np.float32), np.dtype(np.float64),
+                                         np.dtype(np.complex64), np.dtype(np.complex128)):
+            gen = gen.to(D.torch_dtype(cdt))
+        gen = gen.detach().contiguous()
+        h = self._gen_plan(cdt)
+        need = ctypes.c_int64()
+        _lib.call("fm_conv_set_generator_workspace_size", h, ctypes.byref(need))
+        ws = D.workspace(need.value)
+        _lib.call("fm_conv_set_generator", h, D.fm_type(gen.dtype), _lib.ptr(gen),
+                  _lib.ptr(ws), need.value, D.stream())
+        prec = precision_of(cdt)
+        tok = self._gtokens.get(prec, 0) + 1
+        self._gtokens[prec] = tok
+        return h, tok
+
+    def generator_token(self, cdt):
+        """Token of the generator last loaded into the private plan of `cdt`."""
+        return getattr(self, "_gtokens", {}).get(precision_of(cdt), 0)
 
     def _strided_ok(self, dtype):
         if self._multi is not None:

@@ -1,0 +1,1128 @@
+// Uneven batch of Circulant / Toeplitz operators on the FFT tile engine
+// (pattern (b)), its batched generator gradient (pattern (a)) and its
+// batched generator-to-spectrum program.
+//
+// No single reference call does this; it replaces the per-operator loop of
+// forward()/backward() calls a fastmat user writes (Matrix.pyx:1844-2007 once
+// per operator, each one Partial(Product(F.H, Diag, F)), Partial.pyx:268-294,
+// Product.pyx:205-240).
+//
+// Programs.  Every operator's transform is a list of passes (one fused pass,
+// or the passes A, B, C of the four-step).  A program is a list of launches;
+// each launch is one persistent kernel of one (kind, length) class that walks
+// a device table of (descriptor, tile) work items grid-strided, largest
+// first.  Launches go to lanes (the caller's stream and side streams forked
+// from it and joined back with events, all default priority); launches of
+// one lane run in order, lanes run concurrently.  Nothing waits inside a
+// kernel, and an apply only launches, so it is graph-capturable.
+//
+// Schedule of a direction (memory-side-cache grouping).  The four-step
+// intermediates of all operators together (1 GB per direction for the
+// 256-operator uneven batch) exceed the 256 MB memory-side cache (MALL), so
+// passes A, B and C of all operators in turn would stream them through HBM
+// four times.  Instead the multi-pass operators, longest first, are cut into
+// groups whose intermediates fit a budget (group_mb, default 96 MB); a group
+// runs A, B, C back to back, so pass B and C read what the pass before wrote
+// while it is still in the cache, and consecutive groups alternate between
+// `lanes` lanes so one group's passes overlap the next group's.  The one-pass
+// operators run on a lane of their own beside them.  The schedule (grid,
+// group budget, lanes) is set by fm_batch_set_schedule; the autotuner in
+// fastmat_amd/batch.py times candidate schedules on the real batch.
+//
+// Workspace.  Each direction owns two intermediates per multi-pass operator:
+// T1 = pass A's output (the column transforms with the four-step twiddle,
+// i.e. the first half of a plain FFT of the input) and T2 = pass B's output.
+// Keeping T1 lets the generator gradient reuse the forward's column
+// transforms of x and the adjoint's of g.
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "fm_fft_plan.hpp"
+
+using namespace fm;
+
+namespace fm {
+
+template <typename T> struct BatchDesc {
+    TileArgs<T> a;
+    int conv;
+};
+
+// Generator-gradient correlation of one operator (see corr_tile).
+template <typename T> struct CorrDesc {
+    TileArgs<T> a;       // x side load (column 0) and the store
+    const cpx<T> *g;     // g side, column 0
+    int64_t x_col, g_col;
+    int64_t g_valid;     // zero-pad bound of the g side
+    int64_t k;           // columns summed over
+};
+
+// One persistent kernel per (length, conv) class: grid-strided over the
+// class's (descriptor, tile) items.
+template <typename T, int LOGN, bool CONV, bool I32>
+__global__ __launch_bounds__((PassCfg<T, LOGN>::NT)) void batch_pass_kernel(
+    const BatchDesc<T> *__restrict__ descs, const int2 *__restrict__ items, int64_t nitems) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 w = items[it];
+        const TileArgs<T> a = descs[w.x].a;
+        fft_pass_body<T, LOGN, PassCfg<T, LOGN>::NT, CONV, I32, PassCfg<T, LOGN>::LE>(a, w.y, lds);
+        __syncthreads();  // the next item reuses the LDS tile
+    }
+}
+
+// Elements per thread of the correlation tiles: three transforms' worth of
+// registers are live (x, g and the running sum), so half the pass kernels'.
+template <typename T> constexpr int corr_loge() { return sizeof(T) == 4 ? 4 : 3; }
+
+// One tile of the generator-gradient correlation: for the tile's B
+// sequences (rows of a four-step, or the one sequence of a one-pass
+// transform), over all k columns c: X = FFT(x_c rows), G = FFT(g_c rows),
+// acc += G conj(X); then the inverse transform of acc (radices reversed, from
+// the registers) and the store of `a` (four-step twiddle W_L^{-n2 k1} for
+// rows).  In four-step terms: x_c rows are the forward's pass-A output, so
+// FFT(x_c rows) are bins k1 + N1 k2 of x_c's length-L transform, and the
+// stored rows followed by the column inverse (pass C) give the unnormalised
+// inverse transform of sum_c G_c conj(X_c), i.e. L times the correlation
+// sum_c sum_i g_c[i] conj(x_c[(i - d) mod L]).
+template <typename T, int LOGN, int NT, bool I32>
+__device__ __forceinline__ void corr_tile(const CorrDesc<T> &d, int64_t tile, cpx<T> *lds) {
+    constexpr int LE = corr_loge<T>();
+    using S = TileShape<T, LOGN, NT, LE>;
+    using P = typename S::P;
+    constexpr int E = S::E;
+    constexpr int LOGE = S::LOGE;
+    constexpr int LOGB = S::LOGB;
+    cpx<T> acc[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) acc[q] = mk(T(0), T(0));
+    TileArgs<T> ax = d.a, ag = d.a;
+    ag.n_valid = d.g_valid;
+    for (int64_t c = 0; c < d.k; ++c) {
+        ax.x = d.a.x + c * d.x_col;
+        ag.x = d.g + c * d.g_col;
+        cpx<T> vx[E], vg[E];
+        fft_tile_load<T, LOGN, NT, I32, LE>(ax, tile, vx);
+        fft_tile_load<T, LOGN, NT, I32, LE>(ag, tile, vg);
+        if constexpr (P::S > 0) {
+            fft_all<T, LOGN, LOGE, -1, NT, LOGB, 0, 1>(vx, lds, d.a.tw, false);
+            fft_all<T, LOGN, LOGE, -1, NT, LOGB, 0, 1>(vg, lds, d.a.tw, false);
+        }
+#pragma unroll
+        for (int q = 0; q < E; ++q) acc[q] = acc[q] + cmulc(vg[q], vx[q]);
+    }
+    if constexpr (P::S > 0)
+        fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 1, true>(acc, lds, d.a.tw, false);
+    constexpr int LOGRL = P::S > 0 ? P::bits(0) : 0;
+    fft_tile_store<T, LOGN, NT, LOGRL, I32, LE>(d.a, tile, acc, lds, false);
+}
+
+template <typename T, int LOGN, int NT, bool I32>
+__global__ __launch_bounds__(NT) void batch_corr_kernel(const CorrDesc<T> *__restrict__ descs,
+                                                        const int2 *__restrict__ items,
+                                                        int64_t nitems) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 w = items[it];
+        const CorrDesc<T> dsc = descs[w.x];
+        corr_tile<T, LOGN, NT, I32>(dsc, w.y, lds);
+        __syncthreads();
+    }
+}
+
+template <typename T> constexpr int batch_max_log() { return sizeof(T) == 4 ? 13 : 12; }
+
+// Threads of a correlation tile: rows of a four-step get 8 sequences per
+// tile (at least 64 threads), one-pass transforms one sequence per tile.
+constexpr int corr_threads(int logN, int loge, bool rows) {
+    const int per_seq = logN > loge ? (1 << (logN - loge)) : 1;
+    const int nt = rows ? per_seq * 8 : per_seq;
+    return nt < 64 ? 64 : (nt > 512 ? 512 : nt);
+}
+
+enum LaunchKind { LK_FFT = 0, LK_CONV = 1, LK_CORR = 2 };
+
+// One launch of a program: the work items of one kernel class on one lane.
+struct Launch {
+    int kind, logN, nt, lane;
+    int idx32;  // every descriptor of the launch fits 32-bit index arithmetic
+    void *items;
+    int64_t nitems;
+    double bytes;       // algorithmic bytes, for the library's profiler
+    double pass_bytes;  // what the launch reads and writes
+};
+
+struct Program {
+    void *descs = nullptr;   // BatchDesc<T>[]
+    void *cdescs = nullptr;  // CorrDesc<T>[]
+    std::vector<Launch> launches;
+    int lanes = 1;
+    bool bound = false;
+};
+
+// Builder of a program: descriptors, and "pass groups" (a lane and the
+// descriptors that form one step of that lane).  Inside a pass group the
+// descriptors are split by kernel class, and the classes are issued largest
+// first.
+template <typename T> struct ProgBuilder {
+    std::vector<BatchDesc<T>> descs;
+    std::vector<CorrDesc<T>> cdescs;
+    struct Ref {
+        int corr;
+        int index;
+    };
+    struct PassGroup {
+        int lane;
+        std::vector<Ref> refs;
+    };
+    std::deque<PassGroup> groups;  // references stay valid as groups are added
+    int lanes = 1;
+
+    int add(const PassSpec<T> &ps) {
+        TileArgs<T> a = ps.a;
+        descs.push_back(BatchDesc<T>{a, ps.conv ? 1 : 0});
+        return int(descs.size()) - 1;
+    }
+    int add_corr(const CorrDesc<T> &cd) {
+        cdescs.push_back(cd);
+        return int(cdescs.size()) - 1;
+    }
+    PassGroup &group(int lane) {
+        groups.push_back(PassGroup{lane, {}});
+        if (lane + 1 > lanes) lanes = lane + 1;
+        return groups.back();
+    }
+};
+
+}  // namespace fm
+
+struct fm_batch_plan {
+    int device;
+    int prec;
+    int count;
+    std::vector<const fm_conv_plan *> plans;
+    std::vector<int64_t> ks;
+    // per direction and operator: offsets of T1 and T2 in the workspace
+    std::vector<int64_t> t1_off[2], t2_off[2];
+    int64_t ws_bytes;
+    // schedule
+    int grid_per_cu;
+    int group_mb;
+    int lanes;
+    int ncu;
+    // bound programs: 0 forward, 1 adjoint
+    Program dir[2];
+    void *ws_bound[2];                       // workspace of the bound direction
+    std::vector<const void *> ins_bound[2];  // its input blocks
+    std::vector<int64_t> ld_bound[2];
+    // generator gradient: pass A of x and of g (into the directions' T1),
+    // the correlations, pass C of the four-step rows, the fold
+    Program gpassA, gcorr, gpassC;
+    void *gtab;
+    int64_t g_maxgen;
+    void *g_ws;                               // the workspace the gradient reads T1 from
+    std::vector<const void *> g_xs, g_gs;     // its x and g blocks
+    std::vector<int64_t> g_ldx, g_ldg;
+    bool g_bound;
+    // generator-to-spectrum program: one embedding launch over all
+    // operators, then every operator's spectrum FFT
+    Program sprog;
+    void *stab;
+    int s_gen_dtype;
+    int64_t s_maxL;
+    // side streams (default priority) for the lanes, forked from and joined
+    // back into the caller's stream
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide];
+    hipEvent_t fork_ev, join_ev[kSide];
+    bool has_side;
+};
+
+static void free_program(Program &d) {
+    if (d.descs) (void)hipFree(d.descs);
+    if (d.cdescs) (void)hipFree(d.cdescs);
+    for (auto &l : d.launches)
+        if (l.items) (void)hipFree(l.items);
+    d = Program();
+}
+
+namespace fm {
+
+static double pass_bytes(const TileArgs<float> &a, size_t esz) {
+    const double span = double(a.J) * double(int64_t(1) << a.logN);
+    const double in = double(a.O) * (double(a.n_valid) < span ? double(a.n_valid) : span);
+    const double out = double(a.O) * (double(a.m_valid) < span ? double(a.m_valid) : span);
+    return (in + out) * double(esz);
+}
+static double pass_bytes(const TileArgs<double> &a, size_t esz) {
+    const double span = double(a.J) * double(int64_t(1) << a.logN);
+    const double in = double(a.O) * (double(a.n_valid) < span ? double(a.n_valid) : span);
+    const double out = double(a.O) * (double(a.m_valid) < span ? double(a.m_valid) : span);
+    return (in + out) * double(esz);
+}
+
+// Normalise the descriptors, group work items into launches and upload.
+template <typename T>
+static fm_status finalize(ProgBuilder<T> &b, Program &prog, int ncu_lanes_cap, hipStream_t s) {
+    free_program(prog);
+    for (auto &d : b.descs) {
+        fm_status st = normalize_pass(d.a);
+        if (st) return st;
+    }
+    for (auto &c : b.cdescs) {
+        // the correlation tiles use corr_loge elements per thread: set the
+        // tile shape (B sequences) for that shape
+        const int loge = corr_loge<T>();
+        const bool rows = c.a.J > 1;
+        const int nt = corr_threads(c.a.logN, loge, rows);
+        int logb = 0;
+        while ((1 << logb) * (int64_t(1) << c.a.logN) < int64_t(nt) << loge) ++logb;
+        c.a.B = 1 << logb;
+        c.a.ntiles = ceil_div(c.a.O * c.a.J, c.a.B);
+        c.a.logJ = is_pow2(c.a.J) ? ilog2(c.a.J) : -1;
+        c.a.o_group = 1;
+    }
+    prog.lanes = b.lanes < ncu_lanes_cap ? b.lanes : ncu_lanes_cap;
+    if (prog.lanes < 1) prog.lanes = 1;
+    if (!b.descs.empty()) {
+        FM_CHECK_HIP(hipMalloc(&prog.descs, sizeof(BatchDesc<T>) * b.descs.size()));
+        FM_CHECK_HIP(hipMemcpyAsync(prog.descs, b.descs.data(),
+                                    sizeof(BatchDesc<T>) * b.descs.size(),
+                                    hipMemcpyHostToDevice, s));
+    }
+    if (!b.cdescs.empty()) {
+        FM_CHECK_HIP(hipMalloc(&prog.cdescs, sizeof(CorrDesc<T>) * b.cdescs.size()));
+        FM_CHECK_HIP(hipMemcpyAsync(prog.cdescs, b.cdescs.data(),
+                                    sizeof(CorrDesc<T>) * b.cdescs.size(),
+                                    hipMemcpyHostToDevice, s));
+    }
+    const size_t esz = sizeof(cpx<T>);
+    std::vector<std::vector<int2>> staged;  // host item lists, kept until the sync below
+    for (auto &pg : b.groups) {
+        // classes of this pass group: (kind, logN, nt) -> items
+        std::map<std::tuple<int, int, int>, std::vector<int2>> cls;
+        std::map<std::tuple<int, int, int>, double> cbytes, calg;
+        std::map<std::tuple<int, int, int>, bool> cfit;
+        for (const auto &r : pg.refs) {
+            const TileArgs<T> &a = r.corr ? b.cdescs[r.index].a : b.descs[r.index].a;
+            int kind = r.corr ? LK_CORR : (b.descs[r.index].conv ? LK_CONV : LK_FFT);
+            int nt = r.corr ? corr_threads(a.logN, corr_loge<T>(), a.J > 1) : 0;
+            auto key = std::make_tuple(kind, a.logN, nt);
+            bool fit = fits_idx32(a, int64_t(1) << a.logN);
+            if (r.corr) {
+                const CorrDesc<T> &c = b.cdescs[r.index];
+                // column offsets of both sides stay in 32 bits as well
+                const double lim = double((uint64_t(1) << 32) / sizeof(cpx<T>));
+                fit = fit && double(c.k) * double(c.x_col) < lim &&
+                      double(c.k) * double(c.g_col) < lim;
+            }
+            auto fi = cfit.find(key);
+            if (fi == cfit.end()) cfit[key] = fit;
+            else fi->second = fi->second && fit;
+            auto &v = cls[key];
+            for (int64_t t = 0; t < a.ntiles; ++t) {
+                int2 w;
+                w.x = r.index;
+                w.y = int(t);
+                v.push_back(w);
+            }
+            double by = pass_bytes(a, esz);
+            if (r.corr) {
+                const CorrDesc<T> &c = b.cdescs[r.index];
+                by = (double(c.k) * 2.0 * double(int64_t(1) << a.logN) * double(a.J) +
+                      double(a.J) * double(int64_t(1) << a.logN)) * double(esz);
+            }
+            cbytes[key] += by;
+            calg[key] += a.alg_bytes;
+        }
+        std::vector<std::tuple<int, int, int>> order;
+        for (auto &kv : cls) order.push_back(kv.first);
+        std::sort(order.begin(), order.end(),
+                  [&](const std::tuple<int, int, int> &x, const std::tuple<int, int, int> &y) {
+                      return cbytes[x] > cbytes[y];
+                  });
+        for (auto &key : order) {
+            Launch l;
+            l.kind = std::get<0>(key);
+            l.logN = std::get<1>(key);
+            l.nt = std::get<2>(key);
+            l.lane = pg.lane < prog.lanes ? pg.lane : pg.lane % prog.lanes;
+            l.idx32 = cfit[key] ? 1 : 0;
+            l.nitems = int64_t(cls[key].size());
+            l.bytes = calg[key];
+            l.pass_bytes = cbytes[key];
+            l.items = nullptr;
+            FM_CHECK_HIP(hipMalloc(&l.items, sizeof(int2) * cls[key].size()));
+            staged.push_back(std::move(cls[key]));
+            FM_CHECK_HIP(hipMemcpyAsync(l.items, staged.back().data(),
+                                        sizeof(int2) * staged.back().size(),
+                                        hipMemcpyHostToDevice, s));
+            prog.launches.push_back(l);
+        }
+    }
+    FM_CHECK_HIP(hipStreamSynchronize(s));
+    prog.bound = true;
+    return FM_SUCCESS;
+}
+
+// ------------------------------------------------------------- launching
+template <typename T, int LOGN, bool CONV>
+static fm_status launch_pass_class(const fm_batch_plan *p, const Program &d, const Launch &l,
+                                   hipStream_t s) {
+    constexpr int N = 1 << LOGN;
+    const size_t lds = size_t(tile_batch<T>(LOGN)) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)batch_pass_kernel<T, LOGN, CONV, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)batch_pass_kernel<T, LOGN, CONV, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    int64_t grid = int64_t(p->ncu) * p->grid_per_cu;
+    if (grid > l.nitems) grid = l.nitems;
+    ProfToken tok;
+    prof_begin(s, &tok);
+    constexpr int NT = PassCfg<T, LOGN>::NT;
+    if (l.idx32)
+        hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, true>), dim3(unsigned(grid)), dim3(NT),
+                           lds, s, (const BatchDesc<T> *)d.descs, (const int2 *)l.items, l.nitems);
+    else
+        hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, false>), dim3(unsigned(grid)),
+                           dim3(NT), lds, s, (const BatchDesc<T> *)d.descs, (const int2 *)l.items,
+                           l.nitems);
+    FM_CHECK_LAUNCH();
+    if (tok.active) {
+        char name[80];
+        snprintf(name, sizeof(name), "batch_pass_kernel<%s,%d,%s>",
+                 sizeof(T) == 4 ? "float" : "double", LOGN, CONV ? "conv" : "fft");
+        prof_end(s, &tok, name, l.bytes, 0.0, l.pass_bytes);
+    }
+    return FM_SUCCESS;
+}
+
+template <typename T, int LOGN, int NT>
+static fm_status launch_corr_nt(const fm_batch_plan *p, const Program &d, const Launch &l,
+                                hipStream_t s) {
+    constexpr int N = 1 << LOGN;
+    constexpr int LE = corr_loge<T>();
+    using S = TileShape<T, LOGN, NT, LE>;
+    const size_t lds = size_t(S::B) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)batch_corr_kernel<T, LOGN, NT, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void *)batch_corr_kernel<T, LOGN, NT, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    FM_REQUIRE(lds <= 160 * 1024, FM_ERR_VALUE, "correlation tile needs %zu bytes of LDS", lds);
+    int64_t grid = int64_t(p->ncu) * p->grid_per_cu;
+    if (grid > l.nitems) grid = l.nitems;
+    ProfToken tok;
+    prof_begin(s, &tok);
+    if (l.idx32)
+        hipLaunchKernelGGL((batch_corr_kernel<T, LOGN, NT, true>), dim3(unsigned(grid)), dim3(NT),
+                           lds, s, (const CorrDesc<T> *)d.cdescs, (const int2 *)l.items,
+                           l.nitems);
+    else
+        hipLaunchKernelGGL((batch_corr_kernel<T, LOGN, NT, false>), dim3(unsigned(grid)),
+                           dim3(NT), lds, s, (const CorrDesc<T> *)d.cdescs,
+                           (const int2 *)l.items, l.nitems);
+    FM_CHECK_LAUNCH();
+    if (tok.active) {
+        char name[80];
+        snprintf(name, sizeof(name), "batch_corr_kernel<%s,%d,%d>",
+                 sizeof(T) == 4 ? "float" : "double", LOGN, NT);
+        prof_end(s, &tok, name, l.bytes, 0.0, l.pass_bytes);
+    }
+    return FM_SUCCESS;
+}
+
+template <typename T, int LOGN>
+static fm_status launch_corr(const fm_batch_plan *p, const Program &d, const Launch &l,
+                             hipStream_t s) {
+    // the two tile shapes corr_threads chooses for this length
+    constexpr int NT_ROWS = corr_threads(LOGN, corr_loge<T>(), true);
+    constexpr int NT_ONE = corr_threads(LOGN, corr_loge<T>(), false);
+    if (l.nt == NT_ROWS) return launch_corr_nt<T, LOGN, NT_ROWS>(p, d, l, s);
+    if constexpr (NT_ONE != NT_ROWS)
+        if (l.nt == NT_ONE) return launch_corr_nt<T, LOGN, NT_ONE>(p, d, l, s);
+    set_error("correlation launch with %d threads", l.nt);
+    return FM_ERR_VALUE;
+}
+
+template <typename T>
+static fm_status launch_one(const fm_batch_plan *p, const Program &d, const Launch &l,
+                            hipStream_t s) {
+    fm_status st = FM_ERR_VALUE;
+#define FM_BATCH_CASE(L)                                                                     \
+    case L:                                                                                  \
+        st = l.kind == LK_CORR   ? launch_corr<T, L>(p, d, l, s)                             \
+             : l.kind == LK_CONV ? launch_pass_class<T, L, true>(p, d, l, s)                 \
+                                 : launch_pass_class<T, L, false>(p, d, l, s);               \
+        break;
+    switch (l.logN) {
+        FM_BATCH_CASE(0) FM_BATCH_CASE(1) FM_BATCH_CASE(2) FM_BATCH_CASE(3)
+        FM_BATCH_CASE(4) FM_BATCH_CASE(5) FM_BATCH_CASE(6) FM_BATCH_CASE(7)
+        FM_BATCH_CASE(8) FM_BATCH_CASE(9) FM_BATCH_CASE(10) FM_BATCH_CASE(11)
+        FM_BATCH_CASE(12)
+    case 13:
+        if constexpr (sizeof(T) == 4)
+            st = l.kind == LK_CORR   ? launch_corr<T, 13>(p, d, l, s)
+                 : l.kind == LK_CONV ? launch_pass_class<T, 13, true>(p, d, l, s)
+                                     : launch_pass_class<T, 13, false>(p, d, l, s);
+        break;
+    default:
+        break;
+    }
+#undef FM_BATCH_CASE
+    if (st == FM_ERR_VALUE && !(l.logN >= 0 && l.logN <= batch_max_log<T>()))
+        set_error("batched transform length 2^%d out of range", l.logN);
+    return st;
+}
+
+// Issue a program: lanes beyond the first fork from the caller's stream and
+// join back into it (event record / wait only, so it is graph-capturable).
+template <typename T>
+static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream_t s) {
+    const int lanes = p->has_side ? (d.lanes < fm_batch_plan::kSide + 1 ? d.lanes
+                                                                       : fm_batch_plan::kSide + 1)
+                                  : 1;
+    if (lanes > 1) {
+        FM_CHECK_HIP(hipEventRecord(p->fork_ev, s));
+        for (int q = 0; q < lanes - 1; ++q) FM_CHECK_HIP(hipStreamWaitEvent(p->side[q], p->fork_ev, 0));
+    }
+    for (const Launch &l : d.launches) {
+        const int lane = lanes > 1 ? l.lane % lanes : 0;
+        hipStream_t sq = lane == 0 ? s : p->side[lane - 1];
+        fm_status st = launch_one<T>(p, d, l, sq);
+        if (st) return st;
+    }
+    if (lanes > 1) {
+        for (int q = 0; q < lanes - 1; ++q) {
+            FM_CHECK_HIP(hipEventRecord(p->join_ev[q], p->side[q]));
+            FM_CHECK_HIP(hipStreamWaitEvent(s, p->join_ev[q], 0));
+        }
+    }
+    return FM_SUCCESS;
+}
+
+static int64_t up256(int64_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace fm
+
+// ======================================================================
+//                               plan
+// ======================================================================
+extern "C" fm_status fm_batch_plan_create(int count, const fm_conv_plan *const *plans,
+                                          const int64_t *ks, fm_batch_plan **out) {
+    FM_REQUIRE(out != nullptr, FM_ERR_VALUE, "plan pointer is NULL");
+    *out = nullptr;
+    FM_REQUIRE(count >= 1 && plans && ks, FM_ERR_VALUE, "empty batch");
+    for (int i = 0; i < count; ++i)
+        FM_REQUIRE(plans[i] && plans[i]->kind != 2, FM_ERR_NOT_IMPLEMENTED,
+                   "batch entry %d: only one-level convolution plans batch", i);
+    const int prec = plans[0]->prec;
+    auto *p = new fm_batch_plan();
+    p->prec = prec;
+    p->count = count;
+    p->ws_bytes = 0;
+    p->grid_per_cu = 2;
+    p->group_mb = 96;
+    p->lanes = 2;
+    (void)hipGetDevice(&p->device);
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+        ncu <= 0)
+        ncu = 256;
+    p->ncu = ncu;
+    {
+        bool ok = true;
+        for (int q = 0; q < fm_batch_plan::kSide && ok; ++q) {
+            p->side[q] = nullptr;
+            p->join_ev[q] = nullptr;
+            ok = hipStreamCreateWithFlags(&p->side[q], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&p->join_ev[q], hipEventDisableTiming) == hipSuccess;
+        }
+        p->fork_ev = nullptr;
+        ok = ok && hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming) == hipSuccess;
+        p->has_side = ok;
+    }
+    const int maxlog = prec == FM_COMPLEX64 ? batch_max_log<float>() : batch_max_log<double>();
+    int64_t half = 0;
+    std::vector<int64_t> t1, t2;
+    for (int i = 0; i < count; ++i) {
+        if (!plans[i] || plans[i]->prec != prec || ks[i] < 1) {
+            fm_batch_plan_destroy(p);
+            set_error("batch entry %d: missing plan, mixed precision or no columns", i);
+            return FM_ERR_VALUE;
+        }
+        const Geometry &g = plans[i]->geo;
+        const int worst = g.passes == 1 ? g.logL : (g.logN1 > g.logN2 ? g.logN1 : g.logN2);
+        if (worst > maxlog) {
+            fm_batch_plan_destroy(p);
+            set_error("batch entry %d: transform length exceeds the batched kernel limit", i);
+            return FM_ERR_NOT_IMPLEMENTED;
+        }
+        p->plans.push_back(plans[i]);
+        p->ks.push_back(ks[i]);
+        int64_t o1 = -1, o2 = -1;
+        if (g.passes > 1) {
+            const int64_t b = up256(int64_t(esize(prec)) * plans[i]->L * ks[i]);
+            o1 = half;
+            o2 = half + b;
+            half += 2 * b;
+        }
+        t1.push_back(o1);
+        t2.push_back(o2);
+    }
+    for (int d = 0; d < 2; ++d) {
+        p->t1_off[d] = t1;
+        p->t2_off[d] = t2;
+        for (int i = 0; i < count; ++i)
+            if (t1[i] >= 0) {
+                p->t1_off[d][i] += d * half;
+                p->t2_off[d][i] += d * half;
+            }
+    }
+    p->ws_bytes = 2 * half;
+    *out = p;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_workspace_size(const fm_batch_plan *p, int64_t *bytes) {
+    FM_REQUIRE(p && bytes, FM_ERR_VALUE, "NULL argument");
+    *bytes = p->ws_bytes;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_set_grid(fm_batch_plan *p, int workgroups_per_cu) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(workgroups_per_cu >= 1 && workgroups_per_cu <= 16, FM_ERR_VALUE,
+               "workgroups per CU must be in [1, 16]");
+    p->grid_per_cu = workgroups_per_cu;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_set_schedule(fm_batch_plan *p, int workgroups_per_cu, int group_mb,
+                                           int lanes) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(workgroups_per_cu >= 1 && workgroups_per_cu <= 16, FM_ERR_VALUE,
+               "workgroups per CU must be in [1, 16]");
+    FM_REQUIRE(group_mb >= 0, FM_ERR_VALUE, "negative group budget");
+    FM_REQUIRE(lanes >= 1 && lanes <= fm_batch_plan::kSide + 1, FM_ERR_VALUE,
+               "lanes must be in [1, %d]", fm_batch_plan::kSide + 1);
+    p->grid_per_cu = workgroups_per_cu;
+    const bool regroup = group_mb != p->group_mb || lanes != p->lanes;
+    p->group_mb = group_mb;
+    p->lanes = lanes;
+    if (regroup) {
+        // the directions' programs encode the grouping: bind again
+        p->dir[0].bound = false;
+        p->dir[1].bound = false;
+    }
+    return FM_SUCCESS;
+}
+
+// ======================================================================
+//                       forward / adjoint programs
+// ======================================================================
+template <typename T>
+static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *ins,
+                              const int64_t *ld_in, void *const *outs, const int64_t *ld_out,
+                              void *workspace, hipStream_t s) {
+    const int dd = adjoint ? 1 : 0;
+    std::vector<PassList<T>> pipes(p->count);
+    for (int i = 0; i < p->count; ++i) {
+        const fm_conv_plan *c = p->plans[i];
+        const int64_t nin = adjoint ? c->rows : c->cols;
+        const int64_t mout = adjoint ? c->cols : c->rows;
+        FM_REQUIRE(ins[i] && outs[i], FM_ERR_VALUE, "batch entry %d: NULL buffer", i);
+        FM_REQUIRE(ld_in[i] >= nin && ld_out[i] >= mout, FM_ERR_VALUE,
+                   "batch entry %d: leading dimension too small", i);
+        PipeIO io;
+        memset(&io, 0, sizeof(io));
+        io.x = ins[i]; io.ldx = ld_in[i]; io.y = outs[i]; io.ldy = ld_out[i]; io.k = p->ks[i];
+        io.n_in = nin; io.m_out = mout; io.D = c->D; io.conj_d = adjoint ? 1 : 0;
+        if (c->geo.passes > 1) {
+            FM_REQUIRE(workspace != nullptr, FM_ERR_VALUE, "batch needs a workspace");
+            io.work = (char *)workspace + p->t1_off[dd][i];
+        }
+        fm_status st = build_pipeline<T>(c->geo, io, &pipes[i]);
+        if (st) return st;
+        // algorithmic bytes: input and output blocks and the generator,
+        // split over the passes
+        for (auto &ps : pipes[i])
+            ps.a.alg_bytes = double((nin + mout) * p->ks[i] + c->gen_len) * double(sizeof(cpx<T>)) /
+                             double(pipes[i].size());
+        if (c->geo.passes > 1) {
+            // pass B writes T2 (T1 keeps pass A's column transforms), pass C
+            // reads T2
+            cpx<T> *t2 = (cpx<T> *)((char *)workspace + p->t2_off[dd][i]);
+            pipes[i][1].a.y = t2;
+            pipes[i][2].a.x = t2;
+        }
+    }
+    // groups of multi-pass operators, longest transform first, whose
+    // intermediates (T1 + T2) stay within the budget
+    std::vector<int> multi, single;
+    for (int i = 0; i < p->count; ++i) (p->plans[i]->geo.passes > 1 ? multi : single).push_back(i);
+    std::stable_sort(multi.begin(), multi.end(), [&](int a, int b) {
+        return p->plans[a]->L > p->plans[b]->L;
+    });
+    const int64_t budget = int64_t(p->group_mb) << 20;
+    ProgBuilder<T> b;
+    const int lanes = p->lanes;
+    // one-pass operators: a lane of their own when there is one to spare
+    if (!single.empty()) {
+        auto &pg = b.group(lanes > 1 ? lanes : 0);
+        for (int i : single) pg.refs.push_back({0, b.add(pipes[i][0])});
+    }
+    size_t q = 0;
+    int gi = 0;
+    while (q < multi.size()) {
+        size_t r = q;
+        int64_t used = 0;
+        while (r < multi.size()) {
+            const fm_conv_plan *c = p->plans[multi[r]];
+            const int64_t need = 2 * int64_t(sizeof(cpx<T>)) * c->L * p->ks[multi[r]];
+            if (r > q && budget > 0 && used + need > budget) break;
+            used += need;
+            ++r;
+        }
+        const int lane = gi % lanes;
+        for (int ph = 0; ph < 3; ++ph) {
+            auto &pg = b.group(lane);
+            for (size_t t = q; t < r; ++t) pg.refs.push_back({0, b.add(pipes[multi[t]][ph])});
+        }
+        q = r;
+        ++gi;
+    }
+    fm_status st = finalize<T>(b, p->dir[dd], fm_batch_plan::kSide + 1, s);
+    if (st) return st;
+    p->ws_bound[dd] = workspace;
+    p->ins_bound[dd].assign(ins, ins + p->count);
+    p->ld_bound[dd].assign(ld_in, ld_in + p->count);
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_bind(fm_batch_plan *p, int adjoint, const void *const *ins,
+                                   const int64_t *ld_in, void *const *outs,
+                                   const int64_t *ld_out, void *workspace,
+                                   int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p && ins && ld_in && outs && ld_out, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(workspace_bytes >= p->ws_bytes, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)p->ws_bytes);
+    if (p->prec == FM_COMPLEX64)
+        return batch_bind_t<float>(p, adjoint, ins, ld_in, outs, ld_out, workspace,
+                                   as_stream(stream));
+    return batch_bind_t<double>(p, adjoint, ins, ld_in, outs, ld_out, workspace,
+                                as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_apply(const fm_batch_plan *p, int adjoint, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    const Program &d = p->dir[adjoint ? 1 : 0];
+    FM_REQUIRE(d.bound, FM_ERR_VALUE, "batch direction %d is not bound to buffers", adjoint);
+    if (p->prec == FM_COMPLEX64) return run_program<float>(p, d, as_stream(stream));
+    return run_program<double>(p, d, as_stream(stream));
+}
+
+// ======================================================================
+//                     batched generator gradient
+// ======================================================================
+// For every operator i (what fm_conv_generator_grad returns for it alone):
+//   multi-pass: T1x = pass A of x (the forward direction's T1), T1g = pass A
+//     of g (the adjoint direction's T1); one correlation tile per block of
+//     rows k1 sums FFT(T1g rows) conj(FFT(T1x rows)) over the columns,
+//     transforms back and applies W_L^{-n2 k1}; pass C (column inverse)
+//     gives r = the unnormalised inverse transform of
+//     sum_c FFT_L(g_c') conj(FFT_L(x_c')).
+//   one-pass: one correlation tile over the operator's columns straight
+//     from x and g (zero-padded to L), stored as r.
+//   then r is folded onto the generator and scaled by 1/L.
+// fm_batch_grad_apply_reuse skips both pass-A launches: valid when the
+// forward direction was last applied with inputs xs and the adjoint with
+// inputs gs (their T1 hold exactly those column transforms).
+namespace fm {
+template <typename T> struct GradEntry {
+    const cpx<T> *r;
+    cpx<T> *out;
+    int64_t L, head, tail, gen_len;
+    int kind;
+};
+
+template <typename T>
+__global__ void batch_fold_kernel(const GradEntry<T> *__restrict__ tab) {
+    const GradEntry<T> e = tab[blockIdx.y];
+    const T s = T(1.0 / double(e.L));
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < e.gen_len; i += stride) {
+        cpx<T> v;
+        if (e.kind == 0) {
+            v = e.r[i];
+            if (e.tail > 0 && i >= 1) v = v + e.r[e.L - e.tail + i - 1];
+        } else {
+            v = i < e.head ? e.r[i] : e.r[e.L - e.tail + (i - e.head)];
+        }
+        e.out[i] = scale(v, s);
+    }
+}
+
+// Gradient workspace of one operator: rows of the correlation (four-step
+// only) and r.
+static int64_t grad_ws_entry(const fm_conv_plan *c) {
+    const int64_t e = int64_t(esize(c->prec)), L = c->L;
+    return (c->geo.passes > 1 ? up256(e * L) : 0) + up256(e * L);
+}
+
+template <typename T>
+static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, const int64_t *ldx,
+                                   const void *const *gs, const int64_t *ldg,
+                                   void *const *grads, char *ws, hipStream_t s) {
+    FM_REQUIRE(p->ws_bound[0] != nullptr || p->ws_bound[1] != nullptr || p->ws_bytes == 0,
+               FM_ERR_VALUE, "bind the batch's directions (fm_batch_bind) before its gradient");
+    char *bws = (char *)(p->ws_bound[0] ? p->ws_bound[0] : p->ws_bound[1]);
+    ProgBuilder<T> ba, bc, bp;
+    auto &pa_x = ba.group(0);
+    auto &pa_g = ba.group(p->lanes > 1 ? 1 : 0);
+    auto &pc = bc.group(0);
+    auto &pp = bp.group(0);
+    std::vector<GradEntry<T>> tab(p->count);
+    int64_t off = 0, maxgen = 1;
+    for (int i = 0; i < p->count; ++i) {
+        const fm_conv_plan *c = p->plans[i];
+        const int64_t k = p->ks[i], L = c->L;
+        FM_REQUIRE(xs[i] && gs[i] && grads[i], FM_ERR_VALUE, "batch entry %d: NULL buffer", i);
+        FM_REQUIRE(ldx[i] >= c->cols && ldg[i] >= c->rows, FM_ERR_VALUE,
+                   "batch entry %d: leading dimension too small", i);
+        char *e = ws + off;
+        off += grad_ws_entry(c);
+        const bool multi = c->geo.passes > 1;
+        cpx<T> *rows = multi ? (cpx<T> *)e : nullptr;
+        cpx<T> *r = (cpx<T> *)(e + (multi ? up256(int64_t(sizeof(cpx<T>)) * L) : 0));
+        CorrDesc<T> cd;
+        memset(&cd, 0, sizeof(cd));
+        TileArgs<T> a = blank_args<T>();
+        if (multi) {
+            const Geometry &g = c->geo;
+            const int64_t N1 = int64_t(1) << g.logN1, N2 = int64_t(1) << g.logN2;
+            cpx<T> *t1x = (cpx<T> *)(bws + p->t1_off[0][i]);
+            cpx<T> *t1g = (cpx<T> *)(bws + p->t1_off[1][i]);
+            // pass A of x and of g: the conv pipelines' pass A into T1
+            for (int side = 0; side < 2; ++side) {
+                PipeIO io;
+                memset(&io, 0, sizeof(io));
+                io.x = side ? gs[i] : xs[i];
+                io.ldx = side ? ldg[i] : ldx[i];
+                io.y = r;  // unused by pass A
+                io.ldy = L;
+                io.k = k;
+                io.n_in = side ? c->rows : c->cols;
+                io.m_out = L;
+                io.D = c->D;
+                io.work = side ? t1g : t1x;
+                PassList<T> pl;
+                fm_status st = build_pipeline<T>(g, io, &pl);
+                if (st) return st;
+                pl[0].a.alg_bytes = 0;  // the inputs were counted by the applies
+                (side ? pa_g : pa_x).refs.push_back({0, ba.add(pl[0])});
+            }
+            // correlation over rows k1 (contiguous N2-element rows of T1)
+            a.logN = g.logN2;
+            a.O = 1;
+            a.J = N1;
+            a.x = t1x;
+            a.in_os = 0; a.in_js = N2; a.in_ns = 1;
+            a.tw = (const cpx<double> *)g.tw2;
+            a.twb_hi = (const cpx<double> *)g.twb_hi;
+            a.twb_lo = (const cpx<double> *)g.twb_lo;
+            a.twb_shift = g.twb_shift;
+            a.twb_sign = +1;
+            a.y = rows;
+            a.out_os = 0; a.out_js = N2; a.out_ns = 1;
+            a.alg_bytes = double(c->gen_len) * double(sizeof(cpx<T>));
+            cd.a = a;
+            cd.g = t1g;
+            cd.x_col = L;
+            cd.g_col = L;
+            cd.g_valid = INT64_MAX;
+            cd.k = k;
+            pc.refs.push_back({1, bc.add_corr(cd)});
+            // pass C of the rows (column inverse), natural order into r
+            PipeIO io;
+            memset(&io, 0, sizeof(io));
+            io.x = rows; io.ldx = L; io.y = r; io.ldy = L; io.k = 1;
+            io.n_in = L; io.m_out = L; io.D = c->D; io.work = rows;
+            PassList<T> pl;
+            fm_status st = build_pipeline<T>(g, io, &pl);
+            if (st) return st;
+            pl[2].a.alg_bytes = 0;
+            pp.refs.push_back({0, bp.add(pl[2])});
+        } else {
+            const Geometry &g = c->geo;
+            a.logN = g.logL;
+            a.O = 1;
+            a.J = 1;
+            a.x = (const cpx<T> *)xs[i];
+            a.in_os = 0; a.in_js = 0; a.in_ns = 1;
+            a.in_gn = 1; a.in_gj = 0; a.n_valid = c->cols;
+            a.tw = (const cpx<double> *)g.tw_full;
+            a.y = r;
+            a.out_os = 0; a.out_js = 0; a.out_ns = 1;
+            a.out_gn = 1; a.out_gj = 0; a.m_valid = L;
+            a.alg_bytes = double(c->gen_len) * double(sizeof(cpx<T>));
+            cd.a = a;
+            cd.g = (const cpx<T> *)gs[i];
+            cd.x_col = ldx[i];
+            cd.g_col = ldg[i];
+            cd.g_valid = c->rows;
+            cd.k = k;
+            pc.refs.push_back({1, bc.add_corr(cd)});
+        }
+        tab[i] = GradEntry<T>{r, (cpx<T> *)grads[i], L, c->head, c->tail, c->gen_len, c->kind};
+        if (c->gen_len > maxgen) maxgen = c->gen_len;
+    }
+    fm_status st = finalize<T>(ba, p->gpassA, fm_batch_plan::kSide + 1, s);
+    if (!st) st = finalize<T>(bc, p->gcorr, 1, s);
+    if (!st) st = finalize<T>(bp, p->gpassC, 1, s);
+    if (st) return st;
+    if (p->gtab) (void)hipFree(p->gtab);
+    p->gtab = nullptr;
+    FM_CHECK_HIP(hipMalloc(&p->gtab, sizeof(GradEntry<T>) * tab.size()));
+    FM_CHECK_HIP(hipMemcpyAsync(p->gtab, tab.data(), sizeof(GradEntry<T>) * tab.size(),
+                                hipMemcpyHostToDevice, s));
+    FM_CHECK_HIP(hipStreamSynchronize(s));
+    p->g_maxgen = maxgen;
+    p->g_ws = bws;
+    p->g_xs.assign(xs, xs + p->count);
+    p->g_gs.assign(gs, gs + p->count);
+    p->g_ldx.assign(ldx, ldx + p->count);
+    p->g_ldg.assign(ldg, ldg + p->count);
+    p->g_bound = true;
+    return FM_SUCCESS;
+}
+
+template <typename T>
+static fm_status batch_grad_apply_t(const fm_batch_plan *p, bool run_pass_a, hipStream_t s) {
+    fm_status st = FM_SUCCESS;
+    if (run_pass_a) st = run_program<T>(p, p->gpassA, s);
+    if (!st) st = run_program<T>(p, p->gcorr, s);
+    if (!st) st = run_program<T>(p, p->gpassC, s);
+    if (st) return st;
+    int64_t gx = ceil_div(p->g_maxgen, 256);
+    if (gx > 256) gx = 256;
+    hipLaunchKernelGGL(batch_fold_kernel<T>, dim3(unsigned(gx), unsigned(p->count)), dim3(256), 0,
+                       s, (const GradEntry<T> *)p->gtab);
+    FM_CHECK_LAUNCH();
+    return FM_SUCCESS;
+}
+}  // namespace fm
+
+extern "C" fm_status fm_batch_grad_workspace_size(const fm_batch_plan *p, int64_t *bytes) {
+    FM_REQUIRE(p && bytes, FM_ERR_VALUE, "NULL argument");
+    int64_t t = 0;
+    for (int i = 0; i < p->count; ++i) t += grad_ws_entry(p->plans[i]);
+    *bytes = t;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_grad_bind(fm_batch_plan *p, const void *const *xs,
+                                        const int64_t *ldx, const void *const *gs,
+                                        const int64_t *ldg, void *const *grads, void *workspace,
+                                        int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p && xs && ldx && gs && ldg && grads && workspace, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(p->count <= 65535, FM_ERR_VALUE, "batch gradient: more than 65535 operators");
+    int64_t need = 0;
+    fm_batch_grad_workspace_size(p, &need);
+    FM_REQUIRE(workspace_bytes >= need, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)need);
+    if (p->prec == FM_COMPLEX64)
+        return batch_grad_bind_t<float>(p, xs, ldx, gs, ldg, grads, (char *)workspace,
+                                        as_stream(stream));
+    return batch_grad_bind_t<double>(p, xs, ldx, gs, ldg, grads, (char *)workspace,
+                                     as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_grad_apply(const fm_batch_plan *p, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->g_bound && p->gtab, FM_ERR_VALUE,
+               "batch gradient used before fm_batch_grad_bind");
+    if (p->prec == FM_COMPLEX64) return batch_grad_apply_t<float>(p, true, as_stream(stream));
+    return batch_grad_apply_t<double>(p, true, as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_grad_apply_reuse(const fm_batch_plan *p, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->g_bound && p->gtab, FM_ERR_VALUE,
+               "batch gradient used before fm_batch_grad_bind");
+    const bool ok = p->dir[0].bound && p->dir[1].bound && p->ws_bound[0] == p->g_ws &&
+                    p->ws_bound[1] == p->g_ws && p->ins_bound[0] == p->g_xs &&
+                    p->ld_bound[0] == p->g_ldx && p->ins_bound[1] == p->g_gs &&
+                    p->ld_bound[1] == p->g_ldg;
+    FM_REQUIRE(ok, FM_ERR_VALUE,
+               "batch gradient reuse needs the forward direction bound to the gradient's x "
+               "blocks and the adjoint direction to its g blocks");
+    if (p->prec == FM_COMPLEX64) return batch_grad_apply_t<float>(p, false, as_stream(stream));
+    return batch_grad_apply_t<double>(p, false, as_stream(stream));
+}
+
+// ======================================================================
+//                  batched generator -> spectrum
+// ======================================================================
+// For every operator i: t_i = the embedded generator (fm_conv_plan) in the
+// batch precision, then D_i = fft(t_i)/L_i in the order its convolution
+// pipeline reads it (build_spectrum_pipeline), written into the plan's
+// spectrum buffer.  One embedding launch over all operators, then the
+// spectrum FFTs as one batched program.
+namespace fm {
+struct GenEntry {
+    const void *gen;
+    void *t;
+    int64_t L, head, tail, tail_src;
+};
+
+template <typename G, typename T>
+__global__ void batch_embed_kernel(const GenEntry *__restrict__ tab) {
+    const GenEntry e = tab[blockIdx.y];
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < e.L; i += stride) {
+        const cpx<double> v = embedded_at((const G *)e.gen, i, e.L, e.head, e.tail, e.tail_src);
+        ((cpx<T> *)e.t)[i] = mk(T(v.x), T(v.y));
+    }
+}
+
+static int64_t spec_ws_entry(const fm_conv_plan *c) {
+    const int64_t e = int64_t(esize(c->prec));
+    return up256(e * c->L) * (c->geo_fft.passes > 1 ? 2 : 1);
+}
+
+template <typename T>
+static fm_status batch_spec_bind_t(fm_batch_plan *p, const void *const *gens, char *ws,
+                                   hipStream_t s) {
+    std::vector<GenEntry> tab(p->count);
+    ProgBuilder<T> b;
+    auto &pa = b.group(0);
+    auto &pb = b.group(0);
+    int64_t off = 0, maxL = 1;
+    for (int i = 0; i < p->count; ++i) {
+        const fm_conv_plan *c = p->plans[i];
+        FM_REQUIRE(gens[i] != nullptr, FM_ERR_VALUE, "batch entry %d: NULL generator", i);
+        const int64_t L = c->L;
+        cpx<T> *t = (cpx<T> *)(ws + off);
+        cpx<T> *tmp = c->geo_fft.passes > 1
+                          ? (cpx<T> *)(ws + off + up256(int64_t(sizeof(cpx<T>)) * L))
+                          : nullptr;
+        off += spec_ws_entry(c);
+        PassList<T> pl;
+        fm_status st = build_spectrum_pipeline<T>(c->geo_fft, t, tmp, (cpx<T> *)c->D, &pl);
+        if (st) return st;
+        for (auto &ps : pl) ps.a.alg_bytes = 0;  // the generator read is the applies'
+        if (pl.size() == 1) {
+            pb.refs.push_back({0, b.add(pl[0])});
+        } else {
+            pa.refs.push_back({0, b.add(pl[0])});
+            pb.refs.push_back({0, b.add(pl[1])});
+        }
+        tab[i] = GenEntry{gens[i], t, L, c->head, c->tail, c->tail_src};
+        if (L > maxL) maxL = L;
+    }
+    fm_status st = finalize<T>(b, p->sprog, 1, s);
+    if (st) return st;
+    if (p->stab) (void)hipFree(p->stab);
+    p->stab = nullptr;
+    FM_CHECK_HIP(hipMalloc(&p->stab, sizeof(GenEntry) * tab.size()));
+    FM_CHECK_HIP(hipMemcpyAsync(p->stab, tab.data(), sizeof(GenEntry) * tab.size(),
+                                hipMemcpyHostToDevice, s));
+    FM_CHECK_HIP(hipStreamSynchronize(s));
+    p->s_maxL = maxL;
+    return FM_SUCCESS;
+}
+
+template <typename T>
+static fm_status batch_spec_apply_t(const fm_batch_plan *p, hipStream_t s) {
+    int64_t gx = ceil_div(p->s_maxL, 256);
+    if (gx > 64) gx = 64;
+    const dim3 grid(unsigned(gx), unsigned(p->count));
+    const GenEntry *tab = (const GenEntry *)p->stab;
+    switch (p->s_gen_dtype) {
+    case FM_FLOAT32:
+        hipLaunchKernelGGL((batch_embed_kernel<float, T>), grid, dim3(256), 0, s, tab);
+        break;
+    case FM_FLOAT64:
+        hipLaunchKernelGGL((batch_embed_kernel<double, T>), grid, dim3(256), 0, s, tab);
+        break;
+    case FM_COMPLEX64:
+        hipLaunchKernelGGL((batch_embed_kernel<cpx<float>, T>), grid, dim3(256), 0, s, tab);
+        break;
+    default:
+        hipLaunchKernelGGL((batch_embed_kernel<cpx<double>, T>), grid, dim3(256), 0, s, tab);
+        break;
+    }
+    FM_CHECK_LAUNCH();
+    return run_program<T>(p, p->sprog, s);
+}
+}  // namespace fm
+
+extern "C" fm_status fm_batch_gen_workspace_size(const fm_batch_plan *p, int64_t *bytes) {
+    FM_REQUIRE(p && bytes, FM_ERR_VALUE, "NULL argument");
+    int64_t t = 0;
+    for (int i = 0; i < p->count; ++i) t += spec_ws_entry(p->plans[i]);
+    *bytes = t;
+    return FM_SUCCESS;
+}
+
+extern "C" fm_status fm_batch_gen_bind(fm_batch_plan *p, int gen_dtype, const void *const *gens,
+                                       void *workspace, int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p && gens && workspace, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(p->count <= 65535, FM_ERR_VALUE, "batch generators: more than 65535 operators");
+    FM_REQUIRE(gen_dtype == FM_FLOAT32 || gen_dtype == FM_FLOAT64 || gen_dtype == FM_COMPLEX64 ||
+                   gen_dtype == FM_COMPLEX128,
+               FM_ERR_TYPE,
+               "generator element type %d: expected float32, float64, complex64 or complex128",
+               gen_dtype);
+    int64_t need = 0;
+    fm_batch_gen_workspace_size(p, &need);
+    FM_REQUIRE(workspace_bytes >= need, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)need);
+    p->s_gen_dtype = gen_dtype;
+    if (p->prec == FM_COMPLEX64)
+        return batch_spec_bind_t<float>(p, gens, (char *)workspace, as_stream(stream));
+    return batch_spec_bind_t<double>(p, gens, (char *)workspace, as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_gen_apply(const fm_batch_plan *p, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->sprog.bound && p->stab, FM_ERR_VALUE,
+               "batch generators used before fm_batch_gen_bind");
+    if (p->prec == FM_COMPLEX64) return batch_spec_apply_t<float>(p, as_stream(stream));
+    return batch_spec_apply_t<double>(p, as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_plan_destroy(fm_batch_plan *p) {
+    if (!p) return FM_SUCCESS;
+    free_program(p->dir[0]);
+    free_program(p->dir[1]);
+    free_program(p->gpassA);
+    free_program(p->gcorr);
+    free_program(p->gpassC);
+    free_program(p->sprog);
+    if (p->gtab) (void)hipFree(p->gtab);
+    if (p->stab) (void)hipFree(p->stab);
+    for (int q = 0; q < fm_batch_plan::kSide; ++q) {
+        if (p->side[q]) (void)hipStreamDestroy(p->side[q]);
+        if (p->join_ev[q]) (void)hipEventDestroy(p->join_ev[q]);
+    }
+    if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+    delete p;
+    return FM_SUCCESS;
+}

@@ -115,7 +115,10 @@ struct ProfToken {
 };
 bool prof_on();
 void prof_begin(hipStream_t s, ProfToken *t);
-void prof_end(hipStream_t s, ProfToken *t, const char *name, double bytes, double flops);
+// bytes: the launch's algorithmic bytes; pass_bytes: what the launch itself
+// reads and writes (< 0: the same as bytes)
+void prof_end(hipStream_t s, ProfToken *t, const char *name, double bytes, double flops,
+              double pass_bytes = -1.0);
 // prof_end for an elementwise kernel, labelled "<name><<element bytes>B>"
 inline void prof_elem(hipStream_t s, ProfToken *t, const char *name, size_t elem_bytes,
                       double bytes) {

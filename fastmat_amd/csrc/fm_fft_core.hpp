@@ -191,6 +191,10 @@ template <typename T> struct TileArgs {
     int seqfast_out;
     int logJ;                       // log2(J) when J is a power of two, else -1
     int64_t o_group;                // > 1: tiles walk groups of o_group o's (see tile_first_seq)
+    // host only: this pass's share of its transform's algorithmic bytes (the
+    // input and output blocks and the generator, split over the passes),
+    // for the library's profiler; 0 = count the pass's own reads and writes
+    double alg_bytes;
 };
 
 // Index arithmetic of a pass: int64_t in general, uint32_t when every
@@ -394,6 +398,91 @@ __device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile
     }
 }
 
+// Store the last stage's outputs of one tile: v[e*RL + r] holds element
+// i + r*N/RL (RL = 2^LOGRL, the radix of the stage that produced them) of
+// sequence b in mapping cur_sf; re-stage through LDS when the store wants the
+// other mapping, then apply the enclosing four-step twiddle / post-multiply /
+// conj / scale and write, cropping at m_valid.
+template <typename T, int LOGN, int NT, int LOGRL, bool I32 = false, int LE = -1>
+__device__ __forceinline__ void fft_tile_store(const TileArgs<T> &a, int64_t tile, cpx<T> *v,
+                                               cpx<T> *lds, bool cur_sf) {
+    using S = TileShape<T, LOGN, NT, LE>;
+    using Ix = IdxT<I32>;
+    constexpr int E = S::E;
+    constexpr int N = S::N;
+    constexpr int RL = 1 << LOGRL;
+    constexpr int LOGB = S::LOGB;
+    constexpr int B = S::B;
+    constexpr int SB = N + (N >> 5) + 1;
+    const int64_t s0 = tile_first_seq<T, LOGB>(a, tile);
+    constexpr int NBL = E / RL;
+    constexpr int LOGNRL = LOGN - LOGRL, NRL = 1 << LOGNRL;
+    const bool want_sf = a.seqfast_out && B > 1;
+    if (want_sf != cur_sf) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < NBL; ++e) {
+            int b, i;
+            bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, cur_sf, b, i);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) lds[b * SB + lpad(i + r * NRL)] = v[e * RL + r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < NBL; ++e) {
+            int b, i;
+            bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, want_sf, b, i);
+#pragma unroll
+            for (int r = 0; r < RL; ++r) v[e * RL + r] = lds[b * SB + lpad(i + r * NRL)];
+        }
+        cur_sf = want_sf;
+    }
+    const Ix lo_mask = (Ix(1) << a.twb_shift) - 1;
+    const Ix mv = idx_bound<Ix>(a.m_valid);
+    const Ix out_gn = Ix(a.out_gn), out_gj = Ix(a.out_gj), out_ns = Ix(a.out_ns);
+#pragma unroll
+    for (int e = 0; e < NBL; ++e) {
+        int b, i;
+        bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, cur_sf, b, i);
+        const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
+        if (!sp.ok) continue;
+        cpx<T> w[RL];
+        if (a.twb_hi || a.post) {
+            // four-step twiddle W_L^{j n} for n = i + r NRL: the FP64 base
+            // W_L^{j i} times successive FP64 powers of W_L^{j NRL} (two
+            // table lookups each, j n < L), rounded once to T
+            cpx<double> tw = mk(1.0, 0.0), step = mk(1.0, 0.0);
+            if (a.twb_hi) {
+                const Ix e0 = sp.j * Ix(i), e1 = sp.j * Ix(NRL);
+                tw = cmul(ldg(a.twb_hi, Ix(e0 >> a.twb_shift)), ldg(a.twb_lo, Ix(e0 & lo_mask)));
+                step = cmul(ldg(a.twb_hi, Ix(e1 >> a.twb_shift)), ldg(a.twb_lo, Ix(e1 & lo_mask)));
+            }
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const Ix n = Ix(i + r * NRL);
+                cpx<T> t = mk(T(1), T(0));
+                if (a.twb_hi) {
+                    t = mk(T(tw.x), T(a.twb_sign > 0 ? -tw.y : tw.y));
+                    if (r + 1 < RL) tw = cmul(tw, step);
+                }
+                const Ix gidx = n * out_gn + sp.j * out_gj;
+                if (a.post && gidx < mv) t = cmul(t, ldg(a.post, gidx));
+                w[r] = t;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            const Ix n = Ix(i + r * NRL);
+            const Ix gidx = n * out_gn + sp.j * out_gj;
+            if (gidx >= mv) continue;
+            cpx<T> u = v[e * RL + r];
+            if (a.twb_hi || a.post) u = cmul(u, w[r]);
+            if (a.conj_out) u = conj(u);
+            stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
+        }
+    }
+}
+
 // Everything after the stage-0 loads of one tile: conj / pre-multiply, the
 // transform (conv: forward, spectrum, inverse), twiddle / post-multiply and
 // the stores.  `lds` holds B * (N + N/32 + 1) elements.
@@ -488,75 +577,7 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
         if (P::S > 1) cur_sf = false;
     }
 
-    // ---- last-stage outputs: v[e*RL + r] is element i + r*N/RL of sequence
-    // b in mapping cur_sf.  Re-stage through LDS when the store wants the
-    // other mapping (transposed stores write neighbouring sequences together).
-    constexpr int NBL = E / RL;
-    constexpr int LOGNRL = LOGN - LOGRL, NRL = 1 << LOGNRL;
-    const bool want_sf = a.seqfast_out && B > 1;
-    if (want_sf != cur_sf) {
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < NBL; ++e) {
-            int b, i;
-            bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, cur_sf, b, i);
-#pragma unroll
-            for (int r = 0; r < RL; ++r) lds[b * SB + lpad(i + r * NRL)] = v[e * RL + r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < NBL; ++e) {
-            int b, i;
-            bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, want_sf, b, i);
-#pragma unroll
-            for (int r = 0; r < RL; ++r) v[e * RL + r] = lds[b * SB + lpad(i + r * NRL)];
-        }
-        cur_sf = want_sf;
-    }
-    const Ix lo_mask = (Ix(1) << a.twb_shift) - 1;
-    const Ix mv = idx_bound<Ix>(a.m_valid);
-    const Ix out_gn = Ix(a.out_gn), out_gj = Ix(a.out_gj), out_ns = Ix(a.out_ns);
-#pragma unroll
-    for (int e = 0; e < NBL; ++e) {
-        int b, i;
-        bfly_pos<LOGB, LOGNRL>(threadIdx.x + e * NT, cur_sf, b, i);
-        const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
-        if (!sp.ok) continue;
-        cpx<T> w[RL];
-        if (a.twb_hi || a.post) {
-            // four-step twiddle W_L^{j n} for n = i + r NRL: the FP64 base
-            // W_L^{j i} times successive FP64 powers of W_L^{j NRL} (two
-            // table lookups each, j n < L), rounded once to T
-            cpx<double> tw = mk(1.0, 0.0), step = mk(1.0, 0.0);
-            if (a.twb_hi) {
-                const Ix e0 = sp.j * Ix(i), e1 = sp.j * Ix(NRL);
-                tw = cmul(ldg(a.twb_hi, Ix(e0 >> a.twb_shift)), ldg(a.twb_lo, Ix(e0 & lo_mask)));
-                step = cmul(ldg(a.twb_hi, Ix(e1 >> a.twb_shift)), ldg(a.twb_lo, Ix(e1 & lo_mask)));
-            }
-#pragma unroll
-            for (int r = 0; r < RL; ++r) {
-                const Ix n = Ix(i + r * NRL);
-                cpx<T> t = mk(T(1), T(0));
-                if (a.twb_hi) {
-                    t = mk(T(tw.x), T(a.twb_sign > 0 ? -tw.y : tw.y));
-                    if (r + 1 < RL) tw = cmul(tw, step);
-                }
-                const Ix gidx = n * out_gn + sp.j * out_gj;
-                if (a.post && gidx < mv) t = cmul(t, ldg(a.post, gidx));
-                w[r] = t;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RL; ++r) {
-            const Ix n = Ix(i + r * NRL);
-            const Ix gidx = n * out_gn + sp.j * out_gj;
-            if (gidx >= mv) continue;
-            cpx<T> u = v[e * RL + r];
-            if (a.twb_hi || a.post) u = cmul(u, w[r]);
-            if (a.conj_out) u = conj(u);
-            stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
-        }
-    }
+    fft_tile_store<T, LOGN, NT, LOGRL, I32, LE>(a, tile, v, lds, cur_sf);
 }
 
 template <typename T, int LOGN, int NT, bool CONV, bool I32 = false, int LE = -1>

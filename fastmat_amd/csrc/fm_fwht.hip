@@ -59,6 +59,7 @@ struct WhtArgs {
     int seqfast;       // neighbouring vectors are adjacent in memory (J > 1)
     const void *x;
     void *y;
+    int npass;         // passes of the whole transform (profiler attribution)
 };
 
 // Stage plan: S stages of at most LOGE bits, lowest bits first (the
@@ -290,7 +291,9 @@ static fm_status wht_launch_nt(WhtArgs a, hipStream_t stream) {
         const double elems = double(a.O) * double(a.J) * double(N);
         char name[64];
         snprintf(name, sizeof(name), "wht_pass_kernel<%dB,%d>", int(sizeof(S)), LOGN);
-        prof_end(stream, &tok, name, 2.0 * elems * sizeof(S), elems * LOGN);
+        // algorithmic: the transform's input and output, split over its passes
+        const double own = 2.0 * elems * sizeof(S);
+        prof_end(stream, &tok, name, own / (a.npass > 0 ? a.npass : 1), elems * LOGN, own);
     }
     return FM_SUCCESS;
 }
@@ -357,6 +360,7 @@ static fm_status wht_run(int order, const void *x, int64_t ldx, void *y, int64_t
         a.seqfast = Jlow > 1 ? 1 : 0;
         a.x = p == 0 ? x : y;
         a.y = y;
+        a.npass = npass;
         fm_status st = wht_launch<S>(a, stream);
         if (st) return st;
         lo = hi;

@@ -17,14 +17,14 @@ namespace {
 struct Rec {
     std::string name;
     hipEvent_t a, b;
-    double bytes, flops;
+    double bytes, flops, pass_bytes;
 };
 std::atomic<bool> g_on{false};
 std::mutex g_mu;
 std::vector<Rec> g_recs;
 struct Agg {
     int64_t launches = 0;
-    double ms = 0, bytes = 0, flops = 0;
+    double ms = 0, bytes = 0, flops = 0, pass_bytes = 0;
 };
 std::vector<std::pair<std::string, Agg>> g_summary;
 }  // namespace
@@ -47,11 +47,12 @@ void prof_begin(hipStream_t s, ProfToken *t) {
     t->active = true;
 }
 
-void prof_end(hipStream_t s, ProfToken *t, const char *name, double bytes, double flops) {
+void prof_end(hipStream_t s, ProfToken *t, const char *name, double bytes, double flops,
+              double pass_bytes) {
     if (!t->active) return;
     (void)hipEventRecord(t->b, s);
     std::lock_guard<std::mutex> lock(g_mu);
-    g_recs.push_back(Rec{name, t->a, t->b, bytes, flops});
+    g_recs.push_back(Rec{name, t->a, t->b, bytes, flops, pass_bytes < 0 ? bytes : pass_bytes});
 }
 
 static void clear_locked() {
@@ -91,6 +92,7 @@ extern "C" fm_status fm_profiler_count(int *count) {
         g.ms += ms;
         g.bytes += r.bytes;
         g.flops += r.flops;
+        g.pass_bytes += r.pass_bytes;
     }
     g_summary.assign(agg.begin(), agg.end());
     *count = int(g_summary.size());
@@ -99,7 +101,7 @@ extern "C" fm_status fm_profiler_count(int *count) {
 
 extern "C" fm_status fm_profiler_entry(int index, char *name, int name_len, int64_t *launches,
                                        double *total_ms, double *total_bytes,
-                                       double *total_flops) {
+                                       double *total_flops, double *total_pass_bytes) {
     std::lock_guard<std::mutex> lock(g_mu);
     FM_REQUIRE(index >= 0 && index < int(g_summary.size()), FM_ERR_VALUE,
                "profiler entry %d out of range (call fm_profiler_count first)", index);
@@ -112,5 +114,6 @@ extern "C" fm_status fm_profiler_entry(int index, char *name, int name_len, int6
     if (total_ms) *total_ms = e.second.ms;
     if (total_bytes) *total_bytes = e.second.bytes;
     if (total_flops) *total_flops = e.second.flops;
+    if (total_pass_bytes) *total_pass_bytes = e.second.pass_bytes;
     return FM_SUCCESS;
 }

@@ -161,25 +161,53 @@ fm_status fm_conv_generator_grad(const fm_conv_plan *plan, const void *x,
                                  int64_t ldx, const void *g, int64_t ldg,
                                  int64_t k, void *grad, void *workspace,
                                  int64_t workspace_bytes, void *stream);
+/* The spectrum as a function of a generator held on the device (pattern (a):
+ * the operator *is* its generator's spectrum, Circulant.pyx:130
+ * `Diag(fft(tenC)/size)`, Toeplitz.pyx:253-255 `fft(tenThat)`).  Rewrites the
+ * plan's spectrum with fft(t')/L of `generator` (device pointer, gen_len
+ * elements of gen_dtype: FLOAT32, FLOAT64, COMPLEX64 or COMPLEX128; c for a
+ * Circulant, [vecC, vecR] for a Toeplitz), embedded as at plan creation and
+ * transformed in the plan's precision (FP32 butterflies with FP64-generated
+ * twiddles for complex64).  Stream-ordered: applies issued after it on the
+ * same stream use the new spectrum.  Allocates nothing (graph-capturable);
+ * the workspace size comes from fm_conv_set_generator_workspace_size.  Not
+ * for two-level plans (FM_ERR_NOT_IMPLEMENTED).  A plan whose spectrum is
+ * rewritten is no longer immutable: share it only with callers that expect
+ * that, or work on a copy from fm_conv_plan_clone. */
+fm_status fm_conv_set_generator_workspace_size(const fm_conv_plan *plan, int64_t *bytes);
+fm_status fm_conv_set_generator(fm_conv_plan *plan, int gen_dtype, const void *generator,
+                                void *workspace, int64_t workspace_bytes, void *stream);
+/* A copy of a plan with its own spectrum buffer (synchronises `stream`). */
+fm_status fm_conv_plan_clone(const fm_conv_plan *plan, void *stream, fm_conv_plan **copy);
 fm_status fm_conv_plan_destroy(fm_conv_plan *plan);
 
 /* ------------------------------------------------- uneven operator batch
  * Many Circulant/Toeplitz plans of one precision, each with its own k_i
  * right-hand sides, applied together.  No single reference call does this;
  * it replaces the per-operator loop of forward()/backward() calls fastmat
- * users write (Matrix.pyx:1844-2007 once per operator).  Every operator's
- * passes are grouped by phase and by transform length; each group is one
- * persistent launch over a device work table (at most a dozen or so launches
- * per direction, none depending on another inside a phase), so the apply is
- * graph-capturable.  fm_batch_bind() binds one direction to buffers: input i
- * is cols_i x k_i (rows_i x k_i for the adjoint), output the other shape, and
- * uploads the work tables (synchronising `stream`); fm_batch_apply() then
- * only launches.  Plans must outlive the batch plan. */
+ * users write (Matrix.pyx:1844-2007 once per operator).  A direction is a
+ * program of persistent launches, one per (pass, transform length) class,
+ * over device work tables, issued on the caller's stream and on side streams
+ * forked from it and joined back (graph-capturable).  The multi-pass
+ * operators run in groups whose four-step intermediates fit a budget, so
+ * each group's passes find their input in the memory-side cache
+ * (fm_batch_set_schedule).  fm_batch_bind() binds one direction to buffers:
+ * input i is cols_i x k_i (rows_i x k_i for the adjoint), output the other
+ * shape, and uploads the work tables (synchronising `stream`);
+ * fm_batch_apply() then only launches.  The workspace holds, per direction
+ * and multi-pass operator, the pass-A and pass-B intermediates.  Plans must
+ * outlive the batch plan. */
 typedef struct fm_batch_plan fm_batch_plan;
 fm_status fm_batch_plan_create(int count, const fm_conv_plan *const *plans,
                                const int64_t *ks, fm_batch_plan **plan);
 fm_status fm_batch_workspace_size(const fm_batch_plan *plan, int64_t *bytes);
 fm_status fm_batch_set_grid(fm_batch_plan *plan, int workgroups_per_cu);
+/* Schedule of the directions: persistent grid (workgroups per CU), the
+ * budget of one group's intermediates in MB (0: one group) and the number of
+ * lanes (streams, 1..4) the groups alternate between.  A change of the
+ * grouping unbinds both directions (bind them again). */
+fm_status fm_batch_set_schedule(fm_batch_plan *plan, int workgroups_per_cu, int group_mb,
+                                int lanes);
 fm_status fm_batch_bind(fm_batch_plan *plan, int adjoint, const void *const *inputs,
                         const int64_t *ld_in, void *const *outputs,
                         const int64_t *ld_out, void *workspace,
@@ -190,23 +218,47 @@ fm_status fm_batch_apply(const fm_batch_plan *plan, int adjoint, void *stream);
  * dimension ldx[i]) and g_i the gradient of the loss with respect to its
  * output (k_i x rows_i, ldg[i]), grads[i] (gen_len_i elements, the plan's
  * generator layout) receives what fm_conv_generator_grad returns for that
- * operator alone.  fm_batch_grad_bind builds the program for one set of
- * device pointers (allocates its tables, synchronises its stream);
- * fm_batch_grad_apply only launches kernels (capturable).  The workspace
- * holds the operators' spectra and intermediates. */
+ * operator alone.  Per operator: the column transforms (pass A) of x and g,
+ * one correlation launch that sums FFT(g rows) conj(FFT(x rows)) over the
+ * columns and transforms back, the column inverse, the fold.  The pass-A
+ * results live in the directions' workspace, so both directions must have
+ * been bound (fm_batch_bind) first.  fm_batch_grad_bind builds the program
+ * for one set of device pointers (allocates its tables, synchronises its
+ * stream); fm_batch_grad_apply only launches (capturable).
+ * fm_batch_grad_apply_reuse skips the pass-A launches and reads the forward
+ * direction's and the adjoint direction's pass-A intermediates instead: the
+ * forward direction must be bound to the same x blocks and the adjoint to
+ * the same g blocks (checked), and both applied since those intermediates
+ * were last overwritten (the caller's ordering, as in a training step:
+ * forward, then adjoint, then this). */
 fm_status fm_batch_grad_workspace_size(const fm_batch_plan *plan, int64_t *bytes);
 fm_status fm_batch_grad_bind(fm_batch_plan *plan, const void *const *xs, const int64_t *ldx,
                              const void *const *gs, const int64_t *ldg, void *const *grads,
                              void *workspace, int64_t workspace_bytes, void *stream);
 fm_status fm_batch_grad_apply(const fm_batch_plan *plan, void *stream);
+fm_status fm_batch_grad_apply_reuse(const fm_batch_plan *plan, void *stream);
+/* Batched fm_conv_set_generator: every operator's spectrum from its device
+ * generator gens[i] (element type gen_dtype for all), as one embedding launch
+ * and one batched FFT program.  fm_batch_gen_bind builds the program for one
+ * set of generator pointers (allocates its tables, synchronises `stream`);
+ * fm_batch_gen_apply only launches (capturable), rewriting the spectra of the
+ * plans the batch was created from. */
+fm_status fm_batch_gen_workspace_size(const fm_batch_plan *plan, int64_t *bytes);
+fm_status fm_batch_gen_bind(fm_batch_plan *plan, int gen_dtype, const void *const *gens,
+                            void *workspace, int64_t workspace_bytes, void *stream);
+fm_status fm_batch_gen_apply(const fm_batch_plan *plan, void *stream);
 fm_status fm_batch_plan_destroy(fm_batch_plan *plan);
 
 /* --------------------------------------------------------------- dense
  * Y = A X (adjoint: Y = A^H X) for a dense m x n matrix A stored row-major
  * (A[i*lda + j]); replaces Matrix._forward / _backward (Matrix.pyx:1831-1842,
  * 1924-1935, ndarray.dot).  Supported (a_dtype, x_dtype) pairs:
- * (FLOAT32, FLOAT32), (FLOAT32, COMPLEX64), (COMPLEX64, COMPLEX64); Y has the
- * promoted type.  Real-A paths run on FP32 MFMA with FP32 accumulation. */
+ * (FLOAT32, FLOAT32), (FLOAT32, COMPLEX64) on FP32 MFMA with FP32
+ * accumulation (a complex64 matrix goes as its real 2m x 2n form), and
+ * (T, T) for T in INT8, INT16, INT32, INT64, FLOAT64, COMPLEX128: integers
+ * accumulate modulo 2^64 and are truncated to their width (the residue
+ * numpy's integer dot produces, overflow included), float64 / complex128
+ * accumulate in FP64.  Y has the promoted type. */
 fm_status fm_dense_apply(int a_dtype, int x_dtype, int adjoint, int64_t m,
                          int64_t n, const void *A, int64_t lda, const void *x,
                          int64_t ldx, void *y, int64_t ldy, int64_t k,
@@ -337,14 +389,19 @@ fm_status fm_stela_select(int dtype_z, int64_t n, int64_t k, void *z, const void
 /* ------------------------------------------------------------ profiler
  * No counterpart in the reference.  For benchmarks: when enabled, each of the
  * library's kernel launches is bracketed by HIP events on its own stream and
- * tagged with its algorithmic bytes (compulsory reads + writes) and flops.
+ * tagged with its algorithmic bytes (its share of the transforms' input and
+ * output blocks and generators: an operator's (n + m) k elements plus its
+ * generator, split evenly over the operator's passes), the bytes the launch
+ * itself reads and writes (intermediates included) and its flops.  Launches
+ * on a stream being captured into a graph are not timed.
  * fm_profiler_count() synchronises on the recorded events and aggregates per
  * kernel label; fm_profiler_entry() reads one aggregate. */
 fm_status fm_profiler_enable(int on);
 fm_status fm_profiler_reset(void);
 fm_status fm_profiler_count(int *count);
 fm_status fm_profiler_entry(int index, char *name, int name_len, int64_t *launches,
-                            double *total_ms, double *total_bytes, double *total_flops);
+                            double *total_ms, double *total_bytes, double *total_flops,
+                            double *total_pass_bytes);
 
 #ifdef __cplusplus
 }

@@ -122,6 +122,29 @@ def toeplitz(tenT, m, x, adjoint=False, L=None):
 
 
 # -------------------------------------------------------------- Hadamard
+def conv_generator_grad(kind, m, n, x, g):
+    """Gradient of a real loss with respect to the generator of a Circulant
+    (c, length n) or Toeplitz (tenT = [vecC (m), vecR (n-1)]) matrix, given
+    the forward input x (n, k) and the output gradient g = dLoss/dy (m, k),
+    in PyTorch's convention for complex tensors (grad = dL/dRe + i dL/dIm):
+
+        grad[q] = sum_cols sum_{(i, j): idx(i, j) = q} g[i] conj(x[j])
+
+    with idx(i, j) = (i - j) mod n for a Circulant (C[i, j] = c[(i - j) mod N],
+    Circulant.pyx:418-423) and (i - j) mod (m + n - 1) for a Toeplitz
+    (T[i, j] = tenT[(i - j) mod (m + n - 1)], Toeplitz.pyx:719-732).  Computed
+    as FP64 FFT correlations of length n (Circulant) or m + n - 1 (Toeplitz,
+    g and x zero-padded), i.e. without any power-of-two embedding."""
+    x2, _ = _as2d(x)
+    g2, _ = _as2d(g)
+    x2 = x2.astype(np.complex128)
+    g2 = g2.astype(np.complex128)
+    d = n if kind == "circulant" else m + n - 1
+    X = np.fft.fft(x2, n=d, axis=0)
+    G = np.fft.fft(g2, n=d, axis=0)
+    return np.fft.ifft(np.sum(G * np.conj(X), axis=1))
+
+
 def _wrap_dtype(dtype):
     return np.dtype(dtype)
 

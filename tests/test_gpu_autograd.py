@@ -1,0 +1,231 @@
+"""GPU tests of the differentiable structured matvecs (pattern (a)): the
+forward of ``conv_matvec`` / ``batch_conv_matvec_packed`` is the operator of
+the generator tensor's *current* values, recomputed on the device every call
+(the operator is its generator's spectrum, Circulant.pyx:130,
+Toeplitz.pyx:253-255), and the generator gradient is checked against FP64
+central differences of the loss computed by the *device* forward in
+complex128, and against the FP64 closed-form correlation of
+oracle.fast64.conv_generator_grad.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import accept, fast64
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def crandn(rng, *shape, dtype=np.complex64):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    from fastmat_amd import _lib
+    _lib.load()
+    return torch
+
+
+def oracle_apply(kind, m, gen, x, adjoint=False):
+    if kind == "circulant":
+        return fast64.circulant(gen, x, adjoint=adjoint)
+    return fast64.toeplitz(gen, m, x, adjoint=adjoint)
+
+
+def make_op(kind, m, gen):
+    import fastmat_amd as fm
+    return fm.Circulant(gen) if kind == "circulant" else fm.Toeplitz(gen[:m], gen[m:])
+
+
+CASES = [("circulant", 300, 300), ("circulant", 5000, 5000), ("toeplitz", 200, 350),
+         ("toeplitz", 3000, 5000)]
+
+
+@pytest.mark.parametrize("kind,m,n", CASES)
+def test_forward_follows_generator_updates(torch_mod, kind, m, n):
+    """Updating the generator tensor in place (as an optimizer step does)
+    changes the next forward to the new operator; the matrix object keeps
+    its constructor's operator."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    rng = np.random.default_rng(m * 7 + n)
+    d = n if kind == "circulant" else m + n - 1
+    g0 = crandn(rng, d)
+    delta = crandn(rng, d)
+    x = crandn(rng, n, 3)
+    A = make_op(kind, m, g0)
+    gen = torch.from_numpy(g0.copy()).cuda().requires_grad_(True)
+    xt = torch.from_numpy(x).cuda()
+    say("generator update", kind, m, n)
+    y1 = fa.conv_matvec(A, gen, xt).detach().cpu().numpy()
+    assert accept.max_rel_error(y1, oracle_apply(kind, m, g0, x)) < 1e-5
+    with torch.no_grad():
+        gen.add_(torch.from_numpy(delta).cuda())
+    y2 = fa.conv_matvec(A, gen, xt).detach().cpu().numpy()
+    g1 = (g0.astype(np.complex128) + delta.astype(np.complex128))
+    assert accept.max_rel_error(y2, oracle_apply(kind, m, g1, x)) < 1e-5
+    # the matrix object is unchanged
+    assert accept.max_rel_error(A.forward(x), oracle_apply(kind, m, g0, x)) < 1e-5
+    # the gradient after the update belongs to the updated operator
+    y = crandn(rng, m, 3)
+    out = fa.conv_matvec(A, gen, xt)
+    ((out - torch.from_numpy(y).cuda()).abs() ** 2).sum().backward()
+    r = oracle_apply(kind, m, g1, x.astype(np.complex128)) - y
+    ref = fast64.conv_generator_grad(kind, m, n, x, 2 * r)
+    assert accept.max_rel_error(gen.grad.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("kind,m,n", CASES)
+def test_generator_grad_vs_device_finite_differences(torch_mod, kind, m, n):
+    """complex128: the autograd generator gradient against FP64 central
+    differences (step 1e-6) of the loss sum |A(gen) x - y|^2 evaluated by two
+    device forwards per coordinate (the loss is quadratic in gen, so central
+    differences are exact up to rounding)."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    rng = np.random.default_rng(m + 3 * n)
+    d = n if kind == "circulant" else m + n - 1
+    g0 = crandn(rng, d, dtype=np.complex128)
+    x = crandn(rng, n, 2, dtype=np.complex128)
+    y = crandn(rng, m, 2, dtype=np.complex128)
+    A = make_op(kind, m, g0)
+    gen = torch.from_numpy(g0.copy()).cuda().requires_grad_(True)
+    xt = torch.from_numpy(x).cuda().requires_grad_(True)
+    yt = torch.from_numpy(y).cuda()
+    say("device FD", kind, m, n)
+    loss = ((fa.conv_matvec(A, gen, xt) - yt).abs() ** 2).sum()
+    loss.backward()
+    gg = gen.grad.cpu().numpy()
+
+    def dev_loss(g):
+        with torch.no_grad():
+            return float(((fa.conv_matvec(A, g, xt) - yt).abs() ** 2).sum())
+
+    h = 1e-6
+    idx = sorted(set([0, 1, d // 2, d - 1] + list(rng.integers(0, d, 4))))
+    fd = []
+    base = gen.detach()
+    for q in idx:
+        vals = []
+        for unit in (1.0, 1j):
+            e = torch.zeros_like(base)
+            e[q] = h * unit
+            vals.append((dev_loss(base + e) - dev_loss(base - e)) / (2 * h))
+        fd.append(vals[0] + 1j * vals[1])
+    fd = np.array(fd)
+    err = np.max(np.abs(gg[idx] - fd)) / np.max(np.abs(fd))
+    say("  generator grad vs device FD", err)
+    assert err < 1e-6
+    # and the whole gradient against the FP64 closed form
+    r = oracle_apply(kind, m, g0, x) - y
+    ref = fast64.conv_generator_grad(kind, m, n, x, 2 * r)
+    assert accept.max_rel_error(gg, ref) < 1e-12
+    assert accept.max_rel_error(xt.grad.cpu().numpy(),
+                                2 * oracle_apply(kind, m, g0, r, adjoint=True)) < 1e-12
+
+
+def test_backward_after_an_intervening_forward(torch_mod):
+    """Two forwards of one matrix with different generators, then one
+    backward: each output's gradient uses its own generator."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    rng = np.random.default_rng(31)
+    n = 700
+    ga, gb = crandn(rng, n), crandn(rng, n)
+    x = crandn(rng, n, 2)
+    A = make_op("circulant", n, ga)
+    ta = torch.from_numpy(ga).cuda().requires_grad_(True)
+    tb = torch.from_numpy(gb).cuda().requires_grad_(True)
+    xt = torch.from_numpy(x).cuda().requires_grad_(True)
+    say("two forwards, one backward")
+    ya = fa.conv_matvec(A, ta, xt)
+    yb = fa.conv_matvec(A, tb, xt)
+    (ya.abs().pow(2).sum() + 2 * yb.abs().pow(2).sum()).backward()
+    x64 = x.astype(np.complex128)
+    ra = oracle_apply("circulant", n, ga, x64)
+    rb = oracle_apply("circulant", n, gb, x64)
+    ref_x = 2 * oracle_apply("circulant", n, ga, ra, True) + \
+        4 * oracle_apply("circulant", n, gb, rb, True)
+    assert accept.max_rel_error(xt.grad.cpu().numpy(), ref_x) < 1e-5
+    assert accept.max_rel_error(ta.grad.cpu().numpy(),
+                                fast64.conv_generator_grad("circulant", n, n, x, 2 * ra)) < 1e-5
+    assert accept.max_rel_error(tb.grad.cpu().numpy(),
+                                fast64.conv_generator_grad("circulant", n, n, x, 4 * rb)) < 1e-5
+
+
+def test_packed_batch_follows_generators_and_fd_complex128(torch_mod):
+    """The packed batch Function in complex128 on a mixed batch (one- and
+    multi-pass Circulant / Toeplitz): outputs of the generators' current
+    values after an in-place update, every generator gradient against the
+    FP64 closed form, and sampled coordinates against central differences of
+    the device loss."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(41)
+    spec = [("circulant", 300, 300, 2), ("toeplitz", 200, 350, 1),
+            ("circulant", 5000, 5000, 2), ("toeplitz", 3000, 5000, 3),
+            ("circulant", 64, 64, 1)]
+    gens0 = [crandn(rng, n if k_ == "circulant" else m + n - 1, dtype=np.complex128)
+             for k_, m, n, _ in spec]
+    ops = [make_op(k_, m, g) for (k_, m, n, _), g in zip(spec, gens0)]
+    ks = [e[3] for e in spec]
+    batch = ConvBatch(ops, ks, dtype=np.complex128)
+    n_in, n_out = batch.packed_sizes()
+    gens = [torch.from_numpy(g.copy()).cuda().requires_grad_(True) for g in gens0]
+    xp = torch.from_numpy(crandn(rng, n_in, dtype=np.complex128)).cuda().requires_grad_(True)
+    yp = torch.from_numpy(crandn(rng, n_out, dtype=np.complex128)).cuda()
+    deltas = [crandn(rng, g.size, dtype=np.complex128) for g in gens0]
+    with torch.no_grad():
+        for g, dl in zip(gens, deltas):
+            g.add_(torch.from_numpy(dl).cuda())
+    cur = [g0 + dl for g0, dl in zip(gens0, deltas)]
+    say("packed batch complex128, updated generators")
+    out = fa.batch_conv_matvec_packed(batch, gens, xp)
+    ((out - yp).abs() ** 2).sum().backward()
+    xv = batch.packed_views(xp.detach(), inputs=True)
+    yv = batch.packed_views(yp, inputs=False)
+    ov = batch.packed_views(out.detach(), inputs=False)
+    for i, ((k_, m, n, k), g) in enumerate(zip(spec, cur)):
+        x = xv[i].cpu().numpy().T
+        y = yv[i].cpu().numpy().T
+        ref = oracle_apply(k_, m, g, x)
+        assert accept.max_rel_error(ov[i].cpu().numpy().T, ref) < 1e-12, i
+        gref = fast64.conv_generator_grad(k_, m, n, x, 2 * (ref - y))
+        assert accept.max_rel_error(gens[i].grad.cpu().numpy(), gref) < 1e-12, i
+    # central differences (step 1e-6) of operator i's own loss term
+    # |A_i x_i - y_i|^2, each evaluated by a device forward of the batch; the
+    # loss is quadratic in the generator, so the differences carry rounding
+    # only, about eps * loss / h, which sets the bound
+    base = [g.detach().clone() for g in gens]
+
+    def dev_loss(gl, i):
+        with torch.no_grad():
+            o = fa.batch_conv_matvec_packed(batch, gl, xp.detach())
+            oi = batch.packed_views(o, inputs=False)[i]
+            return float(((oi - yv[i]).abs() ** 2).sum())
+
+    h = 1e-6
+    for i in (0, 3):
+        q = int(rng.integers(0, base[i].numel()))
+        fd = 0j
+        for unit in (1.0, 1j):
+            e = torch.zeros_like(base[i])
+            e[q] = h * unit
+            gp = list(base)
+            gm = list(base)
+            gp[i] = base[i] + e
+            gm[i] = base[i] - e
+            fd += unit * (dev_loss(gp, i) - dev_loss(gm, i)) / (2 * h)
+        got = complex(gens[i].grad[q])
+        bound = 1e-7 * abs(got) + 50 * 2.2e-16 * dev_loss(base, i) / h
+        say("  batch FD op", i, "coord", q, "abs err", abs(got - fd), "bound", bound)
+        assert abs(got - fd) < bound

@@ -651,3 +651,48 @@ def test_omp_least_squares_and_orthogonal_residual():
             np.testing.assert_allclose(y[S, l], xs, rtol=1e-10, atol=1e-12)
             r = b[:, l] - A @ y[:, l]
             assert np.max(np.abs(A[:, S].conj().T @ r)) < 1e-10 * np.linalg.norm(b[:, l])
+
+
+@pytest.mark.parametrize("kind,m,n", [("circulant", 7, 7), ("circulant", 16, 16),
+                                      ("toeplitz", 4, 5), ("toeplitz", 6, 3),
+                                      ("toeplitz", 1, 4)])
+def test_conv_generator_grad_brute_force_and_fd(kind, m, n):
+    """fast64.conv_generator_grad against (1) the brute-force sum over the
+    matrix entries that hold each generator element (index maps of
+    Circulant.pyx:418-423 / Toeplitz.pyx:719-732, read off the dense oracle
+    matrices) and (2) FP64 central differences of the loss
+    sum |A(gen) x - y|^2 (PyTorch convention dL/dRe + i dL/dIm)."""
+    rng = np.random.default_rng(hash((kind, m, n)) % 2**32)
+    k = 3
+    d = n if kind == "circulant" else m + n - 1
+    gen = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    x = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    y = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
+
+    def mat(t):
+        if kind == "circulant":
+            return dense.circulant(t)
+        return dense.toeplitz(t[:m], t[m:])
+
+    # which generator element sits at (i, j): dense matrix of indices
+    idx = np.rint(mat(np.arange(d, dtype=np.float64)).real).astype(int)
+    g = 2 * (mat(gen) @ x - y)
+    brute = np.zeros(d, dtype=np.complex128)
+    for i in range(m):
+        for j in range(n):
+            brute[idx[i, j]] += np.sum(g[i] * np.conj(x[j]))
+    got = fast64.conv_generator_grad(kind, m, n, x, g)
+    np.testing.assert_allclose(got, brute, rtol=0, atol=1e-12 * np.abs(brute).max())
+
+    def loss(t):
+        return float(np.sum(np.abs(mat(t) @ x - y) ** 2))
+
+    h = 1e-6
+    fd = np.zeros(d, dtype=np.complex128)
+    for q in range(d):
+        for unit in (1.0, 1j):
+            tp, tm = gen.copy(), gen.copy()
+            tp[q] += h * unit
+            tm[q] -= h * unit
+            fd[q] += unit * (loss(tp) - loss(tm)) / (2 * h)
+    np.testing.assert_allclose(got, fd, rtol=0, atol=1e-6 * np.abs(fd).max())
