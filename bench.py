@@ -1,25 +1,34 @@
 #!/usr/bin/env python
 """Benchmark of the structured-matvec hot path on one MI355X.
 
-Default workload (the configuration the headline metric is quoted on):
-Fourier in complex64 at N = 2^20 (radix path) and N = 1,000,003 (prime,
-Bluestein path), each with X of 64 columns iid complex N(0, 1).  One step is
-the forward and the adjoint transform of both operators on resident device
-data.  ``value`` is whole-job throughput in GB/s over algorithmic bytes
-(each transform reads X once and writes Y once: 2 * N * k * 8 bytes).
+Default workload (the configuration the headline metric, batched forward +
+adjoint throughput against the plain per-operator GPU path, is quoted on):
+the uneven batch of 256 Circulant / Toeplitz operators in complex64 (sizes
+log-uniform in [256, 65536], 1-32 columns each) through the autograd
+Function: one step computes every operator from its generator tensor on the
+device, the forward, loss = sum_i ||A_i x_i - y_i||^2, and the gradients with
+respect to every x_i (the adjoint) and every generator.  The step is one HIP
+graph replay, inputs resident in HBM.  ``value`` is algorithmic bytes per step
+over the step time: per operator the forward (n + m) k elements plus the
+generator, the adjoint the same, plus the generator-gradient write.
 
-Other workloads (``--config``): hadamard (order 20, 64 columns, int32 and
-float32), circulant (N=4096, 32 columns, complex128), kron (Fourier(64) x
-Circulant(128) x dense 128x128, 16 columns, complex64), batch (256 uneven
-Circulant/Toeplitz operators), multilevel (two-level Circulant 1024 x 1024,
-16 columns), fista (50 FISTA iterations on a Circulant of 2^20, 8 columns),
+Other workloads (``--config``): batch (the same batch, forward + adjoint
+only), fourier (N = 2^20 and N = 1,000,003 in complex64, 64 columns),
+hadamard (order 20, 64 columns, int32 and float32), circulant (N=4096, 32
+columns, complex128), kron (Fourier(64) x Circulant(128) x dense 128x128, 16
+columns, complex64), multilevel (two-level Circulant 1024 x 1024, 16
+columns), fista (50 FISTA iterations on a Circulant of 2^20, 8 columns),
 blockdiag (Circulant, Circulant, Fourier, Diag blocks of 2^18, 32 columns),
-lfsr (LFSRCirculant of order 20, 64 columns float32).
+lfsr (LFSRCirculant of order 20, 64 columns float32), omp.
 
 The JSON line also carries
-  roofline      dominant kernel: algorithmic bytes per launch / average launch
-                duration, both from HIP events recorded by the library around
-                its own launches during the timed region;
+  roofline      dominant kernel (most time): its algorithmic bytes per launch
+                (each operator's input, output and generator bytes split over
+                its passes) / its average launch duration, from HIP events the
+                library records around its own launches on their streams
+                (for graph-replayed workloads: the same step issued eagerly,
+                same schedule); pass_bytes_per_launch is what the launch
+                itself reads and writes, intermediates included;
   cpu_baseline  the restated CPU path of the reference (oracle.cpu_path, numpy
                 pocketfft, single core) timed on a bounded sample on this host.
 """
@@ -45,7 +54,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--config", default="fourier",
+    p.add_argument("--config", default="batch_grad",
                    choices=["fourier", "hadamard", "circulant", "kron", "batch", "copy",
                             "multilevel", "fista", "blockdiag", "lfsr", "batch_grad",
                             "omp"])
@@ -241,6 +250,14 @@ def workload_kron(args, rng):
                 dtype="c64", metric=METRIC)
 
 
+def _tuner_times(batch):
+    """Forward + adjoint time of every schedule the autotuner tried
+    (workgroups per CU / group MB / lanes -> ms)."""
+    from fastmat_amd.batch import default_tuner
+    rec = default_tuner().cache.get(batch.signature + "-schedule", {})
+    return {k.replace(" ", ""): round(v, 4) for k, v in rec.get("ms", {}).items()}
+
+
 def make_uneven_batch(seed=20261202 + 4, count=256, lo=256, hi=65536, kmax=32):
     """The uneven batch: `count` operators, half Circulant(N), half
     Toeplitz(m x n); N and m log-uniform in [lo, hi], n = round(m / r) with
@@ -285,8 +302,8 @@ def workload_batch(args, rng):
     gs = [device_block(crandn(rng, (op.numRows, k)), torch.complex64) for op, k in zip(ops, ks)]
     ys = batch.new_outputs(False)
     zs = batch.new_outputs(True)
-    grid_f = batch.tune(xs, ys, False)
-    grid_b = batch.tune(gs, zs, True)
+    sched = batch.tune_schedule(xs, ys, gs, zs)
+    tuned = _tuner_times(batch)
     gf = batch.graph(xs, ys, False)
     gb = batch.graph(gs, zs, True)
     bytes_step = batch.algorithmic_bytes(False) + batch.algorithmic_bytes(True)
@@ -334,7 +351,9 @@ def workload_batch(args, rng):
     n_multi = sum(1 for op in ops if op.fftLength > 8192)
     cfg = {"workload": "uneven_batch_256_circulant_toeplitz_c64_fwd_adj", "operators": len(ops),
            "multi_pass_operators": n_multi, "total_columns": int(sum(ks)),
-           "grid_per_cu": [grid_f, grid_b], "hipgraph": True}
+           "schedule": {"workgroups_per_cu": sched[0], "group_mb": sched[1],
+                        "lanes": sched[2]}, "schedule_candidates_ms": tuned,
+           "hipgraph": True}
     return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check, plain=plain(),
                 per_op=per_op(), cpu=None, cfg=cfg, dtype="c64",
                 metric=METRIC)
@@ -545,17 +564,37 @@ def workload_lfsr(args, rng):
                 dtype="f32", metric=METRIC)
 
 
+def batch_step_bytes(ops, ks, esz=8):
+    """Algorithmic bytes of one autograd step of the uneven batch: per
+    operator the forward (n + m) k elements plus the generator, the adjoint
+    the same, and the generator-gradient write (gen elements)."""
+    tot = 0
+    for op, k in zip(ops, ks):
+        gen = op.numRows + op.numCols - 1 if type(op).__name__ == "Toeplitz" else op.numCols
+        tot += 2 * ((op.numRows + op.numCols) * k + gen) * esz + gen * esz
+    return tot
+
+
 def workload_batch_grad(args, rng):
-    """The uneven batch through the autograd Function: loss = sum_i
-    ||A_i x_i - y_i||^2, then its gradients with respect to every x_i and
-    every generator (the batch's forward, the batch's adjoint and one
-    generator-gradient FFT correlation per operator)."""
+    """The uneven batch of 256 Circulant / Toeplitz operators through the
+    autograd Function (the headline workload): loss = sum_i ||A_i x_i - y_i||^2
+    with every operator computed from its generator tensor on the device, then
+    its gradients with respect to every x_i and every generator (the batch's
+    forward, the batch's adjoint and the batched generator-gradient
+    correlation).  The whole step (generators to spectra, forward, loss,
+    backward) is captured in one HIP graph after an eager warm-up."""
     import torch
     import fastmat_amd.autograd as fa
     from fastmat_amd.batch import ConvBatch
     spec = make_uneven_batch()
     ops, ks = build_batch_ops(spec)
     batch = ConvBatch(ops, ks)
+    # schedule (grid, group budget, lanes) tuned on plain buffers first
+    txs = batch.new_inputs(False, fill=lambda k, n: torch.randn(k, n, dtype=torch.complex64))
+    tgs = batch.new_inputs(True, fill=lambda k, n: torch.randn(k, n, dtype=torch.complex64))
+    sched = batch.tune_schedule(txs, batch.new_outputs(False), tgs, batch.new_outputs(True))
+    tuned = _tuner_times(batch)
+    del txs, tgs
     gens = [torch.from_numpy(np.asarray(e[4], dtype=np.complex64)).cuda().requires_grad_(True)
             for e in spec]
     # packed column-major data: every operator's (n_i, k_i) input (and
@@ -563,11 +602,7 @@ def workload_batch_grad(args, rng):
     n_in, n_out = batch.packed_sizes()
     xp = torch.from_numpy(crandn(rng, (n_in,))).cuda().requires_grad_(True)
     yp = torch.from_numpy(crandn(rng, (n_out,))).cuda()
-    # forward + adjoint inputs/outputs, and per operator the generator
-    # gradient's reads of x and the residual and its write of the generator
-    bytes_step = (batch.algorithmic_bytes(False) + batch.algorithmic_bytes(True) +
-                  sum((op.numRows + op.numCols) * k * 8 + (op.numRows + op.numCols - 1) * 8
-                      for op, k in zip(ops, ks)))
+    bytes_step = batch_step_bytes(ops, ks)
 
     def eager_step():
         for t in gens + [xp]:
@@ -599,10 +634,10 @@ def workload_batch_grad(args, rng):
         return loss_s
 
     def check():
-        from oracle import accept
+        from oracle import accept, fast64
         step()
         torch.cuda.synchronize()
-        g_graph = [g.grad.clone() for g in gens[:8]]
+        g_graph = [g.grad.clone() for g in gens]
         x_graph = xp.grad.clone()
         l_graph = float(loss_s.detach())
         l_eager = float(eager_step().detach())
@@ -611,31 +646,119 @@ def workload_batch_grad(args, rng):
         # order; the loss's reduction may order its sums differently)
         same = (accept.max_rel_error(x_graph.cpu().numpy(), xp.grad.cpu().numpy()) < 1e-6 and
                 all(accept.max_rel_error(a.cpu().numpy(), g.grad.cpu().numpy()) < 1e-6
-                    for a, g in zip(g_graph, gens[:8])) and
+                    for a, g in zip(g_graph, gens)) and
                 abs(l_graph - l_eager) <= 1e-5 * abs(l_eager))
-        errs = []
-        from oracle import fast64
+        errs, gerrs = [], []
         xv = batch.packed_views(xp.detach(), inputs=True)
         yv = batch.packed_views(yp, inputs=False)
         gv = batch.packed_views(xp.grad, inputs=True)
-        for i in range(0, len(ops), 51):
+        for i in range(0, len(ops), 17):
             kind, m, n, k, gen = spec[i]
             x = xv[i].cpu().numpy().T.astype(np.complex128)
             y = yv[i].cpu().numpy().T.astype(np.complex128)
+            g64 = np.asarray(gen, dtype=np.complex64).astype(np.complex128)
             if kind == "circulant":
-                r = fast64.circulant(gen, x) - y
-                ref = 2 * fast64.circulant(gen, r, adjoint=True)
+                r = fast64.circulant(g64, x) - y
+                ref = 2 * fast64.circulant(g64, r, adjoint=True)
             else:
-                r = fast64.toeplitz(gen, m, x) - y
-                ref = 2 * fast64.toeplitz(gen, m, r, adjoint=True)
+                r = fast64.toeplitz(g64, m, x) - y
+                ref = 2 * fast64.toeplitz(g64, m, r, adjoint=True)
             errs.append(accept.max_rel_error(gv[i].cpu().numpy().T, ref))
-        return {"x_grad_sampled_max": max(errs), "graph_matches_eager": bool(same)}
+            gerrs.append(accept.max_rel_error(gens[i].grad.cpu().numpy(),
+                                              fast64.conv_generator_grad(kind, m, n, x, 2 * r)))
+        return {"x_grad_sampled_max": max(errs), "generator_grad_sampled_max": max(gerrs),
+                "graph_matches_eager": bool(same)}
 
-    cfg = {"workload": "uneven_batch_256_autograd_loss_grad_x_and_generators_c64",
-           "operators": len(ops), "total_columns": int(sum(ks)), "data": "packed"}
-    cfg["hipgraph"] = True
-    return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check, plain=None,
-                cpu=None, cfg=cfg, dtype="c64", metric=METRIC)
+    def per_op():
+        # the same step through the library's single-operator Function, one
+        # operator after another
+        xs = [v.detach().clone().t().requires_grad_(True)
+              for v in batch.packed_views(xp.detach(), inputs=True)]
+        ys = [v.t() for v in batch.packed_views(yp, inputs=False)]
+        gs = [g.detach().clone().requires_grad_(True) for g in gens]
+
+        def s():
+            loss = 0
+            for op, g, x, y in zip(ops, gs, xs, ys):
+                loss = loss + ((fa.conv_matvec(op, g, x) - y).abs() ** 2).sum()
+            loss.backward()
+        return s
+
+    def plain_torch():
+        # what a PyTorch user writes per operator: the generator embedded,
+        # its spectrum by torch.fft, pad -> FFT -> spectrum -> inverse FFT ->
+        # crop, all differentiated by torch's autograd
+        from fastmat_amd.circulant import Circulant
+        xs = [v.detach().clone().t().requires_grad_(True)
+              for v in batch.packed_views(xp.detach(), inputs=True)]
+        ys = [v.t() for v in batch.packed_views(yp, inputs=False)]
+        gs = [g.detach().clone().requires_grad_(True) for g in gens]
+        idx = []
+        for op in ops:
+            L, m, n = op.fftLength, op.numRows, op.numCols
+            if isinstance(op, Circulant):
+                pos = (list(range(n)) + list(range(L - (n - 1), L))) if L != n else list(range(n))
+                src = (list(range(n)) + list(range(1, n))) if L != n else list(range(n))
+            else:
+                pos = list(range(m)) + list(range(L - (n - 1), L))
+                src = list(range(m + n - 1))
+            idx.append((torch.tensor(pos, device="cuda"), torch.tensor(src, device="cuda")))
+
+        def s():
+            loss = 0
+            for op, g, x, y, (pos, src) in zip(ops, gs, xs, ys, idx):
+                L = op.fftLength
+                t = torch.zeros(L, dtype=g.dtype, device="cuda").index_add(0, pos, g[src])
+                d = torch.fft.fft(t) / L
+                Y = torch.fft.ifft(torch.fft.fft(x, n=L, dim=0) * d[:, None], dim=0,
+                                   norm="forward")[:op.numRows]
+                loss = loss + ((Y - y).abs() ** 2).sum()
+            loss.backward()
+        return s
+
+    def cpu():
+        # the reference's CPU path (oracle.cpu_path: Partial(Product(F.H,
+        # Diag, F)) with the reference's embedding lengths), forward and
+        # backward of every 8th operator on its own columns
+        from oracle import cpu_path
+        legs = []
+        for i in range(0, len(spec), 8):
+            kind, m, n, k, gen = spec[i]
+            g = np.asarray(gen, dtype=np.complex128)
+            op = cpu_path.circulant(g) if kind == "circulant" else cpu_path.toeplitz(g[:m], g[m:])
+            x = crandn(np.random.default_rng(i), (n, k))
+            if kind == "toeplitz" and m != n:
+                # backward takes an m-row block: time each direction on its
+                # own input
+                legs.append((_OneWay(op, False), x, ((n + m) * k + gen.size) * 8))
+                legs.append((_OneWay(op, True), crandn(np.random.default_rng(i + 1), (m, k)),
+                             ((n + m) * k + gen.size) * 8))
+            else:
+                legs.append((op, x, 2 * ((n + m) * k + gen.size) * 8))
+        return legs, ("every 8th operator of the batch (32 of 256), forward + backward on "
+                      "its own columns, one core; the generator gradient has no counterpart "
+                      "in the reference")
+
+    cfg = {"workload": "uneven_batch_256_circulant_toeplitz_c64_autograd_loss_grad_x_and_generators",
+           "operators": len(ops), "total_columns": int(sum(ks)), "data": "packed",
+           "schedule": {"workgroups_per_cu": sched[0], "group_mb": sched[1], "lanes": sched[2]},
+           "schedule_candidates_ms": tuned, "hipgraph": True}
+    return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check,
+                plain=plain_torch(), per_op=per_op(), cpu=cpu, cfg=cfg, dtype="c64",
+                metric=METRIC)
+
+
+class _OneWay:
+    """A CPU operator timed in one direction only (forward or backward)."""
+
+    def __init__(self, op, adjoint):
+        self.op, self.adjoint = op, adjoint
+
+    def forward(self, x):
+        return self.op.backward(x) if self.adjoint else self.op.forward(x)
+
+    def backward(self, x):
+        return None
 
 
 def workload_omp(args, rng):
@@ -731,26 +854,16 @@ def main():
     results = prof.results
     roof_note = "HIP events around the library's launches in the timed region"
     if not results and w.get("eager_step") is not None:
-        # graph replays record no per-launch events: time the same kernels
-        # launched eagerly, same number of steps, one length class at a time
-        # (in the replays the classes of a phase overlap on side streams, so
-        # an overlapped launch's duration would not be its own)
-        prev = os.environ.get("FASTMAT_AMD_BATCH_STREAMS")
-        os.environ["FASTMAT_AMD_BATCH_STREAMS"] = "1"
-        try:
-            with _lib.Profiler() as prof2:
-                for _ in range(args.steps):
-                    w["eager_step"]()
-                torch.cuda.synchronize()
-        finally:
-            if prev is None:
-                del os.environ["FASTMAT_AMD_BATCH_STREAMS"]
-            else:
-                os.environ["FASTMAT_AMD_BATCH_STREAMS"] = prev
+        # graph replays record no per-launch events: time the same step issued
+        # eagerly (same programs and schedule, so kernels overlap across
+        # lanes as they do in the replays)
+        with _lib.Profiler() as prof2:
+            for _ in range(args.steps):
+                w["eager_step"]()
+            torch.cuda.synchronize()
         results = prof2.results
-        roof_note = ("HIP events around the same launches issued eagerly, one length "
-                     "class at a time (the timed region replays HIP graphs in which "
-                     "the classes of a phase run concurrently)")
+        roof_note = ("HIP events around the same launches issued eagerly with the timed "
+                     "region's schedule (the timed region replays a HIP graph)")
     if not results:  # calibration copy: no library kernel
         print(json.dumps({"metric": w["metric"], "value": round(w["bytes"] / (ms * 1e-3) / 1e9, 2),
                           "unit": "GB/s", "ms_per_step": round(ms, 4), "config": w["cfg"]}))
@@ -759,6 +872,7 @@ def main():
     name, rec = max(results.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = rec["ms"] / rec["launches"]
     bytes_per_launch = rec["bytes"] / rec["launches"]
+    pass_bytes_per_launch = rec.get("pass_bytes", rec["bytes"]) / rec["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     if name.startswith("gemm_"):
         # dense factor: FP32 MFMA-bound (2 M K N flop over far fewer bytes)
@@ -776,9 +890,14 @@ def main():
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profiles(name, args.config), "kernel": name,
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
-                    "algorithmic_bytes_per_launch": bytes_per_launch, "timing": roof_note}
+                    "algorithmic_bytes_per_launch": bytes_per_launch,
+                    "pass_bytes_per_launch": pass_bytes_per_launch,
+                    "pass_GBps": round(pass_bytes_per_launch / (avg_ms * 1e-3) / 1e9, 1),
+                    "timing": roof_note}
     kernels = {k: {"launches": v["launches"], "avg_us": round(v["ms"] / v["launches"] * 1e3, 2),
-                   "GBps": round(v["bytes"] / v["ms"] / 1e6, 1)}
+                   "alg_GBps": round(v["bytes"] / v["ms"] / 1e6, 1),
+                   "pass_GBps": round(v.get("pass_bytes", v["bytes"]) / v["ms"] / 1e6, 1),
+                   "ms_per_step": round(v["ms"] / args.steps, 4)}
                for k, v in results.items()}
 
     extra = {}
