@@ -52,6 +52,7 @@ _PROTOTYPES = {
     "fm_conv_workspace_size": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_i64)]),
     "fm_conv_apply": (_c_int, [_vp, _c_int, _vp, _c_i64, _vp, _c_i64, _c_i64, _vp, _c_i64, _vp]),
     "fm_conv_apply_strided": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _c_i64, _vp]),
+    "fm_conv_apply_noalloc": (_c_int, [_vp, _c_int, _vp, _c_i64, _vp, _c_i64, _c_i64, _vp]),
     "fm_conv_generator_grad_workspace_size": (_c_int, [_vp, _c_i64, ctypes.POINTER(_c_i64)]),
     "fm_conv_generator_grad": (_c_int, [_vp, _vp, _c_i64, _vp, _c_i64, _c_i64, _vp, _vp, _c_i64,
                                         _vp]),

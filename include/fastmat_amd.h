@@ -144,6 +144,18 @@ fm_status fm_conv_workspace_size(const fm_conv_plan *plan, int64_t k,
 fm_status fm_conv_apply(const fm_conv_plan *plan, int adjoint, const void *x,
                         int64_t ldx, void *y, int64_t ldy, int64_t k,
                         void *workspace, int64_t workspace_bytes, void *stream);
+/* Allocation-free fm_conv_apply: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The four-step
+ * intermediate lives in a 64 MiB scratch array of the library (one per
+ * device, held until the process ends) instead of a caller workspace; the
+ * columns run in chunks that fit it.  A transform whose single column does
+ * not fit (L * element size > 64 MiB: L > 2^23 complex64, L > 2^22
+ * complex128) is rejected with FM_ERR_VALUE.  All calls on one device share
+ * the scratch: calls that overlap on one device (other streams, other
+ * threads) race; calls on one stream are ordered. */
+fm_status fm_conv_apply_noalloc(const fm_conv_plan *plan, int adjoint, const void *x,
+                                int64_t ldx, void *y, int64_t ldy, int64_t k, void *stream);
 /* Mode-wise application along a strided axis (square plans, single pass). */
 fm_status fm_conv_apply_strided(const fm_conv_plan *plan, int adjoint,
                                 const void *x, void *y, int64_t outer,
