@@ -7,8 +7,10 @@ the uneven batch of 256 Circulant / Toeplitz operators in complex64 (sizes
 log-uniform in [256, 65536], 1-32 columns each) through the autograd
 Function: one step computes every operator from its generator tensor on the
 device, the forward, loss = sum_i ||A_i x_i - y_i||^2, and the gradients with
-respect to every x_i (the adjoint) and every generator.  The step is one HIP
-graph replay, inputs resident in HBM.  ``value`` is algorithmic bytes per step
+respect to every x_i (the adjoint) and every generator.  The step uses the
+least-squares Function (residual and loss formed inside the forward apply);
+the same step through the generic matvec Function and torch's loss ops is
+timed beside it.  The step is one HIP graph replay, inputs resident in HBM.  ``value`` is algorithmic bytes per step
 over the step time: per operator the forward (n + m) k elements plus the
 generator, the adjoint the same, plus the generator-gradient write.
 
@@ -59,6 +61,9 @@ def parse():
                             "multilevel", "fista", "blockdiag", "lfsr", "batch_grad",
                             "omp"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-legs", action="store_true",
+                   help="time the step only: skip the accuracy check and the comparison "
+                        "legs (per-operator, torch.fft, generic Function, CPU); for profiling")
     p.add_argument("--cols", type=int, default=None, help="override the column count")
     return p.parse_args()
 
@@ -604,34 +609,57 @@ def workload_batch_grad(args, rng):
     yp = torch.from_numpy(crandn(rng, (n_out,))).cuda()
     bytes_step = batch_step_bytes(ops, ks)
 
-    def eager_step():
+    def lsq(g, x):
+        # the least-squares Function: residual and loss inside the forward
+        return fa.batch_least_squares_packed(batch, g, x, yp)
+
+    # the generic leg runs on a batch of its own: a captured graph keeps
+    # pointers to its batch's bound work tables, which a rebind replaces
+    batch_g = ConvBatch(ops, ks)
+    batch_g.set_schedule(*sched)
+
+    def generic(g, x):
+        # the same loss through the generic matvec Function and torch ops
+        return ((fa.batch_conv_matvec_packed(batch_g, g, x) - yp).abs() ** 2).sum()
+
+    def captured(loss_fn):
+        """Eager warm-up on a side stream, then the whole step (generators
+        to spectra, forward, loss, backward) captured into one HIP graph."""
+        def eager():
+            for t in gens + [xp]:
+                t.grad = None
+            loss = loss_fn(gens, xp)
+            loss.backward()
+            return loss
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(3):
+                eager()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
         for t in gens + [xp]:
             t.grad = None
-        out = fa.batch_conv_matvec_packed(batch, gens, xp)
-        loss = ((out - yp).abs() ** 2).sum()
-        loss.backward()
-        return loss
+        with torch.cuda.graph(graph):
+            loss = loss_fn(gens, xp)
+            loss.backward()
+        return eager, graph, loss
 
-    # whole-step capture (forward, loss, backward) into one HIP graph after
-    # eager warm-up on a side stream (plans, bindings and buffers settle)
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        for _ in range(3):
-            eager_step()
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    graph = torch.cuda.CUDAGraph()
-    for t in gens + [xp]:
-        t.grad = None
-    with torch.cuda.graph(graph):
-        out_s = fa.batch_conv_matvec_packed(batch, gens, xp)
-        loss_s = ((out_s - yp).abs() ** 2).sum()
-        loss_s.backward()
+    eager_step, graph, loss_s = captured(lsq)
 
     def step():
         graph.replay()
         return loss_s
+
+    def generic_graph():
+        holder = []
+
+        def s():
+            if not holder:  # captured on first use (after the timed region)
+                holder.append(captured(generic)[1])
+            holder[0].replay()
+        return s
 
     def check():
         from oracle import accept, fast64
@@ -739,13 +767,15 @@ def workload_batch_grad(args, rng):
                       "its own columns, one core; the generator gradient has no counterpart "
                       "in the reference")
 
+    # the generic leg replays its own graph; the timed step's graph is
+    # replayed again before the measurement
     cfg = {"workload": "uneven_batch_256_circulant_toeplitz_c64_autograd_loss_grad_x_and_generators",
            "operators": len(ops), "total_columns": int(sum(ks)), "data": "packed",
            "schedule": {"workgroups_per_cu": sched[0], "group_mb": sched[1], "lanes": sched[2]},
            "schedule_candidates_ms": tuned, "hipgraph": True}
     return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check,
-                plain=plain_torch(), per_op=per_op(), cpu=cpu, cfg=cfg, dtype="c64",
-                metric=METRIC)
+                plain=plain_torch(), per_op=per_op(), generic=generic_graph(), cpu=cpu,
+                cfg=cfg, dtype="c64", metric=METRIC)
 
 
 class _OneWay:
@@ -901,10 +931,14 @@ def main():
                for k, v in results.items()}
 
     extra = {}
-    try:
-        extra["max_rel_err"] = w["check"]()
-    except Exception as e:  # report, do not hide the measurement
-        extra["max_rel_err"] = "check failed: %s" % e
+    if args.no_legs:
+        for leg in ("plain", "per_op", "generic", "cpu"):
+            w[leg] = None
+    else:
+        try:
+            extra["max_rel_err"] = w["check"]()
+        except Exception as e:  # report, do not hide the measurement
+            extra["max_rel_err"] = "check failed: %s" % e
     if w.get("plain") is not None:
         s = w["plain"]
         s()
@@ -916,6 +950,17 @@ def main():
         pms = (time.perf_counter() - t2) * 1e3 / args.steps
         extra["plain_torch_fft_ms_per_step"] = round(pms, 3)
         extra["speedup_vs_plain_torch_fft"] = round(pms / ms, 3)
+    if w.get("generic") is not None:
+        # the same step through the generic matvec Function + torch loss ops
+        s = w["generic"]
+        s()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            s()
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - t2) * 1e3 / args.steps
+        extra["generic_function_graph_ms_per_step"] = round(pms, 3)
     if w.get("per_op") is not None:
         s = w["per_op"]
         s()

@@ -78,6 +78,11 @@ _PROTOTYPES = {
     "fm_batch_bind": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
                                ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _vp, _c_i64, _vp]),
     "fm_batch_apply": (_c_int, [_vp, _c_int, _vp]),
+    "fm_batch_bind_residual": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                                        ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                                        ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                                        ctypes.c_double, _vp, _c_i64, _vp]),
+    "fm_batch_residual_loss": (_c_int, [_vp, _vp, _vp]),
     "fm_batch_grad_workspace_size": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
     "fm_batch_grad_bind": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
                                     ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), ctypes.POINTER(_vp),

@@ -229,3 +229,74 @@ def batch_conv_matvec_packed(batch, gens, xp):
         raise ValueError("packed input must be a flat tensor of %d elements"
                          % batch.packed_sizes()[0])
     return _BatchConvPacked.apply(batch, xp, *gens)
+
+
+class _BatchLeastSquares(torch.autograd.Function):
+    """loss = sum_i ||A_i(gen_i) x_i - y_i||^2 over a packed batch, with the
+    residual formed and summed inside the forward apply's last pass
+    (fm_batch_bind_residual): the forward stores 2 (A x - y), which is
+    dLoss/d(A x), so the backward is the adjoint apply of it and the batched
+    generator gradient, reusing the column transforms the two applies leave
+    in the workspace."""
+
+    @staticmethod
+    def forward(ctx, batch, xp, yp, *gens):
+        r_buf, _, gx_buf = _packed_buffers(batch, xp.dtype, xp.device)
+        ctx.tok = batch.load_generators(gens)
+        X = batch.packed_views(xp, adjoint=False, inputs=True)
+        Y = batch.packed_views(yp, adjoint=False, inputs=False)
+        R = batch.packed_views(r_buf, adjoint=False, inputs=False)
+        batch.bind_residual(X, Y, R, scale=2.0)
+        batch.launch(False)
+        ctx.fwd_count = batch.apply_count[0]
+        loss = getattr(batch, "_lsq_loss", None)
+        if loss is None or loss.device != xp.device:
+            loss = batch._lsq_loss = torch.empty(1, dtype=torch.float64, device=xp.device)
+        batch.residual_loss(loss)
+        ctx.batch = batch
+        ctx.gen_complex = [g.is_complex() for g in gens]
+        ctx.gen_dtype = [g.dtype for g in gens]
+        ctx.save_for_backward(xp, *gens)
+        real_t = torch.float32 if xp.dtype == torch.complex64 else torch.float64
+        return loss[0].to(real_t)
+
+    @staticmethod
+    def backward(ctx, gl):
+        batch = ctx.batch
+        n = len(batch.ops)
+        saved = ctx.saved_tensors
+        xp, gens = saved[0], saved[1:]
+        r_buf, _, gx_buf = _packed_buffers(batch, xp.dtype, xp.device)
+        G = batch.packed_views(r_buf, adjoint=False, inputs=False)  # 2 (A x - y)
+        gxp = None
+        reuse = False
+        if ctx.needs_input_grad[1] or any(ctx.needs_input_grad[3:]):
+            if batch.generator_token != ctx.tok:
+                batch.load_generators(gens)
+            batch.apply(G, batch.packed_views(gx_buf, adjoint=False, inputs=True), adjoint=True)
+            reuse = batch.apply_count[0] == ctx.fwd_count
+        scale = gl.to(xp.dtype)
+        if ctx.needs_input_grad[1]:
+            gxp = gx_buf * scale
+        ggen = [None] * n
+        if any(ctx.needs_input_grad[3:]):
+            X = batch.packed_views(xp, adjoint=False, inputs=True)
+            raw = batch.generator_grads(X, G, reuse=reuse)
+            ggen = [(r * scale).to(dt) if cplx else (r.real * gl).to(dt)
+                    for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
+        return (None, gxp, None, *ggen)
+
+
+def batch_least_squares_packed(batch, gens, xp, yp):
+    """Differentiable loss sum_i ||A_i(gen_i) x_i - y_i||^2 of a batch on
+    packed data (xp the inputs, yp the targets, as batch_conv_matvec_packed
+    packs them), with gradients for xp and every generator.  Equals
+    ((batch_conv_matvec_packed(batch, gens, xp) - yp).abs() ** 2).sum(), with
+    the residual and its sum formed inside the batch's forward apply."""
+    n_in, n_out = batch.packed_sizes()
+    if xp.dim() != 1 or xp.numel() != n_in or yp.dim() != 1 or yp.numel() != n_out:
+        raise ValueError("packed inputs of %d and targets of %d elements expected"
+                         % (n_in, n_out))
+    if yp.dtype != xp.dtype:
+        raise TypeError("targets must have the inputs' dtype")
+    return _BatchLeastSquares.apply(batch, xp, yp.detach().contiguous(), *gens)

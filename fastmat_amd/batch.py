@@ -254,6 +254,45 @@ class ConvBatch:
         self._keep = getattr(self, "_keep", {})
         self._keep[int(adjoint)] = (list(ins), list(outs))
 
+    def bind_residual(self, ins, targets, outs, scale=2.0):
+        """Bind the forward direction in residual mode: outs[i] receives
+        scale * (A_i ins[i] - targets[i]) and ``residual_loss`` the sum of
+        |A_i ins[i] - targets[i]|^2 (fm_batch_bind_residual)."""
+        key = ("res", float(scale)) + tuple(t.data_ptr() for t in ins) + \
+            tuple(t.data_ptr() for t in targets) + tuple(t.data_ptr() for t in outs)
+        if self._bound[0] == key:
+            return
+        for i, (op, k) in enumerate(zip(self.ops, self.ks)):
+            if (tuple(ins[i].shape) != (k, op.numCols) or tuple(outs[i].shape) != (k, op.numRows)
+                    or tuple(targets[i].shape) != (k, op.numRows)):
+                raise ValueError("ConvBatch: buffer %d has the wrong shape" % i)
+            for t in (ins[i], targets[i], outs[i]):
+                if not t.is_contiguous():
+                    raise ValueError("ConvBatch: buffers must be contiguous blocks")
+                if D.np_dtype(t.dtype) != self.dtype:
+                    raise TypeError("ConvBatch: buffers must be %s" % self.dtype)
+        n = len(self.ops)
+        pin = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ins])
+        ptg = (ctypes.c_void_p * n)(*[t.data_ptr() for t in targets])
+        pout = (ctypes.c_void_p * n)(*[t.data_ptr() for t in outs])
+        ldin = (ctypes.c_int64 * n)(*[t.shape[1] for t in ins])
+        ldt = (ctypes.c_int64 * n)(*[t.shape[1] for t in targets])
+        ldout = (ctypes.c_int64 * n)(*[t.shape[1] for t in outs])
+        ws = None if self._ws is None else _lib.ptr(self._ws)
+        _lib.call("fm_batch_bind_residual", self._h, pin, ldin, ptg, ldt, pout, ldout,
+                  float(scale), ws, self._ws_bytes, D.stream())
+        self._bound[0] = key
+        self._keep = getattr(self, "_keep", {})
+        self._keep[0] = (list(ins), list(targets), list(outs))
+
+    def residual_loss(self, out):
+        """Write the residual-mode loss of the last forward apply to the
+        device float64 tensor ``out`` (one element)."""
+        if out.dtype != torch.float64 or out.numel() < 1 or not out.is_cuda:
+            raise TypeError("residual_loss needs a float64 device tensor")
+        _lib.call("fm_batch_residual_loss", self._h, _lib.ptr(out), D.stream())
+        return out
+
     def bind_views(self, ins, outs, adjoint=False):
         """Bind column slices of wider blocks (unit element stride, row
         stride >= width) without copying them; the rows' stride is passed as

@@ -52,12 +52,18 @@ template <typename T> struct BatchDesc {
 };
 
 // Generator-gradient correlation of one operator (see corr_tile).
+enum CorrMode {
+    CORR_FULL = 0,     // sum over the columns, inverse transform, store
+    CORR_PARTIAL = 1,  // sum over a chunk of columns, store the spectrum
+    CORR_SUM = 2       // sum stored partial spectra, inverse transform, store
+};
 template <typename T> struct CorrDesc {
     TileArgs<T> a;       // x side load (column 0) and the store
     const cpx<T> *g;     // g side, column 0
     int64_t x_col, g_col;
     int64_t g_valid;     // zero-pad bound of the g side
-    int64_t k;           // columns summed over
+    int64_t k;           // columns (or partial spectra) summed over
+    int mode;            // CorrMode
 };
 
 // One persistent kernel per (length, conv) class: grid-strided over the
@@ -81,13 +87,16 @@ template <typename T> constexpr int corr_loge() { return sizeof(T) == 4 ? 4 : 3;
 
 // One tile of the generator-gradient correlation: for the tile's B
 // sequences (rows of a four-step, or the one sequence of a one-pass
-// transform), over all k columns c: X = FFT(x_c rows), G = FFT(g_c rows),
-// acc += G conj(X); then the inverse transform of acc (radices reversed, from
-// the registers) and the store of `a` (four-step twiddle W_L^{-n2 k1} for
-// rows).  In four-step terms: x_c rows are the forward's pass-A output, so
-// FFT(x_c rows) are bins k1 + N1 k2 of x_c's length-L transform, and the
-// stored rows followed by the column inverse (pass C) give the unnormalised
-// inverse transform of sum_c G_c conj(X_c), i.e. L times the correlation
+// transform), over the k columns c: X = FFT(x_c rows), G = FFT(g_c rows),
+// acc += G conj(X); then (CORR_FULL) the inverse transform of acc (radices
+// reversed, from the registers) and the store of `a` (four-step twiddle
+// W_L^{-n2 k1} for rows), or (CORR_PARTIAL) the store of acc itself, a
+// partial spectrum in natural order.  CORR_SUM loads k partial spectra,
+// sums them and stores their inverse transform.  In four-step terms: x_c rows
+// are the forward's pass-A output, so FFT(x_c rows) are bins k1 + N1 k2 of
+// x_c's length-L transform, and the stored rows followed by the column
+// inverse (pass C) give the unnormalised inverse transform of
+// sum_c G_c conj(X_c), i.e. L times the correlation
 // sum_c sum_i g_c[i] conj(x_c[(i - d) mod L]).
 template <typename T, int LOGN, int NT, bool I32>
 __device__ __forceinline__ void corr_tile(const CorrDesc<T> &d, int64_t tile, cpx<T> *lds) {
@@ -97,10 +106,27 @@ __device__ __forceinline__ void corr_tile(const CorrDesc<T> &d, int64_t tile, cp
     constexpr int E = S::E;
     constexpr int LOGE = S::LOGE;
     constexpr int LOGB = S::LOGB;
+    // radix of the last stage of a forward (natural-radix) transform, and of
+    // the radix-reversed inverse
+    constexpr int LOGR_FWD = P::S > 0 ? P::bits(P::S - 1) : 0;
+    constexpr int LOGR_REV = P::S > 0 ? P::bits(0) : 0;
     cpx<T> acc[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) acc[q] = mk(T(0), T(0));
-    TileArgs<T> ax = d.a, ag = d.a;
+    TileArgs<T> ax = d.a;
+    if (d.mode == CORR_SUM) {
+        for (int64_t c = 0; c < d.k; ++c) {
+            ax.x = d.a.x + c * d.x_col;
+            cpx<T> v[E];
+            fft_tile_load<T, LOGN, NT, I32, LE>(ax, tile, v);
+#pragma unroll
+            for (int q = 0; q < E; ++q) acc[q] = acc[q] + v[q];
+        }
+        if constexpr (P::S > 0) fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 1>(acc, lds, d.a.tw, false);
+        fft_tile_store<T, LOGN, NT, LOGR_FWD, I32, LE>(d.a, tile, acc, lds, false);
+        return;
+    }
+    TileArgs<T> ag = d.a;
     ag.n_valid = d.g_valid;
     for (int64_t c = 0; c < d.k; ++c) {
         ax.x = d.a.x + c * d.x_col;
@@ -115,10 +141,13 @@ __device__ __forceinline__ void corr_tile(const CorrDesc<T> &d, int64_t tile, cp
 #pragma unroll
         for (int q = 0; q < E; ++q) acc[q] = acc[q] + cmulc(vg[q], vx[q]);
     }
+    if (d.mode == CORR_PARTIAL) {
+        fft_tile_store<T, LOGN, NT, LOGR_FWD, I32, LE>(d.a, tile, acc, lds, false);
+        return;
+    }
     if constexpr (P::S > 0)
         fft_all<T, LOGN, LOGE, +1, NT, LOGB, 0, 1, true>(acc, lds, d.a.tw, false);
-    constexpr int LOGRL = P::S > 0 ? P::bits(0) : 0;
-    fft_tile_store<T, LOGN, NT, LOGRL, I32, LE>(d.a, tile, acc, lds, false);
+    fft_tile_store<T, LOGN, NT, LOGR_REV, I32, LE>(d.a, tile, acc, lds, false);
 }
 
 template <typename T, int LOGN, int NT, bool I32>
@@ -141,9 +170,14 @@ template <typename T> constexpr int batch_max_log() { return sizeof(T) == 4 ? 13
 // tile (at least 64 threads), one-pass transforms one sequence per tile.
 constexpr int corr_threads(int logN, int loge, bool rows) {
     const int per_seq = logN > loge ? (1 << (logN - loge)) : 1;
-    const int nt = rows ? per_seq * 8 : per_seq;
+    const int nt = rows ? (per_seq * 8 < 256 ? 256 : per_seq * 8) : per_seq;
     return nt < 64 ? 64 : (nt > 512 ? 512 : nt);
 }
+
+// Columns per correlation tile of a one-pass transform: an operator's
+// columns are split into chunks that run as separate tiles (partial spectra),
+// then summed by one more tile.
+constexpr int kCorrChunk = 4;
 
 enum LaunchKind { LK_FFT = 0, LK_CONV = 1, LK_CORR = 2 };
 
@@ -235,6 +269,11 @@ struct fm_batch_plan {
     void *stab;
     int s_gen_dtype;
     int64_t s_maxL;
+    // residual mode of the forward direction (fm_batch_bind_residual): tile
+    // partial sums of |A x - t|^2
+    double *res_part;
+    int64_t n_res;
+    bool res_bound;
     // side streams (default priority) for the lanes, forked from and joined
     // back into the caller's stream
     static constexpr int kSide = 3;
@@ -636,7 +675,8 @@ extern "C" fm_status fm_batch_set_schedule(fm_batch_plan *p, int workgroups_per_
 template <typename T>
 static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *ins,
                               const int64_t *ld_in, void *const *outs, const int64_t *ld_out,
-                              void *workspace, hipStream_t s) {
+                              void *workspace, hipStream_t s, const void *const *targets = nullptr,
+                              const int64_t *ld_t = nullptr, double res_scale = 1.0) {
     const int dd = adjoint ? 1 : 0;
     std::vector<PassList<T>> pipes(p->count);
     for (int i = 0; i < p->count; ++i) {
@@ -668,6 +708,34 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
             pipes[i][1].a.y = t2;
             pipes[i][2].a.x = t2;
         }
+    }
+    if (targets) {
+        // residual mode: the last pass of every operator stores
+        // res_scale * (A x - t) and its tiles' sums of |A x - t|^2
+        int64_t slots = 0;
+        for (int i = 0; i < p->count; ++i) {
+            FM_REQUIRE(targets[i] != nullptr && ld_t[i] == ld_out[i], FM_ERR_VALUE,
+                       "batch entry %d: target missing or with another leading dimension", i);
+            TileArgs<T> a = pipes[i].back().a;
+            fm_status st = normalize_pass(a);
+            if (st) return st;
+            slots += a.ntiles;
+        }
+        if (p->res_part) (void)hipFree(p->res_part);
+        p->res_part = nullptr;
+        FM_CHECK_HIP(hipMalloc(&p->res_part, sizeof(double) * size_t(slots > 0 ? slots : 1)));
+        int64_t base = 0;
+        for (int i = 0; i < p->count; ++i) {
+            TileArgs<T> &a = pipes[i].back().a;
+            a.target = (const cpx<T> *)targets[i];
+            a.res_part = p->res_part;
+            a.res_base = base;
+            a.out_scale = T(res_scale);
+            TileArgs<T> n = a;
+            (void)normalize_pass(n);
+            base += n.ntiles;
+        }
+        p->n_res = slots;
     }
     // groups of multi-pass operators, longest transform first, whose
     // intermediates (T1 + T2) stay within the budget
@@ -706,6 +774,7 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
     }
     fm_status st = finalize<T>(b, p->dir[dd], fm_batch_plan::kSide + 1, s);
     if (st) return st;
+    if (dd == 0) p->res_bound = targets != nullptr;
     p->ws_bound[dd] = workspace;
     p->ins_bound[dd].assign(ins, ins + p->count);
     p->ld_bound[dd].assign(ld_in, ld_in + p->count);
@@ -725,6 +794,50 @@ extern "C" fm_status fm_batch_bind(fm_batch_plan *p, int adjoint, const void *co
                                    as_stream(stream));
     return batch_bind_t<double>(p, adjoint, ins, ld_in, outs, ld_out, workspace,
                                 as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_bind_residual(fm_batch_plan *p, const void *const *ins,
+                                            const int64_t *ld_in, const void *const *targets,
+                                            const int64_t *ld_t, void *const *outs,
+                                            const int64_t *ld_out, double scale, void *workspace,
+                                            int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p && ins && ld_in && targets && ld_t && outs && ld_out, FM_ERR_VALUE,
+               "NULL argument");
+    FM_REQUIRE(workspace_bytes >= p->ws_bytes, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)p->ws_bytes);
+    if (p->prec == FM_COMPLEX64)
+        return batch_bind_t<float>(p, 0, ins, ld_in, outs, ld_out, workspace, as_stream(stream),
+                                   targets, ld_t, scale);
+    return batch_bind_t<double>(p, 0, ins, ld_in, outs, ld_out, workspace, as_stream(stream),
+                                targets, ld_t, scale);
+}
+
+namespace fm {
+// loss = sum of the tile partial sums, in slot order (deterministic).
+__global__ __launch_bounds__(256) void residual_sum_kernel(const double *__restrict__ part,
+                                                           int64_t n, double *__restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+}  // namespace fm
+
+extern "C" fm_status fm_batch_residual_loss(const fm_batch_plan *p, double *loss, void *stream) {
+    FM_REQUIRE(p != nullptr && loss != nullptr, FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(p->res_bound && p->res_part, FM_ERR_VALUE,
+               "the forward direction is not bound in residual mode");
+    hipLaunchKernelGGL(residual_sum_kernel, dim3(1), dim3(256), 0, as_stream(stream),
+                       (const double *)p->res_part, p->n_res, loss);
+    FM_CHECK_LAUNCH();
+    return FM_SUCCESS;
 }
 
 extern "C" fm_status fm_batch_apply(const fm_batch_plan *p, int adjoint, void *stream) {
@@ -778,9 +891,16 @@ __global__ void batch_fold_kernel(const GradEntry<T> *__restrict__ tab) {
 
 // Gradient workspace of one operator: rows of the correlation (four-step
 // only) and r.
-static int64_t grad_ws_entry(const fm_conv_plan *c) {
+static int64_t corr_chunks(const fm_conv_plan *c, int64_t k) {
+    return c->geo.passes > 1 ? 1 : ceil_div(k, kCorrChunk);
+}
+
+static int64_t grad_ws_entry(const fm_conv_plan *c, int64_t k) {
     const int64_t e = int64_t(esize(c->prec)), L = c->L;
-    return (c->geo.passes > 1 ? up256(e * L) : 0) + up256(e * L);
+    const int64_t nch = corr_chunks(c, k);
+    // rows (four-step) or partial spectra (one-pass, several chunks), and r
+    const int64_t first = c->geo.passes > 1 ? up256(e * L) : (nch > 1 ? up256(e * L * nch) : 0);
+    return first + up256(e * L);
 }
 
 template <typename T>
@@ -804,10 +924,14 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
         FM_REQUIRE(ldx[i] >= c->cols && ldg[i] >= c->rows, FM_ERR_VALUE,
                    "batch entry %d: leading dimension too small", i);
         char *e = ws + off;
-        off += grad_ws_entry(c);
+        off += grad_ws_entry(c, k);
         const bool multi = c->geo.passes > 1;
+        const int64_t nch = corr_chunks(c, k);
         cpx<T> *rows = multi ? (cpx<T> *)e : nullptr;
-        cpx<T> *r = (cpx<T> *)(e + (multi ? up256(int64_t(sizeof(cpx<T>)) * L) : 0));
+        cpx<T> *parts = (!multi && nch > 1) ? (cpx<T> *)e : nullptr;
+        const int64_t first = multi ? up256(int64_t(sizeof(cpx<T>)) * L)
+                                    : (nch > 1 ? up256(int64_t(sizeof(cpx<T>)) * L * nch) : 0);
+        cpx<T> *r = (cpx<T> *)(e + first);
         CorrDesc<T> cd;
         memset(&cd, 0, sizeof(cd));
         TileArgs<T> a = blank_args<T>();
@@ -855,6 +979,7 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
             cd.g_col = L;
             cd.g_valid = INT64_MAX;
             cd.k = k;
+            cd.mode = CORR_FULL;
             pc.refs.push_back({1, bc.add_corr(cd)});
             // pass C of the rows (column inverse), natural order into r
             PipeIO io;
@@ -879,13 +1004,41 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
             a.out_os = 0; a.out_js = 0; a.out_ns = 1;
             a.out_gn = 1; a.out_gj = 0; a.m_valid = L;
             a.alg_bytes = double(c->gen_len) * double(sizeof(cpx<T>));
-            cd.a = a;
-            cd.g = (const cpx<T> *)gs[i];
+            cd.g_valid = c->rows;
             cd.x_col = ldx[i];
             cd.g_col = ldg[i];
-            cd.g_valid = c->rows;
-            cd.k = k;
-            pc.refs.push_back({1, bc.add_corr(cd)});
+            if (nch == 1) {
+                cd.a = a;
+                cd.g = (const cpx<T> *)gs[i];
+                cd.k = k;
+                cd.mode = CORR_FULL;
+                pc.refs.push_back({1, bc.add_corr(cd)});
+            } else {
+                // chunks of columns -> partial spectra, then their sum
+                for (int64_t q = 0; q < nch; ++q) {
+                    const int64_t c0 = q * kCorrChunk;
+                    CorrDesc<T> pd = cd;
+                    pd.a = a;
+                    pd.a.x = (const cpx<T> *)xs[i] + c0 * ldx[i];
+                    pd.a.y = parts + q * L;
+                    pd.a.alg_bytes = q == 0 ? a.alg_bytes : 0.0;
+                    pd.g = (const cpx<T> *)gs[i] + c0 * ldg[i];
+                    pd.k = k - c0 < kCorrChunk ? k - c0 : kCorrChunk;
+                    pd.mode = CORR_PARTIAL;
+                    pc.refs.push_back({1, bc.add_corr(pd)});
+                }
+                CorrDesc<T> sd;
+                memset(&sd, 0, sizeof(sd));
+                sd.a = a;
+                sd.a.x = parts;
+                sd.a.n_valid = INT64_MAX;
+                sd.a.alg_bytes = 0;
+                sd.x_col = L;
+                sd.k = nch;
+                sd.g_valid = INT64_MAX;
+                sd.mode = CORR_SUM;
+                pp.refs.push_back({1, bp.add_corr(sd)});
+            }
         }
         tab[i] = GradEntry<T>{r, (cpx<T> *)grads[i], L, c->head, c->tail, c->gen_len, c->kind};
         if (c->gen_len > maxgen) maxgen = c->gen_len;
@@ -929,7 +1082,7 @@ static fm_status batch_grad_apply_t(const fm_batch_plan *p, bool run_pass_a, hip
 extern "C" fm_status fm_batch_grad_workspace_size(const fm_batch_plan *p, int64_t *bytes) {
     FM_REQUIRE(p && bytes, FM_ERR_VALUE, "NULL argument");
     int64_t t = 0;
-    for (int i = 0; i < p->count; ++i) t += grad_ws_entry(p->plans[i]);
+    for (int i = 0; i < p->count; ++i) t += grad_ws_entry(p->plans[i], p->ks[i]);
     *bytes = t;
     return FM_SUCCESS;
 }
@@ -1118,6 +1271,7 @@ extern "C" fm_status fm_batch_plan_destroy(fm_batch_plan *p) {
     free_program(p->sprog);
     if (p->gtab) (void)hipFree(p->gtab);
     if (p->stab) (void)hipFree(p->stab);
+    if (p->res_part) (void)hipFree(p->res_part);
     for (int q = 0; q < fm_batch_plan::kSide; ++q) {
         if (p->side[q]) (void)hipStreamDestroy(p->side[q]);
         if (p->join_ev[q]) (void)hipEventDestroy(p->join_ev[q]);

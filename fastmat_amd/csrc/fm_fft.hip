@@ -260,6 +260,24 @@ fm_status make_geometry(int prec, int logL, bool conv, hipStream_t stream, Geome
 // on 2^20 x 64 and Bluestein 1,000,003 x 64 (complex64), a 96 MB budget was
 // 14% slower than one chunk: the smaller grids lose more to tails than the
 // cache saves.
+void lanes_create(Lanes *l) {
+    l->ok = hipStreamCreateWithFlags(&l->side, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&l->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&l->join, hipEventDisableTiming) == hipSuccess;
+}
+
+void lanes_destroy(Lanes *l) {
+    if (l->side) (void)hipStreamDestroy(l->side);
+    if (l->fork) (void)hipEventDestroy(l->fork);
+    if (l->join) (void)hipEventDestroy(l->join);
+    *l = Lanes();
+}
+
+int chunk_lanes(const Lanes *l) {
+    if (!l || !l->ok) return 1;
+    return knob("FASTMAT_AMD_CHUNK_LANES", 1) >= 2 ? 2 : 1;
+}
+
 int64_t chunk_columns(const Geometry &g, int64_t k, size_t esz) {
     if (g.passes == 1) return k;
     const int64_t mb = knob("FASTMAT_AMD_CHUNK_MB", 0);
@@ -372,6 +390,7 @@ struct fm_fourier_plan {
     Geometry geo;
     void *chirp;  // p[n] = exp(-i pi n^2 / N), N entries (Bluestein)
     void *D;      // fft(g)/L, L entries (Bluestein)
+    Lanes lanes;  // second stream for chunked multi-pass applies
 };
 
 extern "C" int fm_abi_version(void) { return FM_ABI_VERSION; }
@@ -423,6 +442,7 @@ extern "C" fm_status fm_fourier_plan_create(int64_t order, int precision, void *
         fm_fourier_plan_destroy(p);
         return st;
     }
+    if (p->geo.passes > 1) lanes_create(&p->lanes);
     *plan = p;
     return FM_SUCCESS;
 }
@@ -440,9 +460,9 @@ extern "C" fm_status fm_fourier_workspace_size(const fm_fourier_plan *p, int64_t
                                                int64_t *bytes) {
     FM_REQUIRE(p != nullptr && bytes != nullptr, FM_ERR_VALUE, "NULL argument");
     FM_REQUIRE(k >= 0, FM_ERR_VALUE, "negative column count");
-    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L *
-                                     chunk_columns(p->geo, k, esize(p->prec))
-                               : 0;
+    const int64_t cc = chunk_columns(p->geo, k, esize(p->prec));
+    const int nl = cc < k ? chunk_lanes(&p->lanes) : 1;
+    *bytes = p->geo.passes > 1 ? int64_t(esize(p->prec)) * p->L * cc * nl : 0;
     return FM_SUCCESS;
 }
 
@@ -472,7 +492,7 @@ extern "C" fm_status fm_fourier_apply(const fm_fourier_plan *p, int adjoint, con
         io.post = p->chirp;
         io.D = p->D;
     }
-    return run_pipeline_any(p->geo, io, as_stream(stream));
+    return run_pipeline_any(p->geo, io, as_stream(stream), &p->lanes);
 }
 
 extern "C" fm_status fm_fourier_apply_strided(const fm_fourier_plan *p, int adjoint,
@@ -492,6 +512,7 @@ extern "C" fm_status fm_fourier_plan_destroy(fm_fourier_plan *p) {
     if (!p) return FM_SUCCESS;
     if (p->chirp) hipFree(p->chirp);
     if (p->D) hipFree(p->D);
+    lanes_destroy(&p->lanes);
     delete p;
     return FM_SUCCESS;
 }

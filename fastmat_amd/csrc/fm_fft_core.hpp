@@ -191,6 +191,13 @@ template <typename T> struct TileArgs {
     int seqfast_out;
     int logJ;                       // log2(J) when J is a power of two, else -1
     int64_t o_group;                // > 1: tiles walk groups of o_group o's (see tile_first_seq)
+    // optional residual epilogue (target != NULL): the store writes
+    // out_scale * (u - t) with t read from `target` at the output's position
+    // (same layout as y), and the tile's sum of |u - t|^2 over the stored
+    // elements goes to res_part[res_base + tile] (FP64)
+    const cpx<T> *target;
+    double *res_part;
+    int64_t res_base;
     // host only: this pass's share of its transform's algorithmic bytes (the
     // input and output blocks and the generator, split over the passes),
     // for the library's profiler; 0 = count the pass's own reads and writes
@@ -440,6 +447,7 @@ __device__ __forceinline__ void fft_tile_store(const TileArgs<T> &a, int64_t til
     const Ix lo_mask = (Ix(1) << a.twb_shift) - 1;
     const Ix mv = idx_bound<Ix>(a.m_valid);
     const Ix out_gn = Ix(a.out_gn), out_gj = Ix(a.out_gj), out_ns = Ix(a.out_ns);
+    double res2 = 0.0;  // residual epilogue: sum of |u - t|^2 of this thread
 #pragma unroll
     for (int e = 0; e < NBL; ++e) {
         int b, i;
@@ -478,8 +486,30 @@ __device__ __forceinline__ void fft_tile_store(const TileArgs<T> &a, int64_t til
             cpx<T> u = v[e * RL + r];
             if (a.twb_hi || a.post) u = cmul(u, w[r]);
             if (a.conj_out) u = conj(u);
+            if (a.target) {
+                const cpx<T> d = u - ldg(a.target, Ix(sp.out_base + n * out_ns));
+                res2 += double(d.x) * double(d.x) + double(d.y) * double(d.y);
+                u = d;
+            }
             stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
         }
+    }
+    if (a.target) {
+        // the tile's sum of |u - t|^2: shuffle within each wave, one value
+        // per wave through LDS, summed in wave order by thread 0
+        double sum = res2;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+        __syncthreads();  // the tile's LDS data is no longer read
+        double *red = reinterpret_cast<double *>(lds);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int q = 0; q < NT / 64; ++q) t += red[q];
+            a.res_part[a.res_base + tile] = t;
+        }
+        __syncthreads();  // red is read before the LDS is written again
     }
 }
 

@@ -270,26 +270,58 @@ inline fm_status build_pipeline(const Geometry &g, const PipeIO &io, PassList<T>
 // Columns per chunk of a multi-pass transform (FASTMAT_AMD_CHUNK_MB).
 int64_t chunk_columns(const Geometry &g, int64_t k, size_t esz);
 
+// A second default-priority stream on which a plan runs every other column
+// chunk of a multi-pass transform, forked from and joined back into the
+// caller's stream with events (so the apply stays stream-ordered and
+// capturable).  With chunks sized for the memory-side cache, one chunk's
+// passes overlap the next chunk's while each chunk's intermediate is still
+// cached.  Used when FASTMAT_AMD_CHUNK_LANES is 2 and the columns are chunked.
+struct Lanes {
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool ok = false;
+};
+void lanes_create(Lanes *l);
+void lanes_destroy(Lanes *l);
+// Lanes a chunked transform uses (1 or 2).
+int chunk_lanes(const Lanes *l);
+
 template <typename T>
-inline fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t stream) {
+inline fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t stream,
+                              const Lanes *lanes = nullptr) {
     const int64_t cc = chunk_columns(g, io.k, sizeof(cpx<T>));
-    for (int64_t c0 = 0; c0 < io.k; c0 += cc) {
+    const int nl = cc < io.k ? chunk_lanes(lanes) : 1;
+    const int64_t lane_bytes = (int64_t(1) << g.logL) * cc * int64_t(sizeof(cpx<T>));
+    if (nl > 1) {
+        FM_CHECK_HIP(hipEventRecord(lanes->fork, stream));
+        FM_CHECK_HIP(hipStreamWaitEvent(lanes->side, lanes->fork, 0));
+    }
+    int64_t q = 0;
+    for (int64_t c0 = 0; c0 < io.k; c0 += cc, ++q) {
+        const int lane = int(q % nl);
+        hipStream_t s = lane ? lanes->side : stream;
         PipeIO sub = io;
         sub.k = io.k - c0 < cc ? io.k - c0 : cc;
         sub.x = (const cpx<T> *)io.x + c0 * io.ldx;
         sub.y = (cpx<T> *)io.y + c0 * io.ldy;
+        if (io.work) sub.work = (char *)io.work + lane * lane_bytes;
         PassList<T> passes;
         fm_status st = build_pipeline<T>(g, sub, &passes);
-        for (int q = 0; !st && q < passes.size(); ++q)
-            st = launch_tile<T>(passes[q].a, passes[q].conv, stream);
+        for (int p = 0; !st && p < passes.size(); ++p)
+            st = launch_tile<T>(passes[p].a, passes[p].conv, s);
         if (st) return st;
+    }
+    if (nl > 1) {
+        FM_CHECK_HIP(hipEventRecord(lanes->join, lanes->side));
+        FM_CHECK_HIP(hipStreamWaitEvent(stream, lanes->join, 0));
     }
     return FM_SUCCESS;
 }
 
-inline fm_status run_pipeline_any(const Geometry &g, const PipeIO &io, hipStream_t stream) {
-    return g.prec == FM_COMPLEX64 ? run_pipeline<float>(g, io, stream)
-                                  : run_pipeline<double>(g, io, stream);
+inline fm_status run_pipeline_any(const Geometry &g, const PipeIO &io, hipStream_t stream,
+                                  const Lanes *lanes = nullptr) {
+    return g.prec == FM_COMPLEX64 ? run_pipeline<float>(g, io, stream, lanes)
+                                  : run_pipeline<double>(g, io, stream, lanes);
 }
 
 // Spectrum D = fft(t')/L of a complex128 generator vector, stored in `prec`

@@ -225,6 +225,20 @@ fm_status fm_batch_bind(fm_batch_plan *plan, int adjoint, const void *const *inp
                         const int64_t *ld_out, void *workspace,
                         int64_t workspace_bytes, void *stream);
 fm_status fm_batch_apply(const fm_batch_plan *plan, int adjoint, void *stream);
+/* Residual mode of the forward direction (the least-squares step
+ * sum_i ||A_i x_i - t_i||^2 of a training loop): like fm_batch_bind for the
+ * forward direction, but output i receives scale * (A_i x_i - t_i) (t_i a
+ * rows_i x k_i target block with the output's leading dimension) and every
+ * tile adds up |A_i x_i - t_i|^2 of what it stores; after fm_batch_apply,
+ * fm_batch_residual_loss writes the sum over all operators (FP64, summed in a
+ * fixed order) to the device double *loss.  Gradient reuse
+ * (fm_batch_grad_apply_reuse) works as with fm_batch_bind. */
+fm_status fm_batch_bind_residual(fm_batch_plan *plan, const void *const *inputs,
+                                 const int64_t *ld_in, const void *const *targets,
+                                 const int64_t *ld_t, void *const *outputs,
+                                 const int64_t *ld_out, double scale, void *workspace,
+                                 int64_t workspace_bytes, void *stream);
+fm_status fm_batch_residual_loss(const fm_batch_plan *plan, double *loss, void *stream);
 /* Batched generator gradient (pattern (a) over the uneven batch): for every
  * operator i, with x_i its forward input block (k_i x cols_i, leading
  * dimension ldx[i]) and g_i the gradient of the loss with respect to its

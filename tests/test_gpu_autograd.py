@@ -229,3 +229,109 @@ def test_packed_batch_follows_generators_and_fd_complex128(torch_mod):
         bound = 1e-7 * abs(got) + 50 * 2.2e-16 * dev_loss(base, i) / h
         say("  batch FD op", i, "coord", q, "abs err", abs(got - fd), "bound", bound)
         assert abs(got - fd) < bound
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_least_squares_function_matches_composition(torch_mod, dtype):
+    """batch_least_squares_packed (residual and loss formed inside the
+    forward apply) against the generic composition
+    ((batch_conv_matvec_packed(...) - y).abs() ** 2).sum() and the FP64
+    oracle: loss, x gradient and every generator gradient, on a mixed batch of
+    one- and multi-pass operators; twice (bindings reused)."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(51)
+    spec = [("circulant", 300, 300, 2), ("toeplitz", 200, 350, 5),
+            ("circulant", 5000, 5000, 6), ("toeplitz", 3000, 9000, 3),
+            ("circulant", 64, 64, 1), ("toeplitz", 9000, 7000, 2)]
+    gens0 = [crandn(rng, n if k_ == "circulant" else m + n - 1, dtype=dtype)
+             for k_, m, n, _ in spec]
+    ops = [make_op(k_, m, g) for (k_, m, n, _), g in zip(spec, gens0)]
+    batch = ConvBatch(ops, [e[3] for e in spec], dtype=dtype)
+    n_in, n_out = batch.packed_sizes()
+    tol = 1e-5 if dtype == np.complex64 else 1e-11
+    for rep in range(2):
+        gens = [torch.from_numpy(g.copy()).cuda().requires_grad_(True) for g in gens0]
+        xp = torch.from_numpy(crandn(rng, n_in, dtype=dtype)).cuda().requires_grad_(True)
+        yp = torch.from_numpy(crandn(rng, n_out, dtype=dtype)).cuda()
+        say("least squares", np.dtype(dtype).name, "rep", rep)
+        loss = fa.batch_least_squares_packed(batch, gens, xp, yp)
+        loss.backward()
+        gens2 = [g.detach().clone().requires_grad_(True) for g in gens]
+        xp2 = xp.detach().clone().requires_grad_(True)
+        out = fa.batch_conv_matvec_packed(batch, gens2, xp2)
+        loss2 = ((out - yp).abs() ** 2).sum()
+        loss2.backward()
+        assert abs(float(loss) - float(loss2)) <= tol * abs(float(loss2))
+        assert accept.max_rel_error(xp.grad.cpu().numpy(), xp2.grad.cpu().numpy()) < tol
+        for g, g2 in zip(gens, gens2):
+            assert accept.max_rel_error(g.grad.cpu().numpy(), g2.grad.cpu().numpy()) < tol
+        # FP64 oracle
+        xv = batch.packed_views(xp.detach(), inputs=True)
+        yv = batch.packed_views(yp, inputs=False)
+        gxv = batch.packed_views(xp.grad, inputs=True)
+        tot = 0.0
+        for i, ((k_, m, n, k), g0) in enumerate(zip(spec, gens0)):
+            x = xv[i].cpu().numpy().T.astype(np.complex128)
+            y = yv[i].cpu().numpy().T.astype(np.complex128)
+            r = oracle_apply(k_, m, g0.astype(np.complex128), x) - y
+            tot += float(np.sum(np.abs(r) ** 2))
+            gx = 2 * oracle_apply(k_, m, g0.astype(np.complex128), r, adjoint=True)
+            assert accept.max_rel_error(gxv[i].cpu().numpy().T, gx) < 10 * tol, i
+            assert accept.max_rel_error(gens[i].grad.cpu().numpy(),
+                                        fast64.conv_generator_grad(k_, m, n, x, 2 * r)) < 10 * tol
+        assert abs(float(loss) - tot) <= 10 * tol * tot
+
+
+def test_least_squares_step_replays_in_a_graph(torch_mod):
+    """The fused least-squares step (generators to spectra, forward with
+    residual, loss, adjoint, generator gradient) captured in one HIP graph:
+    every replay reproduces the eager step, with eager work between."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(52)
+    spec = [("circulant", 300, 300, 2), ("toeplitz", 3000, 9000, 3),
+            ("circulant", 5000, 5000, 4)]
+    gens0 = [crandn(rng, n if k_ == "circulant" else m + n - 1) for k_, m, n, _ in spec]
+    ops = [make_op(k_, m, g) for (k_, m, n, _), g in zip(spec, gens0)]
+    batch = ConvBatch(ops, [e[3] for e in spec])
+    n_in, n_out = batch.packed_sizes()
+    gens = [torch.from_numpy(g).cuda().requires_grad_(True) for g in gens0]
+    xp = torch.from_numpy(crandn(rng, n_in)).cuda().requires_grad_(True)
+    yp = torch.from_numpy(crandn(rng, n_out)).cuda()
+
+    def eager():
+        for t in gens + [xp]:
+            t.grad = None
+        loss = fa.batch_least_squares_packed(batch, gens, xp, yp)
+        loss.backward()
+        return loss
+
+    say("least squares graph capture")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            ref_loss = float(eager())
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    ref_x = xp.grad.clone()
+    ref_g = [g.grad.clone() for g in gens]
+    graph = torch.cuda.CUDAGraph()
+    for t in gens + [xp]:
+        t.grad = None
+    with torch.cuda.graph(graph):
+        loss = fa.batch_least_squares_packed(batch, gens, xp, yp)
+        loss.backward()
+    gx, gg = xp.grad, [g.grad for g in gens]
+    for rep in range(3):
+        gx.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert float(loss) == ref_loss, rep
+        assert torch.equal(gx, ref_x), rep
+        for a, b in zip(gg, ref_g):
+            assert torch.equal(a, b), rep
+        _ = (yp * 2).sum().item()  # eager work between replays

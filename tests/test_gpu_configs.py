@@ -123,3 +123,50 @@ def test_uneven_batch_config_gradients(torch_mod):
         worst = [max(a, b) for a, b in zip(worst, e)]
         assert max(e) < TOL32, (i, kind, m, n, k, e)
     say("  max rel err: output %.2e, x-grad %.2e, generator grad %.2e" % tuple(worst))
+
+
+def test_uneven_batch_config_least_squares(torch_mod):
+    """The 256-operator batch through the fused least-squares Function (the
+    benchmark's step): the loss and every x and generator gradient against
+    FP64."""
+    torch = torch_mod
+    import bench
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    spec = bench.make_uneven_batch()
+    ops, ks = bench.build_batch_ops(spec)
+    batch = ConvBatch(ops, ks)
+    rng = np.random.default_rng(20261202 + 6)
+    n_in, n_out = batch.packed_sizes()
+    gens = [torch.from_numpy(np.asarray(e[4], dtype=np.complex64)).cuda().requires_grad_(True)
+            for e in spec]
+    xp = torch.from_numpy(crandn(rng, n_in)).cuda().requires_grad_(True)
+    yp = torch.from_numpy(crandn(rng, n_out)).cuda()
+    say("uneven batch least squares", len(ops), "ops")
+    loss = fa.batch_least_squares_packed(batch, gens, xp, yp)
+    loss.backward()
+    torch.cuda.synchronize()
+    xv = batch.packed_views(xp.detach(), inputs=True)
+    yv = batch.packed_views(yp, inputs=False)
+    gxv = batch.packed_views(xp.grad, inputs=True)
+    tot = 0.0
+    worst = [0.0, 0.0]
+    for i, (kind, m, n, k, gen) in enumerate(spec):
+        x = xv[i].cpu().numpy().T.astype(np.complex128)
+        y = yv[i].cpu().numpy().T.astype(np.complex128)
+        g64 = np.asarray(gen, dtype=np.complex64).astype(np.complex128)
+        if kind == "circulant":
+            r = fast64.circulant(g64, x) - y
+            gx_ref = 2 * fast64.circulant(g64, r, adjoint=True)
+        else:
+            r = fast64.toeplitz(g64, m, x) - y
+            gx_ref = 2 * fast64.toeplitz(g64, m, r, adjoint=True)
+        tot += float(np.sum(np.abs(r) ** 2))
+        e = (accept.max_rel_error(gxv[i].cpu().numpy().T, gx_ref),
+             accept.max_rel_error(gens[i].grad.cpu().numpy(),
+                                  fast64.conv_generator_grad(kind, m, n, x, 2 * r)))
+        worst = [max(a, b) for a, b in zip(worst, e)]
+        assert max(e) < TOL32, (i, kind, m, n, k, e)
+    say("  loss rel err %.2e, x-grad %.2e, generator grad %.2e"
+        % (abs(float(loss) - tot) / tot, worst[0], worst[1]))
+    assert abs(float(loss) - tot) < TOL32 * tot
