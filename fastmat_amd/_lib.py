@@ -36,6 +36,7 @@ _PROTOTYPES = {
     "fm_get_last_error": (ctypes.c_char_p, []),
     "fm_hadamard_apply": (_c_int, [_c_int, _c_int, _vp, _c_i64, _vp, _c_i64, _c_i64, _vp]),
     "fm_hadamard_apply_strided": (_c_int, [_c_int, _c_int, _vp, _vp, _c_i64, _c_i64, _vp]),
+    "fm_wht_tile_check": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(_c_i64)]),
     "fm_fourier_plan_create": (_c_int, [_c_i64, _c_int, _vp, _pp]),
     "fm_fourier_plan_info": (_c_int, [_vp, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int),
                                       ctypes.POINTER(_c_int)]),

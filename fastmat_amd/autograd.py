@@ -217,7 +217,7 @@ class _BatchConvPacked(torch.autograd.Function):
             reuse = (ctx.needs_input_grad[1] and
                      batch.apply_count[0] == ctx.fwd_count)
             raw = batch.generator_grads(X, G, reuse=reuse)
-            ggen = [r.to(dt) if cplx else r.real.to(dt)
+            ggen = [(r if r.dtype == dt else r.to(dt)) if cplx else r.real.to(dt)
                     for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
         return (None, gxp, *ggen)
 
@@ -281,8 +281,8 @@ class _BatchLeastSquares(torch.autograd.Function):
         ggen = [None] * n
         if any(ctx.needs_input_grad[3:]):
             X = batch.packed_views(xp, adjoint=False, inputs=True)
-            raw = batch.generator_grads(X, G, reuse=reuse)
-            ggen = [(r * scale).to(dt) if cplx else (r.real * gl).to(dt)
+            raw = batch.generator_grads(X, G, reuse=reuse, scale=scale)
+            ggen = [(r if r.dtype == dt else r.to(dt)) if cplx else r.real.to(dt)
                     for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
         return (None, gxp, None, *ggen)
 

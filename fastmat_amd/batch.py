@@ -323,14 +323,15 @@ class ConvBatch:
         # while the same addresses are handed in again
         self._bound[int(adjoint)] = key
 
-    def generator_grads(self, xs, gs, reuse=False):
+    def generator_grads(self, xs, gs, reuse=False, scale=None):
         """Gradients of a real loss with respect to every operator's
         generator, given the forward inputs xs[i] (k_i, cols_i) and the
         output gradients gs[i] (k_i, rows_i) as contiguous blocks: what
         ``op.generator_grad(x, g)`` returns per operator, for the whole batch
         in one batched program (fm_batch_grad_*).  reuse=True reads the
         column transforms the last forward apply (of xs) and adjoint apply
-        (of gs) left in the workspace instead of recomputing them."""
+        (of gs) left in the workspace instead of recomputing them.  ``scale``
+        (a device scalar tensor) multiplies all gradients in one pass."""
         n = len(self.ops)
         for i, (op, k) in enumerate(zip(self.ops, self.ks)):
             if tuple(xs[i].shape) != (k, op.numCols) or tuple(gs[i].shape) != (k, op.numRows):
@@ -371,7 +372,7 @@ class ConvBatch:
             self._gbound = key
         _lib.call("fm_batch_grad_apply_reuse" if reuse else "fm_batch_grad_apply", self._h,
                   D.stream())
-        out = self._gout.clone()
+        out = self._gout.clone() if scale is None else self._gout * scale
         return [out[int(off[i]):int(off[i + 1])] for i in range(n)]
 
     def set_grid(self, workgroups_per_cu):
@@ -421,8 +422,11 @@ class ConvBatch:
         return self.apply(ins, outs, True)
 
     # ---------------------------------------------------------- tuning
-    DEFAULT_SCHEDULES = ((2, 96, 2), (2, 64, 2), (2, 128, 1), (2, 192, 1), (2, 0, 1),
-                         (1, 96, 2), (3, 96, 2))
+    # (workgroups per CU, group budget MB, lanes): ungrouped schedules (the
+    # multi-pass operators' passes as one launch per length class) and
+    # groups sized for the memory-side cache on one or two lanes
+    DEFAULT_SCHEDULES = ((2, 0, 2), (3, 0, 2), (4, 0, 2), (2, 48, 2), (3, 48, 2),
+                         (2, 96, 2), (3, 96, 2), (4, 96, 2), (2, 128, 1), (3, 192, 1))
 
     def tune(self, ins, outs, adjoint=False, tuner=None, candidates=(1, 2, 3, 4)):
         """Pick the persistent grid (workgroups per CU) of one direction."""

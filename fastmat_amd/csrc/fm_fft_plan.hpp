@@ -267,8 +267,6 @@ inline fm_status build_pipeline(const Geometry &g, const PipeIO &io, PassList<T>
     return FM_SUCCESS;
 }
 
-// Columns per chunk of a multi-pass transform (FASTMAT_AMD_CHUNK_MB).
-int64_t chunk_columns(const Geometry &g, int64_t k, size_t esz);
 
 // A second default-priority stream on which a plan runs every other column
 // chunk of a multi-pass transform, forked from and joined back into the
@@ -285,11 +283,14 @@ void lanes_create(Lanes *l);
 void lanes_destroy(Lanes *l);
 // Lanes a chunked transform uses (1 or 2).
 int chunk_lanes(const Lanes *l);
+// Columns per chunk of a multi-pass transform (lanes: the plan's second
+// stream, or NULL).
+int64_t chunk_columns(const Geometry &g, int64_t k, size_t esz, const Lanes *l);
 
 template <typename T>
 inline fm_status run_pipeline(const Geometry &g, const PipeIO &io, hipStream_t stream,
                               const Lanes *lanes = nullptr) {
-    const int64_t cc = chunk_columns(g, io.k, sizeof(cpx<T>));
+    const int64_t cc = chunk_columns(g, io.k, sizeof(cpx<T>), lanes);
     const int nl = cc < io.k ? chunk_lanes(lanes) : 1;
     const int64_t lane_bytes = (int64_t(1) << g.logL) * cc * int64_t(sizeof(cpx<T>));
     if (nl > 1) {

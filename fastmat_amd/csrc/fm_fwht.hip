@@ -18,9 +18,14 @@
 // stages.  Strided passes read their first stage straight from HBM with
 // neighbouring lanes on neighbouring vectors; contiguous passes load the
 // tile coalesced into LDS first.  The last stage stores straight to HBM.
+#include <algorithm>
+#include <vector>
+
 #include "fm_fft_core.hpp"  // lpad, shared helpers
 
 namespace fm {
+
+int knob(const char *name, int dflt);
 
 // Compute type per storage type: wrap-around integer types, or the float itself.
 template <typename S> struct WhtTraits;
@@ -74,8 +79,19 @@ template <int LOGN, int LOGE> struct WhtPlan {
     }
 };
 
+// Tile shape of a length-2^LOGN pass with NT threads holding 2^LOGE elements
+// each: B = NT * 2^LOGE / 2^LOGN vectors per tile.
+template <int LOGN, int NT, int LOGE> struct WhtShape {
+    static_assert(NT == 256 || NT == 512 || NT == 1024, "tile of 256, 512 or 1024 threads");
+    static constexpr int LOGNT = NT == 256 ? 8 : (NT == 512 ? 9 : 10);
+    static constexpr int LOGB = LOGNT + LOGE - LOGN;
+    static constexpr int B = 1 << (LOGB > 0 ? LOGB : 0);
+    static constexpr int N = 1 << LOGN;
+    static constexpr int SB = N + (N >> 5) + 1;  // padded LDS row
+};
+
 // Position of element t of butterfly group u for a stage on bits [Q, Q+R).
-template <int Q, int R> __device__ __forceinline__ int wht_pos(int u, int t) {
+template <int Q, int R> __host__ __device__ __forceinline__ int wht_pos(int u, int t) {
     const int low = u & ((1 << Q) - 1);
     const int high = u >> Q;
     return (high << (Q + R)) | (t << Q) | low;
@@ -110,7 +126,7 @@ __device__ __forceinline__ void wht_seq(const WhtArgs &a, int64_t s, bool &ok, i
 }
 
 template <int LOGB, int LOGU>
-__device__ __forceinline__ void wht_map(int g, bool seqfast, int &b, int &u) {
+__host__ __device__ __forceinline__ void wht_map(int g, bool seqfast, int &b, int &u) {
     if (seqfast) {
         b = g & ((1 << LOGB) - 1);
         u = g >> LOGB;
@@ -118,6 +134,14 @@ __device__ __forceinline__ void wht_map(int g, bool seqfast, int &b, int &u) {
         u = g & ((1 << LOGU) - 1);
         b = g >> LOGU;
     }
+}
+
+// Strided last-stage re-stage: thread `tid` stores element
+// p = (tid >> LOGB) + e * (NT >> LOGB) of vector b = tid mod B, e < E.
+template <int LOGB, int NT>
+__host__ __device__ __forceinline__ void wht_restage_pos(int tid, int e, int &b, int &p) {
+    b = tid & ((1 << LOGB) - 1);
+    p = (tid >> LOGB) + e * (NT >> LOGB);
 }
 
 // One stage s of the plan on registers v.  SRC: 0 = v holds the inputs
@@ -171,10 +195,10 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
     constexpr int LOGE = wht_loge<C>(), E = 1 << LOGE;
     constexpr int N = 1 << LOGN;
     using P = WhtPlan<LOGN, LOGE>;
-    constexpr int LOGNT = NT == 256 ? 8 : 9;
-    constexpr int LOGB = LOGNT + LOGE - LOGN;
-    constexpr int B = 1 << LOGB;
-    constexpr int SB = N + (N >> 5) + 1;
+    using Sh = WhtShape<LOGN, NT, LOGE>;
+    constexpr int LOGB = Sh::LOGB;
+    constexpr int B = Sh::B;
+    constexpr int SB = Sh::SB;
     static_assert(LOGB >= 0, "tile must hold a vector");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *lds = reinterpret_cast<C *>(smem_raw);
@@ -237,14 +261,15 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
         }
         __syncthreads();
         // idx = tid + e NT: vector b = idx mod B is the same for all e
-        const int b = threadIdx.x & (B - 1);
+        int b, p;
+        wht_restage_pos<LOGB, NT>(threadIdx.x, 0, b, p);
         bool ok;
         int64_t base;
         wht_seq(a, s0 + b, ok, base);
         if (ok) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const int p = (threadIdx.x >> LOGB) + e * (NT >> LOGB);
+                wht_restage_pos<LOGB, NT>(threadIdx.x, e, b, p);
                 y[base + int64_t(p) * a.ns] = to_s<S, C>(lds[b * SB + lpad(p)]);
             }
         }
@@ -269,9 +294,10 @@ template <typename S, int LOGN, int NT>
 static fm_status wht_launch_nt(WhtArgs a, hipStream_t stream) {
     using C = typename WhtTraits<S>::C;
     constexpr int LOGE = wht_loge<C>();
-    constexpr int B = (NT << LOGE) >> LOGN;
-    constexpr int N = 1 << LOGN;
-    const size_t lds = size_t(B) * (N + (N >> 5) + 1) * sizeof(C);
+    using Sh = WhtShape<LOGN, NT, LOGE>;
+    constexpr int B = Sh::B;
+    constexpr int N = Sh::N;
+    const size_t lds = size_t(B) * Sh::SB * sizeof(C);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)wht_pass_kernel<S, LOGN, NT>,
@@ -303,15 +329,108 @@ static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
     using C = typename WhtTraits<S>::C;
     constexpr int LOGE = wht_loge<C>();
     constexpr int NT = LOGN > LOGE + 8 ? 512 : 256;
-    constexpr int N = 1 << LOGN;
-    // strided vectors: 512-thread tiles when they still leave room for two
-    // tiles per CU, so every row segment a wave moves is twice as long
+    // strided vectors: wider tiles (more vectors side by side, so every row
+    // segment a wave moves is longer) when they fit the LDS: 1024 threads
+    // (one tile per CU; for 4-byte types 32 vectors, 128-byte segments) by
+    // default, else 512 (two tiles per CU).  Order 20, 64 columns, int32 and
+    // float32 forward + adjoint: 1.177 ms per step with 512, 1.016 with 1024.
+    // FASTMAT_AMD_WHT_STRIDED_NT=512 selects the narrower tile.
     if constexpr (NT == 256) {
-        constexpr int B2 = (512 << LOGE) >> LOGN;
-        constexpr size_t lds2 = size_t(B2) * (N + (N >> 5) + 1) * sizeof(C);
-        if (a.seqfast && a.J >= B2 && lds2 <= 80 * 1024) return wht_launch_nt<S, LOGN, 512>(a, stream);
+        const int want = knob("FASTMAT_AMD_WHT_STRIDED_NT", 1024);
+        if (a.seqfast && want >= 1024) {
+            using S4 = WhtShape<LOGN, 1024, LOGE>;
+            if constexpr (S4::LOGB >= 0) {
+                constexpr size_t lds4 = size_t(S4::B) * S4::SB * sizeof(C);
+                if (a.J >= S4::B && lds4 <= 160 * 1024)
+                    return wht_launch_nt<S, LOGN, 1024>(a, stream);
+            }
+        }
+        using S2 = WhtShape<LOGN, 512, LOGE>;
+        constexpr size_t lds2 = size_t(S2::B) * S2::SB * sizeof(C);
+        if (a.seqfast && a.J >= S2::B && lds2 <= 80 * 1024) return wht_launch_nt<S, LOGN, 512>(a, stream);
     }
     return wht_launch_nt<S, LOGN, NT>(a, stream);
+}
+
+// Host-side check of a tile's index mapping (the formulas the kernel runs):
+// every (vector, element) of the B x N tile must be loaded by the strided
+// stage-0 mapping, written by every stage's LDS exchange and stored by the
+// strided re-stage exactly once.  The tile runs NT threads; its shape
+// (vectors per tile B, the re-stage strides) is derived for SHAPE_NT threads,
+// which the kernel takes equal to NT (a shape derived for another thread
+// count is the error a kernel with a wrong log2(NT) makes).  Returns the
+// number of slots that are not hit exactly once (0 = a bijection).
+template <int LOGN, int NT, int LOGE, int SHAPE_NT = NT> static int64_t wht_tile_check() {
+    using Sh = WhtShape<LOGN, SHAPE_NT, LOGE>;
+    using P = WhtPlan<LOGN, LOGE>;
+    constexpr int LOGB = Sh::LOGB, B = Sh::B, N = Sh::N, E = 1 << LOGE;
+    if constexpr (LOGB < 0) {
+        return -1;
+    } else {
+        std::vector<int> hits(size_t(B) * N);
+        int64_t bad = 0;
+        auto flush = [&]() {
+            for (int h : hits) bad += h != 1;
+            std::fill(hits.begin(), hits.end(), 0);
+        };
+        auto mark = [&](int b, int pos) {
+            if (b < 0 || b >= B || pos < 0 || pos >= N) ++bad;
+            else ++hits[size_t(b) * N + pos];
+        };
+        // stage 0, strided vectors (sequence-fastest)
+        {
+            constexpr int LR = P::bits(0), R = 1 << LR, NB = E / R, LOGU = LOGN - LR;
+            for (int tid = 0; tid < NT; ++tid)
+                for (int e = 0; e < NB; ++e) {
+                    int b, u;
+                    wht_map<LOGB, LOGU>(tid + e * NT, B > 1, b, u);
+                    for (int t = 0; t < R; ++t) mark(b, wht_pos<0, LR>(u, t));
+                }
+            flush();
+        }
+        // every stage's exchange (butterfly-fastest after stage 0)
+        auto stage = [&](auto sc) {
+            constexpr int st = decltype(sc)::value;
+            constexpr int LR = P::bits(st), R = 1 << LR, Q = P::q(st), NB = E / R;
+            constexpr int LOGU = LOGN - LR;
+            for (int tid = 0; tid < NT; ++tid)
+                for (int e = 0; e < NB; ++e) {
+                    int b, u;
+                    wht_map<LOGB, LOGU>(tid + e * NT, false, b, u);
+                    for (int t = 0; t < R; ++t) mark(b, wht_pos<Q, LR>(u, t));
+                }
+            flush();
+        };
+        if constexpr (P::S > 0) stage(std::integral_constant<int, 0>());
+        if constexpr (P::S > 1) stage(std::integral_constant<int, 1>());
+        if constexpr (P::S > 2) stage(std::integral_constant<int, 2>());
+        if constexpr (P::S > 3) stage(std::integral_constant<int, 3>());
+        if constexpr (P::S > 4) stage(std::integral_constant<int, 4>());
+        static_assert(P::S <= 5, "more stages than the check covers");
+        // strided re-stage of the last stage (the kernel takes it when the
+        // pass has more than one stage, i.e. N > E, so NT >= B)
+        if constexpr (P::S > 1) {
+            for (int tid = 0; tid < NT; ++tid)
+                for (int e = 0; e < E; ++e) {
+                    int b, pp;
+                    wht_restage_pos<LOGB, SHAPE_NT>(tid, e, b, pp);
+                    mark(b, pp);
+                }
+            flush();
+        }
+        return bad;
+    }
+}
+
+template <int NT, int LOGE, int SHAPE_NT = NT> static int64_t wht_tile_check_n(int logn) {
+    switch (logn) {
+#define FM_WC(L) case L: return wht_tile_check<L, NT, LOGE, SHAPE_NT>();
+        FM_WC(1) FM_WC(2) FM_WC(3) FM_WC(4) FM_WC(5) FM_WC(6) FM_WC(7) FM_WC(8) FM_WC(9)
+        FM_WC(10) FM_WC(11) FM_WC(12) FM_WC(13) FM_WC(14)
+#undef FM_WC
+    default:
+        return -2;
+    }
 }
 
 template <typename S> constexpr int wht_maxlog() {
@@ -431,4 +550,24 @@ extern "C" fm_status fm_hadamard_apply_strided(int dtype, int order, const void 
     if (outer == 0) return FM_SUCCESS;
     FM_REQUIRE(x && y, FM_ERR_VALUE, "NULL data pointer");
     return wht_dispatch(dtype, order, x, 0, y, 0, outer, inner, true, as_stream(stream));
+}
+
+extern "C" fm_status fm_wht_tile_check(int logn, int threads, int shape_threads, int loge,
+                                       int64_t *bad) {
+    FM_REQUIRE(bad != nullptr, FM_ERR_VALUE, "NULL argument");
+    int64_t r = -2;
+#define FM_WCN(NT, LE)                                                              \
+    if (threads == NT && loge == LE) {                                              \
+        if (shape_threads == NT) r = wht_tile_check_n<NT, LE>(logn);                \
+        else if (shape_threads == 512 && NT == 1024) r = wht_tile_check_n<1024, LE, 512>(logn); \
+    }
+    FM_WCN(256, 5) FM_WCN(512, 5) FM_WCN(1024, 5)
+    FM_WCN(256, 4) FM_WCN(512, 4) FM_WCN(1024, 4)
+    FM_WCN(256, 3) FM_WCN(512, 3) FM_WCN(1024, 3)
+#undef FM_WCN
+    FM_REQUIRE(r != -2, FM_ERR_VALUE,
+               "no tile of %d threads (shape of %d), 2^%d elements, 2^%d per thread", threads,
+               shape_threads, logn, loge);
+    *bad = r;
+    return FM_SUCCESS;
 }

@@ -79,6 +79,16 @@ fm_status fm_hadamard_apply_strided(int dtype, int order, const void *x,
                                     void *y, int64_t outer, int64_t inner,
                                     void *stream);
 
+/* Host-side self-check of the Hadamard pass kernel's tile index mapping for a
+ * tile of `threads` threads (256, 512 or 1024) holding 2^loge elements each
+ * over vectors of 2^logn, with the tile shape derived for `shape_threads`
+ * (equal to `threads` in the kernel; 512 with 1024 threads restates the
+ * shape error a wrong log2 of the thread count makes): *bad receives how many
+ * (vector, element) slots the strided load, the stage exchanges or the
+ * strided re-stage store do not hit exactly once (0 = bijective; -1 = no
+ * such tile).  No counterpart in the reference; runs on the CPU. */
+fm_status fm_wht_tile_check(int logn, int threads, int shape_threads, int loge, int64_t *bad);
+
 /* ------------------------------------------------------------- Fourier
  * Unnormalised DFT of order N, F[j,l] = exp(-2 pi i j l / N); adjoint = F^H.
  * Replaces Fourier.__init__ / _forward / _backward (Fourier.pyx:63-238).

@@ -235,3 +235,34 @@ def test_partial_selection_follows_numpy_indexing(fm):
     np.testing.assert_array_equal(P2._reference(), np.eye(3)[[2, -1, 2], :])
     with pytest.raises(ValueError):
         fm.Partial(fm.Eye(3), cols=[0, 5])
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(
+    __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.dirname(
+        __import__("os").path.abspath(__file__))), "fastmat_amd", "libfastmat_amd.so")),
+    reason="library not built")
+def test_hadamard_tile_index_mapping_is_bijective():
+    """The Hadamard pass kernel's tile index formulas (strided stage-0 load,
+    every stage's LDS exchange, the strided re-stage store), run on the host
+    by the library itself: every (vector, element) slot of a tile is hit
+    exactly once for 256-, 512- and 1024-thread tiles.  A tile of 1024
+    threads whose shape is derived as for 512 (log2 of the thread count taken
+    as 9, the mistake that made the 1024-thread strided variant give wrong
+    int32 results) is caught."""
+    import ctypes
+    from fastmat_amd import _lib
+    lib = _lib.load()
+    bad = ctypes.c_int64()
+    checked = 0
+    for loge in (3, 4, 5):
+        for nt in (256, 512, 1024):
+            for logn in range(1, 15):
+                _lib.check(lib.fm_wht_tile_check(logn, nt, nt, loge, ctypes.byref(bad)))
+                assert bad.value in (0, -1), (logn, nt, loge, bad.value)
+                checked += bad.value == 0
+    assert checked > 60
+    # the order-20 strided pass with 1024 threads (2^10 elements, 32 per thread)
+    _lib.check(lib.fm_wht_tile_check(10, 1024, 1024, 5, ctypes.byref(bad)))
+    assert bad.value == 0
+    _lib.check(lib.fm_wht_tile_check(10, 1024, 512, 5, ctypes.byref(bad)))
+    assert bad.value > 0
