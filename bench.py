@@ -365,19 +365,34 @@ def workload_batch(args, rng):
 
 
 def workload_copy(args, rng):
-    """Calibration only: a plain 1 GiB device copy (read 1 GiB, write 1 GiB),
-    used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    """Calibration only: the library's plain copy kernel over 1 GiB (read 1 GiB,
+    write 1 GiB), the HBM bandwidth reference and the scale for rocprofv3
+    FETCH_SIZE / WRITE_SIZE readings."""
     import torch
+    from fastmat_amd import _device as D
+    from fastmat_amd import _lib
     n = 1 << 28
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
 
     def step():
+        _lib.call("fm_device_copy", _lib.ptr(dst), _lib.ptr(src), 4 * n, D.stream())
+
+    def runtime_copy():
+        # the HIP runtime's own device copy of the same buffers, for reference
         dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        return {"runtime_copy_ms": round(ms, 4), "runtime_copy_GBps": round(8 * n / ms / 1e6, 1)}
 
     cfg = {"workload": "calibration_copy_1GiB", "bytes_read": 4 * n, "bytes_written": 4 * n}
     return dict(step=step, bytes=8 * n, check=lambda: {}, plain=None, cpu=None, cfg=cfg,
-                dtype="f32", metric="device copy GB/s (calibration)")
+                dtype="f32", metric="device copy GB/s (calibration)", extra=runtime_copy)
 
 
 def workload_multilevel(args, rng):
@@ -939,6 +954,8 @@ def main():
             extra["max_rel_err"] = w["check"]()
         except Exception as e:  # report, do not hide the measurement
             extra["max_rel_err"] = "check failed: %s" % e
+    if w.get("extra") is not None and not args.no_legs:
+        extra.update(w["extra"]())
     if w.get("plain") is not None:
         s = w["plain"]
         s()

@@ -82,8 +82,13 @@ __global__ __launch_bounds__((PassCfg<T, LOGN>::NT)) void batch_pass_kernel(
 }
 
 // Elements per thread of the correlation tiles: three transforms' worth of
-// registers are live (x, g and the running sum), so half the pass kernels'.
-template <typename T> constexpr int corr_loge() { return sizeof(T) == 4 ? 4 : 3; }
+// registers are live (x, g and the running sum), so 8 (a quarter of the
+// complex64 pass kernels'), which leaves room for two or more waves per
+// SIMD; complex64 transforms of 8192 points hold 16, as a tile has at most
+// 512 threads.
+template <typename T> constexpr int corr_loge(int logN) {
+    return sizeof(T) == 4 && logN > 12 ? 4 : 3;
+}
 
 // One tile of the generator-gradient correlation: for the tile's B
 // sequences (rows of a four-step, or the one sequence of a one-pass
@@ -100,7 +105,7 @@ template <typename T> constexpr int corr_loge() { return sizeof(T) == 4 ? 4 : 3;
 // sum_c sum_i g_c[i] conj(x_c[(i - d) mod L]).
 template <typename T, int LOGN, int NT, bool I32>
 __device__ __forceinline__ void corr_tile(const CorrDesc<T> &d, int64_t tile, cpx<T> *lds) {
-    constexpr int LE = corr_loge<T>();
+    constexpr int LE = corr_loge<T>(LOGN);
     using S = TileShape<T, LOGN, NT, LE>;
     using P = typename S::P;
     constexpr int E = S::E;
@@ -316,7 +321,7 @@ static fm_status finalize(ProgBuilder<T> &b, Program &prog, int ncu_lanes_cap, h
     for (auto &c : b.cdescs) {
         // the correlation tiles use corr_loge elements per thread: set the
         // tile shape (B sequences) for that shape
-        const int loge = corr_loge<T>();
+        const int loge = corr_loge<T>(c.a.logN);
         const bool rows = c.a.J > 1;
         const int nt = corr_threads(c.a.logN, loge, rows);
         int logb = 0;
@@ -350,7 +355,7 @@ static fm_status finalize(ProgBuilder<T> &b, Program &prog, int ncu_lanes_cap, h
         for (const auto &r : pg.refs) {
             const TileArgs<T> &a = r.corr ? b.cdescs[r.index].a : b.descs[r.index].a;
             int kind = r.corr ? LK_CORR : (b.descs[r.index].conv ? LK_CONV : LK_FFT);
-            int nt = r.corr ? corr_threads(a.logN, corr_loge<T>(), a.J > 1) : 0;
+            int nt = r.corr ? corr_threads(a.logN, corr_loge<T>(a.logN), a.J > 1) : 0;
             auto key = std::make_tuple(kind, a.logN, nt);
             bool fit = fits_idx32(a, int64_t(1) << a.logN);
             if (r.corr) {
@@ -449,7 +454,7 @@ template <typename T, int LOGN, int NT>
 static fm_status launch_corr_nt(const fm_batch_plan *p, const Program &d, const Launch &l,
                                 hipStream_t s) {
     constexpr int N = 1 << LOGN;
-    constexpr int LE = corr_loge<T>();
+    constexpr int LE = corr_loge<T>(LOGN);
     using S = TileShape<T, LOGN, NT, LE>;
     const size_t lds = size_t(S::B) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
     static bool attr = false;
@@ -487,8 +492,8 @@ template <typename T, int LOGN>
 static fm_status launch_corr(const fm_batch_plan *p, const Program &d, const Launch &l,
                              hipStream_t s) {
     // the two tile shapes corr_threads chooses for this length
-    constexpr int NT_ROWS = corr_threads(LOGN, corr_loge<T>(), true);
-    constexpr int NT_ONE = corr_threads(LOGN, corr_loge<T>(), false);
+    constexpr int NT_ROWS = corr_threads(LOGN, corr_loge<T>(LOGN), true);
+    constexpr int NT_ONE = corr_threads(LOGN, corr_loge<T>(LOGN), false);
     if (l.nt == NT_ROWS) return launch_corr_nt<T, LOGN, NT_ROWS>(p, d, l, s);
     if constexpr (NT_ONE != NT_ROWS)
         if (l.nt == NT_ONE) return launch_corr_nt<T, LOGN, NT_ONE>(p, d, l, s);
@@ -913,8 +918,12 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
     ProgBuilder<T> ba, bc, bp;
     auto &pa_x = ba.group(0);
     auto &pa_g = ba.group(p->lanes > 1 ? 1 : 0);
+    // the four-step operators' correlations and row inverses on lane 0, the
+    // one-pass operators' on a lane of their own (they share no data)
     auto &pc = bc.group(0);
+    auto &pc1 = bc.group(p->lanes > 1 ? 1 : 0);
     auto &pp = bp.group(0);
+    auto &pp1 = bp.group(p->lanes > 1 ? 1 : 0);
     std::vector<GradEntry<T>> tab(p->count);
     int64_t off = 0, maxgen = 1;
     for (int i = 0; i < p->count; ++i) {
@@ -1012,7 +1021,7 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
                 cd.g = (const cpx<T> *)gs[i];
                 cd.k = k;
                 cd.mode = CORR_FULL;
-                pc.refs.push_back({1, bc.add_corr(cd)});
+                pc1.refs.push_back({1, bc.add_corr(cd)});
             } else {
                 // chunks of columns -> partial spectra, then their sum
                 for (int64_t q = 0; q < nch; ++q) {
@@ -1025,7 +1034,7 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
                     pd.g = (const cpx<T> *)gs[i] + c0 * ldg[i];
                     pd.k = k - c0 < kCorrChunk ? k - c0 : kCorrChunk;
                     pd.mode = CORR_PARTIAL;
-                    pc.refs.push_back({1, bc.add_corr(pd)});
+                    pc1.refs.push_back({1, bc.add_corr(pd)});
                 }
                 CorrDesc<T> sd;
                 memset(&sd, 0, sizeof(sd));
@@ -1037,15 +1046,15 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
                 sd.k = nch;
                 sd.g_valid = INT64_MAX;
                 sd.mode = CORR_SUM;
-                pp.refs.push_back({1, bp.add_corr(sd)});
+                pp1.refs.push_back({1, bp.add_corr(sd)});
             }
         }
         tab[i] = GradEntry<T>{r, (cpx<T> *)grads[i], L, c->head, c->tail, c->gen_len, c->kind};
         if (c->gen_len > maxgen) maxgen = c->gen_len;
     }
     fm_status st = finalize<T>(ba, p->gpassA, fm_batch_plan::kSide + 1, s);
-    if (!st) st = finalize<T>(bc, p->gcorr, 1, s);
-    if (!st) st = finalize<T>(bp, p->gpassC, 1, s);
+    if (!st) st = finalize<T>(bc, p->gcorr, fm_batch_plan::kSide + 1, s);
+    if (!st) st = finalize<T>(bp, p->gpassC, fm_batch_plan::kSide + 1, s);
     if (st) return st;
     if (p->gtab) (void)hipFree(p->gtab);
     p->gtab = nullptr;
@@ -1165,6 +1174,7 @@ static fm_status batch_spec_bind_t(fm_batch_plan *p, const void *const *gens, ch
     ProgBuilder<T> b;
     auto &pa = b.group(0);
     auto &pb = b.group(0);
+    auto &p1 = b.group(p->lanes > 1 ? 1 : 0);  // one-pass spectra, independent of pa/pb
     int64_t off = 0, maxL = 1;
     for (int i = 0; i < p->count; ++i) {
         const fm_conv_plan *c = p->plans[i];
@@ -1180,7 +1190,7 @@ static fm_status batch_spec_bind_t(fm_batch_plan *p, const void *const *gens, ch
         if (st) return st;
         for (auto &ps : pl) ps.a.alg_bytes = 0;  // the generator read is the applies'
         if (pl.size() == 1) {
-            pb.refs.push_back({0, b.add(pl[0])});
+            p1.refs.push_back({0, b.add(pl[0])});
         } else {
             pa.refs.push_back({0, b.add(pl[0])});
             pb.refs.push_back({0, b.add(pl[1])});
@@ -1188,7 +1198,7 @@ static fm_status batch_spec_bind_t(fm_batch_plan *p, const void *const *gens, ch
         tab[i] = GenEntry{gens[i], t, L, c->head, c->tail, c->tail_src};
         if (L > maxL) maxL = L;
     }
-    fm_status st = finalize<T>(b, p->sprog, 1, s);
+    fm_status st = finalize<T>(b, p->sprog, fm_batch_plan::kSide + 1, s);
     if (st) return st;
     if (p->stab) (void)hipFree(p->stab);
     p->stab = nullptr;

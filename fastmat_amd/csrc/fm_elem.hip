@@ -222,3 +222,45 @@ extern "C" fm_status fm_axpby(int dtype, int64_t n, int64_t k, const double *alp
     FM_REQUIRE(ldx >= n && ldy >= n, FM_ERR_VALUE, "leading dimension below n");
     FM_DISPATCH_ALL(dtype, axpby_t, n, k, alpha, x, ldx, conj_x, beta, y, ldy, as_stream(stream));
 }
+
+namespace fm {
+// Plain device copy in 16-byte vectors (grid-strided, four loads in flight
+// per thread before their stores), the reference point for HBM bandwidth and
+// for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src,
+                                                      uint4 *__restrict__ dst, int64_t n) {
+    constexpr int U = 4;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    for (; i + (U - 1) * stride < n; i += U * stride) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace fm
+
+extern "C" fm_status fm_device_copy(void *dst, const void *src, int64_t bytes, void *stream) {
+    FM_REQUIRE(bytes >= 0, FM_ERR_VALUE, "negative size");
+    if (bytes == 0) return FM_SUCCESS;
+    FM_REQUIRE(dst && src && (bytes % 16) == 0 && (uintptr_t(dst) % 16) == 0 &&
+                   (uintptr_t(src) % 16) == 0,
+               FM_ERR_VALUE, "copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    const int64_t n = bytes / 16;
+    int ncu = 256;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    int64_t grid = ceil_div(n, 256);
+    if (grid > int64_t(ncu) * 8) grid = int64_t(ncu) * 8;
+    ProfToken tok;
+    prof_begin(as_stream(stream), &tok);
+    hipLaunchKernelGGL(copy16_kernel, dim3(unsigned(grid)), dim3(256), 0, as_stream(stream),
+                       (const uint4 *)src, (uint4 *)dst, n);
+    FM_CHECK_LAUNCH();
+    if (tok.active) prof_end(as_stream(stream), &tok, "copy16_kernel", 2.0 * double(bytes), 0.0);
+    return FM_SUCCESS;
+}

@@ -141,7 +141,8 @@ __device__ __forceinline__ cpx<T> twiddle_mul(cpx<T> v, cpx<T> w) {
 // possible, larger radices first.
 template <int LOGN, int LOGE, bool REV = false> struct FftPlan {
     static constexpr int S = LOGN == 0 ? 0 : (LOGN + LOGE - 1) / LOGE;
-    static constexpr int fbits(int s) { return LOGN / S + (s < LOGN % S ? 1 : 0); }
+    static constexpr int SD = S > 0 ? S : 1;  // (LOGN == 0 has no stages)
+    static constexpr int fbits(int s) { return LOGN / SD + (s < LOGN % SD ? 1 : 0); }
     // REV runs the same radices last-to-first (any order is a valid Stockham
     // factorisation); a transform whose first radix equals the previous
     // transform's last radix can start from that transform's registers.

@@ -24,6 +24,8 @@
 
 namespace fm {
 
+int knob(const char *name, int dflt);
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct GemmArgs {
@@ -149,16 +151,19 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(const GemmArgs g) {
 // owns rows [16w, 16w + 16); per k-step of 4 it issues four
 // v_mfma_f32_16x16x4f32 (one per 16-column block).  B tiles are staged
 // through LDS (coalesced along the data's contiguous axis), results through
-// LDS again so that stores are as contiguous as the layout allows.  Two
-// workgroups share a CU, so one streams while the other multiplies.
+// LDS again so that stores are as contiguous as the layout allows.  The next
+// tile's B is loaded into registers while the current one multiplies (its
+// column table in the other half of a double-buffered table), so the loads'
+// latency hides behind the MFMA work.  A in registers takes ~180 VGPRs, so
+// one workgroup runs per CU (2 waves per SIMD).
 constexpr int SK_N = 64;
 constexpr int SK_K = 128;
 constexpr int SK_T = 512;
 
 __global__ __launch_bounds__(SK_T) void gemm_smallk_kernel(const GemmArgs g) {
     __shared__ float Bs[SK_K][SK_N + 1];  // B tile, then the output tile
-    __shared__ int xcol[SK_N];
-    __shared__ int ycol[SK_N];
+    __shared__ int xcol[2][SK_N];
+    __shared__ int ycol[2][SK_N];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int s = g.cplx ? 2 : 1;
     const bool b_kfast = g.brs == 1;
@@ -166,6 +171,8 @@ __global__ __launch_bounds__(SK_T) void gemm_smallk_kernel(const GemmArgs g) {
     const int64_t ntiles = ceil_div(g.NC, SK_N);
     // element offsets fit in 32 bits (checked by the launcher)
     const int xk = int(s * g.brs), yr = int(s * g.yrs);
+    int64_t t = blockIdx.x;
+    if (t >= ntiles) return;  // uniform over the workgroup
 
     // a[st] = A_eff[16*wave + (lane & 15)][4*st + (lane >> 4)]; zero beyond M, K
     float a[SK_K / 4];
@@ -177,10 +184,10 @@ __global__ __launch_bounds__(SK_T) void gemm_smallk_kernel(const GemmArgs g) {
             a[st] = (i < g.M && k < g.K) ? g.A[i * g.a_rs + k * g.a_cs] : 0.f;
         }
     }
-    constexpr int PER = SK_K * SK_N / SK_T;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // column tables of tile tt into half h (-1: past the last column)
+    auto tables = [&](int64_t tt, int h) {
         if (tid < SK_N) {
-            const int64_t gc = t * SK_N + tid;
+            const int64_t gc = tt * SK_N + tid;
             int xb = -1, yb = -1;
             if (gc < g.NC) {
                 int64_t cc = gc;
@@ -193,31 +200,43 @@ __global__ __launch_bounds__(SK_T) void gemm_smallk_kernel(const GemmArgs g) {
                 xb = int(s * (o * g.bos + j * g.bjs) + part);
                 yb = int(s * (o * g.yos + j * g.yjs) + part);
             }
-            xcol[tid] = xb;
-            ycol[tid] = yb;
+            xcol[h][tid] = xb;
+            ycol[h][tid] = yb;
         }
-        __syncthreads();
-        {
-            float pre[PER];
+    };
+    constexpr int PER = SK_K * SK_N / SK_T;
+    // B elements of this thread for the tile whose table is in half h
+    auto load = [&](int h, float *pre) {
 #pragma unroll
-            for (int e = 0; e < PER; ++e) {
-                const int idx = tid + e * SK_T;
-                int kk, c;
-                if (b_kfast) { kk = idx & (SK_K - 1); c = idx >> 7; }
-                else { c = idx & (SK_N - 1); kk = idx >> 6; }
-                const int xb = xcol[c];
-                pre[e] = (xb >= 0 && kk < g.K) ? g.X[xb + kk * xk] : 0.f;
-            }
-#pragma unroll
-            for (int e = 0; e < PER; ++e) {
-                const int idx = tid + e * SK_T;
-                int kk, c;
-                if (b_kfast) { kk = idx & (SK_K - 1); c = idx >> 7; }
-                else { c = idx & (SK_N - 1); kk = idx >> 6; }
-                Bs[kk][c] = pre[e];
-            }
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + e * SK_T;
+            int kk, c;
+            if (b_kfast) { kk = idx & (SK_K - 1); c = idx >> 7; }
+            else { c = idx & (SK_N - 1); kk = idx >> 6; }
+            const int xb = xcol[h][c];
+            pre[e] = (xb >= 0 && kk < g.K) ? g.X[xb + kk * xk] : 0.f;
         }
+    };
+    float pre[PER];
+    tables(t, 0);
+    __syncthreads();
+    load(0, pre);
+    for (int cur = 0; t < ntiles; t += gridDim.x, cur ^= 1) {
+        // Bs is free: the previous tile's stores ended with a barrier
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + e * SK_T;
+            int kk, c;
+            if (b_kfast) { kk = idx & (SK_K - 1); c = idx >> 7; }
+            else { c = idx & (SK_N - 1); kk = idx >> 6; }
+            Bs[kk][c] = pre[e];
+        }
+        // the other half was last read two barriers ago (the previous
+        // tile's stores, the prefetch before them)
+        const int64_t tn = t + gridDim.x;
+        tables(tn, cur ^ 1);
         __syncthreads();
+        if (tn < ntiles) load(cur ^ 1, pre);
         f32x4 acc[4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -243,10 +262,10 @@ __global__ __launch_bounds__(SK_T) void gemm_smallk_kernel(const GemmArgs g) {
             int r, c;
             if (y_rfast) { r = idx & (SK_K - 1); c = idx >> 7; }
             else { c = idx & (SK_N - 1); r = idx >> 6; }
-            const int yb = ycol[c];
+            const int yb = ycol[cur][c];
             if (yb >= 0 && r < g.M) g.Y[yb + r * yr] = Bs[r][c];
         }
-        __syncthreads();  // Bs and the column tables are rewritten next iteration
+        __syncthreads();  // Bs and this half of the tables are rewritten next
     }
 }
 
@@ -388,7 +407,10 @@ static fm_status gemm_launch(const GemmArgs &g, hipStream_t stream) {
         int dev = 0, ncu = 256;
         if (hipGetDevice(&dev) == hipSuccess)
             (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        int64_t grid = int64_t(ncu) * 2;
+        // one workgroup fits a CU (A in registers: ~180 VGPRs, 2 waves per
+        // SIMD), so the grid is one workgroup per CU walking its tiles
+        static const int per_cu = knob("FASTMAT_AMD_SMALLK_WG_PER_CU", 1);
+        int64_t grid = int64_t(ncu) * (per_cu >= 1 && per_cu <= 4 ? per_cu : 1);
         if (grid > tiles) grid = tiles;
         ProfToken tok;
         prof_begin(stream, &tok);
