@@ -250,8 +250,24 @@ def workload_kron(args, rng):
         return {"fwd": accept.max_rel_error(K._apply(X[:1].contiguous(), False).cpu().numpy().T,
                                             ref)}
 
+    def cpu():
+        # the reference's term loop on the host (oracle/cpu_path.KronCPU): the
+        # Fourier and Circulant terms through numpy's FFT (one thread), the
+        # dense term through ndarray.dot (the BLAS, all its threads)
+        from oracle import cpu_path
+        from threadpoolctl import threadpool_info
+        kc = 2
+        xc = crandn(np.random.default_rng(7), (N, kc))
+        K_cpu = cpu_path.KronCPU([cpu_path.FourierCPU(64), cpu_path.circulant(c),
+                                  cpu_path.DenseCPU(A)], (64, 128, 128))
+        blas = [i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"]
+        return ([(K_cpu, xc, 2 * (2 * N * kc * 8 + 128 * 128 * 4 + 128 * 8))],
+                "%d of the %d columns, forward + backward (Kron.pyx term loop; numpy FFT on one "
+                "thread, ndarray.dot on the BLAS's %d threads)" % (kc, k, max(blas or [1])),
+                max(blas or [1]))
+
     cfg = {"workload": "kron_F64_C128_M128_c64_fwd_adj", "N": N, "cols": k}
-    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=None, cfg=cfg,
+    return dict(step=step, bytes=bytes_step, check=check, plain=None, cpu=cpu, cfg=cfg,
                 dtype="c64", metric=METRIC)
 
 
@@ -919,8 +935,10 @@ def main():
     bytes_per_launch = rec["bytes"] / rec["launches"]
     pass_bytes_per_launch = rec.get("pass_bytes", rec["bytes"]) / rec["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    if name.startswith("gemm_"):
-        # dense factor: FP32 MFMA-bound (2 M K N flop over far fewer bytes)
+    if name.startswith("gemm_") or name.startswith("kron_"):
+        # dense factor (alone or fused with the Circulant mode): FP32
+        # MFMA-bound (2 M K N flop, 32 per byte moved, above the ~20 of the
+        # FP32 MFMA / HBM ridge)
         flops_per_launch = rec["flops"] / rec["launches"]
         tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(tflops, 2), "peak": FP32_MFMA_PEAK_TFLOPS,
@@ -992,8 +1010,10 @@ def main():
 
     cpu_baseline = None
     if not args.no_cpu and w.get("cpu") is not None:
-        legs, sample = w["cpu"]()
-        cpu_baseline = {"value": round(time_cpu(legs), 4), "unit": "GB/s", "cores": 1,
+        got = w["cpu"]()
+        legs, sample = got[0], got[1]
+        cores = got[2] if len(got) > 2 else 1  # threads the leg runs on
+        cpu_baseline = {"value": round(time_cpu(legs), 4), "unit": "GB/s", "cores": cores,
                         "kind": "port", "sample": sample,
                         "host_cpus": os.cpu_count()}
 

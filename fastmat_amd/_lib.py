@@ -65,6 +65,7 @@ _PROTOTYPES = {
                                 _vp, _c_i64, _c_i64, _vp]),
     "fm_dense_apply_strided": (_c_int, [_c_int, _c_int, _c_int, _c_i64, _vp, _c_i64, _vp, _vp,
                                         _c_i64, _c_i64, _vp]),
+    "fm_kron_conv_dense_apply": (_c_int, [_vp, _c_int, _vp, _c_i64, _vp, _vp, _c_i64, _vp]),
     "fm_diag_apply": (_c_int, [_c_int, _c_int, _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _c_i64,
                                _vp]),
     "fm_gather_rows": (_c_int, [_c_int, _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _c_i64, _vp]),

@@ -9,8 +9,10 @@ other factor falls back to moving the axis to the end, applying the factor
 and moving it back.  dtype = promote(factors) with safe type expansion; the
 input is widened to it (Kron.pyx:104-129)."""
 import numpy as np
+import torch
 
 from . import _device as D
+from . import _lib
 from .matrix import Matrix
 
 
@@ -59,10 +61,23 @@ class Kron(Matrix):
     def _run(self, X, adjoint):
         k = X.shape[0]
         dims = self._dims
-        for m, f in enumerate(self._content):
+        fused = _fused_tail(self._content, X.dtype)
+        last = len(self._content) - (2 if fused else 0)
+        for m, f in enumerate(self._content[:last]):
             outer = k * int(np.prod(dims[:m], dtype=np.int64))
             inner = int(np.prod(dims[m + 1:], dtype=np.int64))
             X = _apply_mode(f, X, adjoint, outer, dims[m], inner)
+        if fused:
+            # the last two factors (Circulant(128), dense 128 x 128 float32)
+            # in one pass over the data (fm_kron_conv_dense_apply)
+            C, A = self._content[-2:]
+            Xc = X.contiguous()
+            Y = torch.empty_like(Xc)
+            outer = k * int(np.prod(dims[:-2], dtype=np.int64))
+            _lib.call("fm_kron_conv_dense_apply", C.plan_handle(np.complex64), int(adjoint),
+                      _lib.ptr(A._device_f32(transposed=adjoint)), 128, _lib.ptr(Xc),
+                      _lib.ptr(Y), outer, D.stream())
+            X = Y
         return X
 
     def _fwd(self, X):
@@ -76,6 +91,20 @@ class Kron(Matrix):
         for f in self._content[1:]:
             res = np.kron(res, f.reference())
         return res
+
+
+def _fused_tail(content, dtype):
+    """True when the product ends in a one-level Circulant of 128 points and a
+    dense 128 x 128 float32 Matrix and the data is complex64: those two modes
+    then run as one pass (the fused kernel's only shape)."""
+    from .circulant import Circulant
+    if len(content) < 2 or D.np_dtype(dtype) != np.dtype(np.complex64):
+        return False
+    C, A = content[-2:]
+    return (type(C) is Circulant and C._multi is None and C.numRows == 128 and
+            C.compute_dtype(dtype) == np.dtype(np.complex64) and
+            type(A) is Matrix and A.numRows == 128 and A.numCols == 128 and
+            A._array.dtype == np.float32)
 
 
 def _apply_mode(f, X, adjoint, outer, n, inner):

@@ -358,10 +358,19 @@ class Matrix:
         return (type(self) is Matrix and self._array.dtype == np.float32 and
                 D.np_dtype(dtype) in (np.dtype(np.float32), np.dtype(np.complex64)))
 
-    def _apply_strided(self, X, adjoint, outer, inner):
-        A_d = self._dev.get("f32")
+    def _device_f32(self, transposed=False):
+        """The float32 array on the device, contiguous row-major (or its
+        transpose), cached."""
+        key = "f32T" if transposed else "f32"
+        A_d = self._dev.get(key)
         if A_d is None:
-            A_d = self._dev["f32"] = D.as_tensor(self._array, np.float32)
+            A = np.asarray(self._array, dtype=np.float32)
+            A_d = self._dev[key] = D.as_tensor(np.ascontiguousarray(A.T if transposed else A),
+                                               np.float32)
+        return A_d
+
+    def _apply_strided(self, X, adjoint, outer, inner):
+        A_d = self._device_f32()
         Y = X.new_empty(X.shape)
         n = self.numRows
         _lib.call("fm_dense_apply_strided", D.fm_type(np.float32), D.fm_type(X.dtype),

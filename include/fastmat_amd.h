@@ -305,6 +305,18 @@ fm_status fm_dense_apply_strided(int a_dtype, int x_dtype, int adjoint,
                                  int64_t n, const void *A, int64_t lda,
                                  const void *x, void *y, int64_t outer,
                                  int64_t inner, void *stream);
+/* The last two factors of a Kron product in one pass (replaces their two
+ * mode passes of Kron._forward / _backward, Kron.pyx:267-303): x is viewed
+ * as (outer, 128, 128) complex64 slabs, contiguous, the last axis fastest;
+ * y = (C (x) M) x per slab, with C the 128-point Circulant plan (complex64,
+ * one level) along the middle axis, its spectrum conjugated when
+ * conj_circulant is set, and M a 128 x 128 row-major float32 matrix
+ * (ldm = 128) along the last axis.  The adjoint of (C (x) A) for a real A is
+ * conj_circulant = 1 with M = A^T.  x and y must not overlap; y 16-byte
+ * aligned. */
+fm_status fm_kron_conv_dense_apply(const fm_conv_plan *plan, int conj_circulant,
+                                   const void *M, int64_t ldm, const void *x,
+                                   void *y, int64_t outer, void *stream);
 
 /* ---------------------------------------------------------- elementwise
  * All element types; operands share `dtype`. */

@@ -14,6 +14,8 @@ it takes on the host is what fastmat would take for the same call.
   (Partial.pyx:268-280), ``astype`` (Product.pyx:215), fft, diagonal multiply
   (Diag.pyx:149-157), conj/fft/conj (Fourier.pyx:235-238), row gather.
 * Hadamard: the C restatement of the typed radix-2 loop (oracle/fwht.c).
+* Kron: the term loop of Kron.pyx:267-339 with its reshapes; dense terms are
+  ``ndarray.dot`` (Matrix.pyx:1831-1842, 1924-1935).
 """
 import numpy as np
 
@@ -119,6 +121,51 @@ def toeplitz(vecC, vecR, optimize=True):
     t = np.concatenate([vecC, np.zeros(L - d, dtype=vecC.dtype), vecR])
     # Toeplitz builds its Fourier with optimize=False (Toeplitz.pyx:219-220)
     return ConvCPU(t, m, n, fourier_optimize=False)
+
+
+class DenseCPU:
+    """Matrix._forward / _backward (Matrix.pyx:1831-1842, 1924-1935):
+    ``ndarray.dot`` with the array and its conjugate transpose."""
+
+    def __init__(self, A):
+        self.A = np.asarray(A)
+        self.AH = self.A.T.conj()
+
+    def forward(self, X):
+        return self.A.dot(X)
+
+    def backward(self, X):
+        return self.AH.dot(X)
+
+
+class KronCPU:
+    """Kron._forward / _backward (Kron.pyx:267-339): for every term in order,
+    reshape the data to (head, rest) in C order, to (term rows, rest') in
+    Fortran order, apply the term, reshape back in Fortran order; a last C
+    order reshape to (N, k).  ``terms`` have forward/backward on 2-D arrays."""
+
+    def __init__(self, terms, dims):
+        self.terms = list(terms)
+        self.dims = tuple(int(d) for d in dims)
+        self.N = int(np.prod(self.dims))
+
+    def _run(self, X, backward):
+        N, k = self.N, X.shape[1]
+        data = X
+        head = 1
+        for term, d in zip(self.terms, self.dims):
+            head *= d
+            data = np.reshape(data, (head, k * N // head), order="C")
+            data = np.reshape(data, (d, k * N // d), order="F")
+            data = term.backward(data) if backward else term.forward(data)
+            data = np.reshape(data, (head, k * N // head), order="F")
+        return np.reshape(data, (N, k), order="C")
+
+    def forward(self, X):
+        return self._run(X, False)
+
+    def backward(self, X):
+        return self._run(X, True)
 
 
 def hadamard_forward(X):

@@ -314,6 +314,44 @@ def test_kron_config4_shape_small_k(fm):
     assert relerr(K.backward(x), refb) < 1e-5
 
 
+@pytest.mark.parametrize("lead,k", [((), 3), ((4,), 1), ((2, 8), 5)])
+def test_kron_fused_circulant_dense_tail(fm, lead, k):
+    """Kron products ending in Circulant(128) x dense 128 x 128 float32 run
+    those two modes as one pass (fm_kron_conv_dense_apply): forward and
+    backward against the FP64 mode-wise oracle, with a complex generator and
+    0, 1 or 2 leading factors (Fourier, Hadamard)."""
+    import torch
+    from fastmat_amd import kron as fkron
+    rng = np.random.default_rng(40 + len(lead))
+    A = (rng.standard_normal((128, 128)) / np.sqrt(128)).astype(np.float32)
+    c = crandn(rng, 128, dtype=np.complex64)
+    heads = [fm.Fourier(lead[0])] if len(lead) >= 1 else []
+    if len(lead) == 2:
+        heads = [fm.Hadamard(1), fm.Fourier(lead[1])]
+    K = fm.Kron(*(heads + [fm.Circulant(c), fm.Matrix(A)])) if heads else \
+        fm.Kron(fm.Circulant(c), fm.Matrix(A))
+    assert fkron._fused_tail(K._content, torch.complex64)
+    dims = tuple((2,) if len(lead) == 2 else ()) + tuple(lead[-1:]) + (128, 128)
+    n = int(np.prod(dims))
+    x = crandn(rng, n, k, dtype=np.complex64)
+    say("kron fused tail", dims, "k", k)
+    C64 = sla.circulant(c.astype(np.complex128))
+    lead_ops = []
+    if len(lead) == 2:
+        lead_ops.append(lambda v: dense.hadamard(1) @ v)
+    if lead:
+        lead_ops.append(lambda v: fast64.fourier(v))
+    ops = lead_ops + [lambda v: C64 @ v, lambda v: A.astype(np.float64) @ v]
+    assert relerr(K.forward(x), fast64.kron_apply(ops, dims, x)) < 1e-5
+    lead_b = []
+    if len(lead) == 2:
+        lead_b.append(lambda v: dense.hadamard(1) @ v)
+    if lead:
+        lead_b.append(lambda v: fast64.fourier(v, adjoint=True))
+    opsb = lead_b + [lambda v: C64.conj().T @ v, lambda v: A.T.astype(np.float64) @ v]
+    assert relerr(K.backward(x), fast64.kron_apply(opsb, dims, x)) < 1e-5
+
+
 # ---------------------------------------------- composition and elementwise
 def test_composition(fm):
     rng = np.random.default_rng(5)

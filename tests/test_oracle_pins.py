@@ -696,3 +696,23 @@ def test_conv_generator_grad_brute_force_and_fd(kind, m, n):
             tm[q] -= h * unit
             fd[q] += unit * (loss(tp) - loss(tm)) / (2 * h)
     np.testing.assert_allclose(got, fd, rtol=0, atol=1e-6 * np.abs(fd).max())
+
+
+def test_kron_cpu_path_matches_dense_kron():
+    """oracle.cpu_path.KronCPU (the reference's term loop with its C / Fortran
+    order reshapes) against np.kron of the dense factors, forward and
+    backward, on Fourier(4) x Circulant(8) x dense 5 x 5."""
+    import scipy.linalg as sla
+    from oracle import cpu_path
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((5, 5)).astype(np.float32)
+    c = (rng.standard_normal(8) + 1j * rng.standard_normal(8)).astype(np.complex64)
+    K = cpu_path.KronCPU([cpu_path.FourierCPU(4), cpu_path.circulant(c), cpu_path.DenseCPU(A)],
+                         (4, 8, 5))
+    ref = np.kron(np.kron(dense.fourier(4), sla.circulant(c.astype(np.complex128))),
+                  A.astype(np.float64))
+    x = rng.standard_normal((160, 3)) + 1j * rng.standard_normal((160, 3))
+    y = ref @ x
+    assert np.abs(K.forward(x) - y).max() <= 1e-12 * np.abs(y).max()
+    yb = ref.conj().T @ x
+    assert np.abs(K.backward(x) - yb).max() <= 1e-12 * np.abs(yb).max()
