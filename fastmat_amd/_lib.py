@@ -84,6 +84,11 @@ _PROTOTYPES = {
                                         ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
                                         ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
                                         ctypes.c_double, _vp, _c_i64, _vp]),
+    "fm_batch_bind_lsq": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
+                                   ctypes.c_double, _vp, _c_i64, _vp]),
     "fm_batch_residual_loss": (_c_int, [_vp, _vp, _vp]),
     "fm_batch_grad_workspace_size": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
     "fm_batch_grad_bind": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),

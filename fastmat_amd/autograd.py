@@ -232,21 +232,28 @@ def batch_conv_matvec_packed(batch, gens, xp):
 
 
 class _BatchLeastSquares(torch.autograd.Function):
-    """loss = sum_i ||A_i(gen_i) x_i - y_i||^2 over a packed batch, with the
-    residual formed and summed inside the forward apply's last pass
-    (fm_batch_bind_residual): the forward stores 2 (A x - y), which is
-    dLoss/d(A x), so the backward is the adjoint apply of it and the batched
-    generator gradient, reusing the column transforms the two applies leave
-    in the workspace."""
+    """loss = sum_i ||A_i(gen_i) x_i - y_i||^2 over a packed batch in one
+    least-squares binding of the batch (fm_batch_bind_lsq): the forward forms
+    the residual 2 (A x - y) (dLoss/d(A x)) and the loss inside its last pass,
+    and each multi-pass operator's last pass also runs the adjoint's column
+    transform of that residual; the backward is the rest of the adjoint
+    apply and the batched generator gradient, reusing the column transforms
+    of x and of the residual that the forward left in the workspace."""
 
     @staticmethod
-    def forward(ctx, batch, xp, yp, *gens):
+    def _bind(batch, xp, yp):
         r_buf, _, gx_buf = _packed_buffers(batch, xp.dtype, xp.device)
-        ctx.tok = batch.load_generators(gens)
         X = batch.packed_views(xp, adjoint=False, inputs=True)
         Y = batch.packed_views(yp, adjoint=False, inputs=False)
         R = batch.packed_views(r_buf, adjoint=False, inputs=False)
-        batch.bind_residual(X, Y, R, scale=2.0)
+        GX = batch.packed_views(gx_buf, adjoint=False, inputs=True)
+        batch.bind_lsq(X, Y, R, GX, scale=2.0)
+        return X, R, gx_buf
+
+    @staticmethod
+    def forward(ctx, batch, xp, yp, *gens):
+        ctx.tok = batch.load_generators(gens)
+        _BatchLeastSquares._bind(batch, xp, yp)
         batch.launch(False)
         ctx.fwd_count = batch.apply_count[0]
         loss = getattr(batch, "_lsq_loss", None)
@@ -256,7 +263,7 @@ class _BatchLeastSquares(torch.autograd.Function):
         ctx.batch = batch
         ctx.gen_complex = [g.is_complex() for g in gens]
         ctx.gen_dtype = [g.dtype for g in gens]
-        ctx.save_for_backward(xp, *gens)
+        ctx.save_for_backward(xp, yp, *gens)
         real_t = torch.float32 if xp.dtype == torch.complex64 else torch.float64
         return loss[0].to(real_t)
 
@@ -265,23 +272,23 @@ class _BatchLeastSquares(torch.autograd.Function):
         batch = ctx.batch
         n = len(batch.ops)
         saved = ctx.saved_tensors
-        xp, gens = saved[0], saved[1:]
-        r_buf, _, gx_buf = _packed_buffers(batch, xp.dtype, xp.device)
-        G = batch.packed_views(r_buf, adjoint=False, inputs=False)  # 2 (A x - y)
-        gxp = None
-        reuse = False
-        if ctx.needs_input_grad[1] or any(ctx.needs_input_grad[3:]):
-            if batch.generator_token != ctx.tok:
-                batch.load_generators(gens)
-            batch.apply(G, batch.packed_views(gx_buf, adjoint=False, inputs=True), adjoint=True)
-            reuse = batch.apply_count[0] == ctx.fwd_count
-        scale = gl.to(xp.dtype)
-        if ctx.needs_input_grad[1]:
-            gxp = gx_buf * scale
+        xp, yp, gens = saved[0], saved[1], saved[2:]
+        need_x, need_g = ctx.needs_input_grad[1], any(ctx.needs_input_grad[3:])
         ggen = [None] * n
-        if any(ctx.needs_input_grad[3:]):
-            X = batch.packed_views(xp, adjoint=False, inputs=True)
-            raw = batch.generator_grads(X, G, reuse=reuse, scale=scale)
+        if not (need_x or need_g):
+            return (None, None, None, *ggen)
+        X, R, gx_buf = _BatchLeastSquares._bind(batch, xp, yp)
+        if batch.generator_token != ctx.tok or batch.apply_count[0] != ctx.fwd_count:
+            # another forward ran on this batch since: run this step's
+            # forward again (its last pass holds the adjoint's column
+            # transforms of the residual)
+            batch.load_generators(gens)
+            batch.launch(False)
+        batch.launch(True)
+        scale = gl.to(xp.dtype)
+        gxp = gx_buf * scale if need_x else None
+        if need_g:
+            raw = batch.generator_grads(X, R, reuse=True, scale=scale)
             ggen = [(r if r.dtype == dt else r.to(dt)) if cplx else r.real.to(dt)
                     for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
         return (None, gxp, None, *ggen)

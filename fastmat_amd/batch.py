@@ -285,6 +285,45 @@ class ConvBatch:
         self._keep = getattr(self, "_keep", {})
         self._keep[0] = (list(ins), list(targets), list(outs))
 
+    def bind_lsq(self, ins, targets, residuals, grads_x, scale=2.0):
+        """Bind both directions for a least-squares step
+        (fm_batch_bind_lsq): the forward in residual mode (ins, targets,
+        residuals), the adjoint on the residuals into grads_x, with each
+        multi-pass operator's forward column inverse and adjoint column
+        transform fused (its residual is formed on the chip and never
+        written).  launch(False) then launch(True) run the step's applies;
+        generator_grads(ins, residuals, reuse=True) its gradients."""
+        key = ("lsq", float(scale)) + tuple(t.data_ptr() for t in ins) + \
+            tuple(t.data_ptr() for t in targets) + tuple(t.data_ptr() for t in residuals) + \
+            tuple(t.data_ptr() for t in grads_x)
+        if self._bound[0] == key and self._bound[1] == key:
+            return
+        for i, (op, k) in enumerate(zip(self.ops, self.ks)):
+            shapes = ((ins[i], op.numCols), (targets[i], op.numRows), (residuals[i], op.numRows),
+                      (grads_x[i], op.numCols))
+            for t, w in shapes:
+                if tuple(t.shape) != (k, w):
+                    raise ValueError("ConvBatch: buffer %d has the wrong shape" % i)
+                if not t.is_contiguous():
+                    raise ValueError("ConvBatch: buffers must be contiguous blocks")
+                if D.np_dtype(t.dtype) != self.dtype:
+                    raise TypeError("ConvBatch: buffers must be %s" % self.dtype)
+        n = len(self.ops)
+
+        def ptrs(ts):
+            return (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
+
+        def lds(ts):
+            return (ctypes.c_int64 * n)(*[t.shape[1] for t in ts])
+        ws = None if self._ws is None else _lib.ptr(self._ws)
+        _lib.call("fm_batch_bind_lsq", self._h, ptrs(ins), lds(ins), ptrs(targets), lds(targets),
+                  ptrs(residuals), lds(residuals), ptrs(grads_x), lds(grads_x), float(scale), ws,
+                  self._ws_bytes, D.stream())
+        self._bound[0] = self._bound[1] = key
+        self._keep = getattr(self, "_keep", {})
+        self._keep[0] = (list(ins), list(targets), list(residuals), list(grads_x))
+        self._keep[1] = self._keep[0]
+
     def residual_loss(self, out):
         """Write the residual-mode loss of the last forward apply to the
         device float64 tensor ``out`` (one element)."""

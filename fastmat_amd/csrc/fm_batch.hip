@@ -677,6 +677,48 @@ extern "C" fm_status fm_batch_set_schedule(fm_batch_plan *p, int workgroups_per_
 // ======================================================================
 //                       forward / adjoint programs
 // ======================================================================
+// Pass groups of a direction: the one-pass operators on a lane of their own
+// (when there is one to spare); the multi-pass operators, longest transform
+// first, in groups whose intermediates (T1 + T2) stay within the budget, each
+// group's phases (pass q of every operator of the group) in order on one
+// lane, the groups alternating between the lanes.
+template <typename T>
+static void group_pipes(const fm_batch_plan *p, std::vector<PassList<T>> &pipes,
+                        ProgBuilder<T> &b) {
+    std::vector<int> multi, single;
+    for (int i = 0; i < p->count; ++i) (pipes[i].size() > 1 ? multi : single).push_back(i);
+    std::stable_sort(multi.begin(), multi.end(), [&](int a, int c) {
+        return p->plans[a]->L > p->plans[c]->L;
+    });
+    const int64_t budget = int64_t(p->group_mb) << 20;
+    const int lanes = p->lanes;
+    if (!single.empty()) {
+        auto &pg = b.group(lanes > 1 ? lanes : 0);
+        for (int i : single) pg.refs.push_back({0, b.add(pipes[i][0])});
+    }
+    size_t q = 0;
+    int gi = 0;
+    while (q < multi.size()) {
+        size_t r = q;
+        int64_t used = 0;
+        while (r < multi.size()) {
+            const fm_conv_plan *c = p->plans[multi[r]];
+            const int64_t need = 2 * int64_t(sizeof(cpx<T>)) * c->L * p->ks[multi[r]];
+            if (r > q && budget > 0 && used + need > budget) break;
+            used += need;
+            ++r;
+        }
+        const int lane = gi % lanes;
+        const int phases = int(pipes[multi[q]].size());
+        for (int ph = 0; ph < phases; ++ph) {
+            auto &pg = b.group(lane);
+            for (size_t t = q; t < r; ++t) pg.refs.push_back({0, b.add(pipes[multi[t]][ph])});
+        }
+        q = r;
+        ++gi;
+    }
+}
+
 template <typename T>
 static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *ins,
                               const int64_t *ld_in, void *const *outs, const int64_t *ld_out,
@@ -742,41 +784,8 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
         }
         p->n_res = slots;
     }
-    // groups of multi-pass operators, longest transform first, whose
-    // intermediates (T1 + T2) stay within the budget
-    std::vector<int> multi, single;
-    for (int i = 0; i < p->count; ++i) (p->plans[i]->geo.passes > 1 ? multi : single).push_back(i);
-    std::stable_sort(multi.begin(), multi.end(), [&](int a, int b) {
-        return p->plans[a]->L > p->plans[b]->L;
-    });
-    const int64_t budget = int64_t(p->group_mb) << 20;
     ProgBuilder<T> b;
-    const int lanes = p->lanes;
-    // one-pass operators: a lane of their own when there is one to spare
-    if (!single.empty()) {
-        auto &pg = b.group(lanes > 1 ? lanes : 0);
-        for (int i : single) pg.refs.push_back({0, b.add(pipes[i][0])});
-    }
-    size_t q = 0;
-    int gi = 0;
-    while (q < multi.size()) {
-        size_t r = q;
-        int64_t used = 0;
-        while (r < multi.size()) {
-            const fm_conv_plan *c = p->plans[multi[r]];
-            const int64_t need = 2 * int64_t(sizeof(cpx<T>)) * c->L * p->ks[multi[r]];
-            if (r > q && budget > 0 && used + need > budget) break;
-            used += need;
-            ++r;
-        }
-        const int lane = gi % lanes;
-        for (int ph = 0; ph < 3; ++ph) {
-            auto &pg = b.group(lane);
-            for (size_t t = q; t < r; ++t) pg.refs.push_back({0, b.add(pipes[multi[t]][ph])});
-        }
-        q = r;
-        ++gi;
-    }
+    group_pipes<T>(p, pipes, b);
     fm_status st = finalize<T>(b, p->dir[dd], fm_batch_plan::kSide + 1, s);
     if (st) return st;
     if (dd == 0) p->res_bound = targets != nullptr;
@@ -819,6 +828,118 @@ extern "C" fm_status fm_batch_bind_residual(fm_batch_plan *p, const void *const 
 }
 
 namespace fm {
+// Both directions of a least-squares step (fm_batch_bind_lsq).  Forward:
+// the one-pass operators store scale * (A x - t) (residual epilogue); the
+// multi-pass operators run passes A and B, then one fused pass that is the
+// forward's column inverse (pass C) and, on the same columns, the adjoint's
+// column transform (pass A) of the residual: between the two transforms the
+// tile forms scale * (A x - t) (0 beyond the m rows) and its |A x - t|^2,
+// and it stores the adjoint's T1 directly, so the residual of a multi-pass
+// operator is never written.  Adjoint: the one-pass operators read the
+// residual blocks; the multi-pass operators start at pass B.
+template <typename T>
+static fm_status batch_bind_lsq_t(fm_batch_plan *p, const void *const *ins, const int64_t *ld_in,
+                                  const void *const *targets, const int64_t *ld_t,
+                                  void *const *routs, const int64_t *ld_r, void *const *gouts,
+                                  const int64_t *ld_g, double scale, void *ws, hipStream_t s) {
+    std::vector<PassList<T>> fw(p->count), ad(p->count);
+    const double esz = double(sizeof(cpx<T>));
+    for (int i = 0; i < p->count; ++i) {
+        const fm_conv_plan *c = p->plans[i];
+        const int64_t k = p->ks[i], m = c->rows, n = c->cols;
+        FM_REQUIRE(ins[i] && targets[i] && routs[i] && gouts[i], FM_ERR_VALUE,
+                   "batch entry %d: NULL buffer", i);
+        FM_REQUIRE(ld_in[i] >= n && ld_r[i] >= m && ld_g[i] >= n && ld_t[i] == ld_r[i],
+                   FM_ERR_VALUE,
+                   "batch entry %d: leading dimension too small, or targets laid out unlike "
+                   "the residual",
+                   i);
+        const bool multi = c->geo.passes > 1;
+        FM_REQUIRE(!multi || ws != nullptr, FM_ERR_VALUE, "batch needs a workspace");
+        for (int dir = 0; dir < 2; ++dir) {
+            PipeIO io;
+            memset(&io, 0, sizeof(io));
+            io.x = dir ? (const void *)routs[i] : ins[i];
+            io.ldx = dir ? ld_r[i] : ld_in[i];
+            io.y = dir ? gouts[i] : routs[i];
+            io.ldy = dir ? ld_g[i] : ld_r[i];
+            io.k = k;
+            io.n_in = dir ? m : n;
+            io.m_out = dir ? n : m;
+            io.D = c->D;
+            io.conj_d = dir;
+            if (multi) io.work = (char *)ws + p->t1_off[dir][i];
+            PassList<T> &pl = dir ? ad[i] : fw[i];
+            fm_status st = build_pipeline<T>(c->geo, io, &pl);
+            if (st) return st;
+            for (auto &ps : pl)
+                ps.a.alg_bytes = double((n + m) * k + c->gen_len) * esz / double(pl.size());
+            if (multi) {
+                cpx<T> *t2 = (cpx<T> *)((char *)ws + p->t2_off[dir][i]);
+                pl[1].a.y = t2;
+                pl[2].a.x = t2;
+            }
+        }
+        if (multi) {
+            // the fused pass: pass C's input side and inverse transform, the
+            // residual, pass A's forward transform, twiddle and output side
+            TileArgs<T> f = fw[i][2].a;
+            const TileArgs<T> &pa = ad[i][0].a;
+            f.y = pa.y;
+            f.out_os = pa.out_os; f.out_js = pa.out_js; f.out_ns = pa.out_ns;
+            f.out_gn = pa.out_gn; f.out_gj = pa.out_gj; f.m_valid = pa.m_valid;
+            f.post = pa.post; f.conj_out = pa.conj_out; f.seqfast_out = pa.seqfast_out;
+            f.twb_hi = pa.twb_hi; f.twb_lo = pa.twb_lo;
+            f.twb_shift = pa.twb_shift; f.twb_sign = pa.twb_sign;
+            f.out_scale = T(1);
+            f.mid_t = (const cpx<T> *)targets[i];
+            f.mid_os = ld_t[i];
+            f.mid_gn = fw[i][2].a.out_gn;
+            f.mid_gj = fw[i][2].a.out_gj;
+            f.mid_valid = m;
+            f.mid_scale = T(scale);
+            f.alg_bytes = fw[i][2].a.alg_bytes + pa.alg_bytes;
+            fw[i][2] = PassSpec<T>{f, true};
+            ad[i].pop_front();
+        } else {
+            TileArgs<T> &a = fw[i][0].a;
+            a.target = (const cpx<T> *)targets[i];
+            a.out_scale = T(scale);
+        }
+    }
+    // residual slots: one per tile of every operator's residual pass
+    int64_t slots = 0;
+    std::vector<int64_t> base(p->count);
+    for (int i = 0; i < p->count; ++i) {
+        TileArgs<T> a = fw[i].back().a;
+        fm_status st = normalize_pass(a);
+        if (st) return st;
+        base[i] = slots;
+        slots += a.ntiles;
+    }
+    if (p->res_part) (void)hipFree(p->res_part);
+    p->res_part = nullptr;
+    FM_CHECK_HIP(hipMalloc(&p->res_part, sizeof(double) * size_t(slots > 0 ? slots : 1)));
+    for (int i = 0; i < p->count; ++i) {
+        fw[i].back().a.res_part = p->res_part;
+        fw[i].back().a.res_base = base[i];
+    }
+    p->n_res = slots;
+    ProgBuilder<T> bf, ba;
+    group_pipes<T>(p, fw, bf);
+    group_pipes<T>(p, ad, ba);
+    fm_status st = finalize<T>(bf, p->dir[0], fm_batch_plan::kSide + 1, s);
+    if (!st) st = finalize<T>(ba, p->dir[1], fm_batch_plan::kSide + 1, s);
+    if (st) return st;
+    p->res_bound = true;
+    for (int d = 0; d < 2; ++d) p->ws_bound[d] = ws;
+    p->ins_bound[0].assign(ins, ins + p->count);
+    p->ld_bound[0].assign(ld_in, ld_in + p->count);
+    p->ins_bound[1].assign(routs, routs + p->count);
+    p->ld_bound[1].assign(ld_r, ld_r + p->count);
+    return FM_SUCCESS;
+}
+
 // loss = sum of the tile partial sums, in slot order (deterministic).
 __global__ __launch_bounds__(256) void residual_sum_kernel(const double *__restrict__ part,
                                                            int64_t n, double *__restrict__ out) {
@@ -834,6 +955,24 @@ __global__ __launch_bounds__(256) void residual_sum_kernel(const double *__restr
     if (threadIdx.x == 0) *out = red[0];
 }
 }  // namespace fm
+
+extern "C" fm_status fm_batch_bind_lsq(fm_batch_plan *p, const void *const *ins,
+                                       const int64_t *ld_in, const void *const *targets,
+                                       const int64_t *ld_t, void *const *residuals,
+                                       const int64_t *ld_r, void *const *grads_x,
+                                       const int64_t *ld_gx, double scale, void *workspace,
+                                       int64_t workspace_bytes, void *stream) {
+    FM_REQUIRE(p && ins && ld_in && targets && ld_t && residuals && ld_r && grads_x && ld_gx,
+               FM_ERR_VALUE, "NULL argument");
+    FM_REQUIRE(workspace_bytes >= p->ws_bytes, FM_ERR_VALUE,
+               "workspace of %lld bytes is smaller than the %lld required",
+               (long long)workspace_bytes, (long long)p->ws_bytes);
+    if (p->prec == FM_COMPLEX64)
+        return batch_bind_lsq_t<float>(p, ins, ld_in, targets, ld_t, residuals, ld_r, grads_x,
+                                       ld_gx, scale, workspace, as_stream(stream));
+    return batch_bind_lsq_t<double>(p, ins, ld_in, targets, ld_t, residuals, ld_r, grads_x, ld_gx,
+                                    scale, workspace, as_stream(stream));
+}
 
 extern "C" fm_status fm_batch_residual_loss(const fm_batch_plan *p, double *loss, void *stream) {
     FM_REQUIRE(p != nullptr && loss != nullptr, FM_ERR_VALUE, "NULL argument");

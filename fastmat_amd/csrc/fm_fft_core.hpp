@@ -199,6 +199,15 @@ template <typename T> struct TileArgs {
     const cpx<T> *target;
     double *res_part;
     int64_t res_base;
+    // optional residual between the two transforms of a conv-mode pass, in
+    // place of the spectrum (mid_t != NULL): element n of sequence (o, j)
+    // of the first transform's output stands at index g = n*mid_gn + j*mid_gj
+    // of column o; for g < mid_valid it becomes mid_scale * (u - t) with
+    // t = mid_t[o*mid_os + g] (|u - t|^2 adds to the tile's residual sum
+    // res_part[res_base + tile]), beyond mid_valid it becomes 0
+    const cpx<T> *mid_t;
+    int64_t mid_os, mid_gn, mid_gj, mid_valid;
+    T mid_scale;
     // host only: this pass's share of its transform's algorithmic bytes (the
     // input and output blocks and the generator, split over the passes),
     // for the library's profiler; 0 = count the pass's own reads and writes
@@ -406,6 +415,28 @@ __device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile
     }
 }
 
+// The tile's sum of the threads' res2 (sums of |u - t|^2) into
+// a.res_part[a.res_base + tile]: shuffle within each wave, one value per
+// wave through LDS, summed in wave order by thread 0 (deterministic).  Every
+// thread of the tile calls it; it waits for the tile's earlier LDS reads.
+template <typename T, int NT>
+__device__ __forceinline__ void tile_residual_sum(const TileArgs<T> &a, int64_t tile, double res2,
+                                                  cpx<T> *lds) {
+    double sum = res2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+    __syncthreads();  // the tile's LDS data is no longer read
+    double *red = reinterpret_cast<double *>(lds);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < NT / 64; ++q) t += red[q];
+        a.res_part[a.res_base + tile] = t;
+    }
+    __syncthreads();  // red is read before the LDS is written again
+}
+
 // Store the last stage's outputs of one tile: v[e*RL + r] holds element
 // i + r*N/RL (RL = 2^LOGRL, the radix of the stage that produced them) of
 // sequence b in mapping cur_sf; re-stage through LDS when the store wants the
@@ -495,23 +526,7 @@ __device__ __forceinline__ void fft_tile_store(const TileArgs<T> &a, int64_t til
             stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
         }
     }
-    if (a.target) {
-        // the tile's sum of |u - t|^2: shuffle within each wave, one value
-        // per wave through LDS, summed in wave order by thread 0
-        double sum = res2;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
-        __syncthreads();  // the tile's LDS data is no longer read
-        double *red = reinterpret_cast<double *>(lds);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int q = 0; q < NT / 64; ++q) t += red[q];
-            a.res_part[a.res_base + tile] = t;
-        }
-        __syncthreads();  // red is read before the LDS is written again
-    }
+    if (a.target) tile_residual_sum<T, NT>(a, tile, res2, lds);
 }
 
 // Everything after the stage-0 loads of one tile: conj / pre-multiply, the
@@ -589,17 +604,47 @@ __device__ __forceinline__ void fft_tile_compute(const TileArgs<T> &a, int64_t t
             constexpr int RF = 1 << P::bits(P::S - 1);
             constexpr int NBF = E / RF;
             constexpr int LOGNRF = LOGN - P::bits(P::S - 1), NRF = 1 << LOGNRF;
+            if (a.mid_t) {
+                // residual in place of the spectrum (see TileArgs::mid_t)
+                const Ix mv = idx_bound<Ix>(a.mid_valid);
+                double res2 = 0.0;
 #pragma unroll
-            for (int e = 0; e < NBF; ++e) {
-                int b, i;
-                bfly_pos<LOGB, LOGNRF>(threadIdx.x + e * NT, cur_sf, b, i);
-                const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
-                if (!sp.ok) continue;
+                for (int e = 0; e < NBF; ++e) {
+                    int b, i;
+                    bfly_pos<LOGB, LOGNRF>(threadIdx.x + e * NT, cur_sf, b, i);
+                    const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
+                    if (!sp.ok) continue;
+                    const Ix si = Ix(s0 + b);
+                    const Ix o = a.logJ >= 0 ? Ix(si >> a.logJ) : Ix(si / Ix(a.J));
+                    const Ix tb = o * Ix(a.mid_os);
 #pragma unroll
-                for (int r = 0; r < RF; ++r) {
-                    const Ix n = Ix(i + r * NRF);
-                    const cpx<T> d = ldg(a.D, Ix(n * Ix(a.d_gn) + sp.j * Ix(a.d_gj)));
-                    v[e * RF + r] = a.conj_d ? cmulc(v[e * RF + r], d) : cmul(v[e * RF + r], d);
+                    for (int r = 0; r < RF; ++r) {
+                        const Ix n = Ix(i + r * NRF);
+                        const Ix g = n * Ix(a.mid_gn) + sp.j * Ix(a.mid_gj);
+                        cpx<T> u = mk(T(0), T(0));
+                        if (g < mv) {
+                            const cpx<T> d = v[e * RF + r] - ldg(a.mid_t, Ix(tb + g));
+                            res2 += double(d.x) * double(d.x) + double(d.y) * double(d.y);
+                            u = scale(d, a.mid_scale);
+                        }
+                        v[e * RF + r] = u;
+                    }
+                }
+                tile_residual_sum<T, NT>(a, tile, res2, lds);
+            } else {
+#pragma unroll
+                for (int e = 0; e < NBF; ++e) {
+                    int b, i;
+                    bfly_pos<LOGB, LOGNRF>(threadIdx.x + e * NT, cur_sf, b, i);
+                    const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
+                    if (!sp.ok) continue;
+#pragma unroll
+                    for (int r = 0; r < RF; ++r) {
+                        const Ix n = Ix(i + r * NRF);
+                        const cpx<T> d = ldg(a.D, Ix(n * Ix(a.d_gn) + sp.j * Ix(a.d_gj)));
+                        v[e * RF + r] =
+                            a.conj_d ? cmulc(v[e * RF + r], d) : cmul(v[e * RF + r], d);
+                    }
                 }
             }
         }

@@ -51,7 +51,10 @@ template <typename T> inline bool fits_idx32(const TileArgs<T> &a, int64_t N) {
            below(double(O1) * a.out_os + double(J1) * a.out_js + double(N1) * a.out_ns) &&
            below(double(N1) * a.in_gn + double(J1) * a.in_gj) &&
            below(double(N1) * a.out_gn + double(J1) * a.out_gj) &&
-           below(double(N1) * a.d_gn + double(J1) * a.d_gj) && below(double(a.J) * double(N));
+           below(double(N1) * a.d_gn + double(J1) * a.d_gj) && below(double(a.J) * double(N)) &&
+           (a.mid_t == nullptr ||
+            (a.mid_os >= 0 && a.mid_gn >= 0 && a.mid_gj >= 0 &&
+             below(double(O1) * a.mid_os + double(N1) * a.mid_gn + double(J1) * a.mid_gj)));
 }
 
 // Fill in the tile shape (batch, tile count, log2 J) of a pass.
@@ -129,6 +132,10 @@ template <typename T> struct PassList {
     PassSpec<T> &operator[](int i) { return v[i]; }
     const PassSpec<T> &operator[](int i) const { return v[i]; }
     PassSpec<T> &back() { return v[n - 1]; }
+    void pop_front() {
+        for (int i = 1; i < n; ++i) v[i - 1] = v[i];
+        --n;
+    }
     PassSpec<T> *begin() { return v; }
     PassSpec<T> *end() { return v + n; }
     const PassSpec<T> *begin() const { return v; }

@@ -249,6 +249,26 @@ fm_status fm_batch_bind_residual(fm_batch_plan *plan, const void *const *inputs,
                                  const int64_t *ld_out, double scale, void *workspace,
                                  int64_t workspace_bytes, void *stream);
 fm_status fm_batch_residual_loss(const fm_batch_plan *plan, double *loss, void *stream);
+/* Both directions of a least-squares step at once: the forward direction in
+ * residual mode on inputs x_i and targets t_i (as fm_batch_bind_residual,
+ * residual blocks r_i with the targets' leading dimension), the adjoint
+ * direction on inputs r_i with outputs gx_i = A_i^H r_i.  Each multi-pass
+ * operator's forward column inverse and the adjoint's column transform of
+ * its residual run as one pass (the residual is formed between the two
+ * transforms and never written: r_i holds the residual of the one-pass
+ * operators only), so after fm_batch_apply(plan, 0) (and
+ * fm_batch_residual_loss) fm_batch_apply(plan, 1) starts at the adjoint's
+ * row pass.  fm_batch_grad_apply_reuse then gives the generator gradients
+ * (the forward's and the adjoint's column transforms are in the workspace);
+ * fm_batch_grad_apply without reuse is not valid on these bindings.
+ * Backward of Sum_i ||A_i x_i - t_i||^2 (Circulant / Toeplitz backward,
+ * Circulant.pyx:104-137, Toeplitz.pyx:211-316, applied to the residual). */
+fm_status fm_batch_bind_lsq(fm_batch_plan *plan, const void *const *inputs,
+                            const int64_t *ld_in, const void *const *targets,
+                            const int64_t *ld_t, void *const *residuals,
+                            const int64_t *ld_r, void *const *grads_x,
+                            const int64_t *ld_gx, double scale, void *workspace,
+                            int64_t workspace_bytes, void *stream);
 /* Batched generator gradient (pattern (a) over the uneven batch): for every
  * operator i, with x_i its forward input block (k_i x cols_i, leading
  * dimension ldx[i]) and g_i the gradient of the loss with respect to its

@@ -284,6 +284,43 @@ def test_least_squares_function_matches_composition(torch_mod, dtype):
         assert abs(float(loss) - tot) <= 10 * tol * tot
 
 
+def test_least_squares_backward_after_another_forward(torch_mod):
+    """Two least-squares forwards on one batch (different x and y), then
+    the first one's backward, then the second's: each backward must give its
+    own step's gradients (the forward's last pass holds the adjoint's column
+    transforms, so a stale one is run again), against the FP64 oracle."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(53)
+    spec = [("circulant", 300, 300, 2), ("toeplitz", 3000, 9000, 3),
+            ("circulant", 5000, 5000, 4)]
+    gens0 = [crandn(rng, n if k_ == "circulant" else m + n - 1) for k_, m, n, _ in spec]
+    ops = [make_op(k_, m, g) for (k_, m, n, _), g in zip(spec, gens0)]
+    batch = ConvBatch(ops, [e[3] for e in spec])
+    n_in, n_out = batch.packed_sizes()
+    steps = []
+    for _ in range(2):
+        gens = [torch.from_numpy(g.copy()).cuda().requires_grad_(True) for g in gens0]
+        xp = torch.from_numpy(crandn(rng, n_in)).cuda().requires_grad_(True)
+        yp = torch.from_numpy(crandn(rng, n_out)).cuda()
+        steps.append((gens, xp, yp, fa.batch_least_squares_packed(batch, gens, xp, yp)))
+    say("least squares backward after another forward")
+    for gens, xp, yp, loss in steps:
+        loss.backward()
+        xv = batch.packed_views(xp.detach(), inputs=True)
+        yv = batch.packed_views(yp, inputs=False)
+        gxv = batch.packed_views(xp.grad, inputs=True)
+        for i, ((k_, m, n, k), g0) in enumerate(zip(spec, gens0)):
+            x = xv[i].cpu().numpy().T.astype(np.complex128)
+            y = yv[i].cpu().numpy().T.astype(np.complex128)
+            r = oracle_apply(k_, m, g0.astype(np.complex128), x) - y
+            gx = 2 * oracle_apply(k_, m, g0.astype(np.complex128), r, adjoint=True)
+            assert accept.max_rel_error(gxv[i].cpu().numpy().T, gx) < 1e-5, i
+            assert accept.max_rel_error(gens[i].grad.cpu().numpy(),
+                                        fast64.conv_generator_grad(k_, m, n, x, 2 * r)) < 1e-5
+
+
 def test_least_squares_step_replays_in_a_graph(torch_mod):
     """The fused least-squares step (generators to spectra, forward with
     residual, loss, adjoint, generator gradient) captured in one HIP graph:
