@@ -99,6 +99,7 @@ _PROTOTYPES = {
     "fm_batch_gen_workspace_size": (_c_int, [_vp, ctypes.POINTER(_c_i64)]),
     "fm_batch_gen_bind": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), _vp, _c_i64, _vp]),
     "fm_batch_gen_apply": (_c_int, [_vp, _vp]),
+    "fm_batch_gen_forward_apply": (_c_int, [_vp, _vp]),
     "fm_batch_plan_destroy": (_c_int, [_vp]),
     "fm_shrink_step": (_c_int, [_c_int, _c_i64, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
                                 _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),

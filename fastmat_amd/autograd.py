@@ -252,9 +252,10 @@ class _BatchLeastSquares(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, batch, xp, yp, *gens):
-        ctx.tok = batch.load_generators(gens)
         _BatchLeastSquares._bind(batch, xp, yp)
-        batch.launch(False)
+        # spectra from the generators, then the forward (its column
+        # transforms of x overlap the spectrum computation)
+        ctx.tok = batch.launch_with_generators(gens)
         ctx.fwd_count = batch.apply_count[0]
         loss = getattr(batch, "_lsq_loss", None)
         if loss is None or loss.device != xp.device:
@@ -282,8 +283,7 @@ class _BatchLeastSquares(torch.autograd.Function):
             # another forward ran on this batch since: run this step's
             # forward again (its last pass holds the adjoint's column
             # transforms of the residual)
-            batch.load_generators(gens)
-            batch.launch(False)
+            batch.launch_with_generators(gens)
         batch.launch(True)
         scale = gl.to(xp.dtype)
         gxp = gx_buf * scale if need_x else None

@@ -145,6 +145,12 @@ class ConvBatch:
         (Circulant.pyx:130, Toeplitz.pyx:253-255), as one embedding launch and
         one batched FFT program (fm_batch_gen_*).  Returns a token that
         changes with every load."""
+        self._bind_generators(gens)
+        _lib.call("fm_batch_gen_apply", self._h, D.stream())
+        self._spec_tok = self.generator_token + 1
+        return self._spec_tok
+
+    def _bind_generators(self, gens):
         if len(gens) != len(self.ops):
             raise ValueError("ConvBatch: need one generator per operator")
         gts = []
@@ -173,8 +179,16 @@ class ConvBatch:
                       D.stream())
             self._sbound = key
         self._sgens = gts  # the bound generators stay alive with the binding
-        _lib.call("fm_batch_gen_apply", self._h, D.stream())
+
+    def launch_with_generators(self, gens):
+        """load_generators(gens) then launch(False) (the bound forward), with
+        the forward's column transforms of its inputs overlapping the
+        spectrum computation (fm_batch_gen_forward_apply).  Returns the
+        generator token."""
+        self._bind_generators(gens)
+        _lib.call("fm_batch_gen_forward_apply", self._h, D.stream())
         self._spec_tok = self.generator_token + 1
+        self.apply_count[0] += 1
         return self._spec_tok
 
     @property

@@ -189,6 +189,7 @@ enum LaunchKind { LK_FFT = 0, LK_CONV = 1, LK_CORR = 2 };
 // One launch of a program: the work items of one kernel class on one lane.
 struct Launch {
     int kind, logN, nt, lane;
+    int early;  // from a pass group that needs no spectrum
     int idx32;  // every descriptor of the launch fits 32-bit index arithmetic
     void *items;
     int64_t nitems;
@@ -218,6 +219,7 @@ template <typename T> struct ProgBuilder {
     struct PassGroup {
         int lane;
         std::vector<Ref> refs;
+        bool early;  // needs no spectrum: may run while the spectra are formed
     };
     std::deque<PassGroup> groups;  // references stay valid as groups are added
     int lanes = 1;
@@ -231,8 +233,8 @@ template <typename T> struct ProgBuilder {
         cdescs.push_back(cd);
         return int(cdescs.size()) - 1;
     }
-    PassGroup &group(int lane) {
-        groups.push_back(PassGroup{lane, {}});
+    PassGroup &group(int lane, bool early = false) {
+        groups.push_back(PassGroup{lane, {}, early});
         if (lane + 1 > lanes) lanes = lane + 1;
         return groups.back();
     }
@@ -396,6 +398,7 @@ static fm_status finalize(ProgBuilder<T> &b, Program &prog, int ncu_lanes_cap, h
             l.logN = std::get<1>(key);
             l.nt = std::get<2>(key);
             l.lane = pg.lane < prog.lanes ? pg.lane : pg.lane % prog.lanes;
+            l.early = pg.early ? 1 : 0;
             l.idx32 = cfit[key] ? 1 : 0;
             l.nitems = int64_t(cls[key].size());
             l.bytes = calg[key];
@@ -533,8 +536,10 @@ static fm_status launch_one(const fm_batch_plan *p, const Program &d, const Laun
 
 // Issue a program: lanes beyond the first fork from the caller's stream and
 // join back into it (event record / wait only, so it is graph-capturable).
+// skip_early leaves out the launches of the groups that need no spectrum.
 template <typename T>
-static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream_t s) {
+static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream_t s,
+                             bool skip_early = false) {
     const int lanes = p->has_side ? (d.lanes < fm_batch_plan::kSide + 1 ? d.lanes
                                                                        : fm_batch_plan::kSide + 1)
                                   : 1;
@@ -543,6 +548,7 @@ static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream
         for (int q = 0; q < lanes - 1; ++q) FM_CHECK_HIP(hipStreamWaitEvent(p->side[q], p->fork_ev, 0));
     }
     for (const Launch &l : d.launches) {
+        if (skip_early && l.early) continue;
         const int lane = lanes > 1 ? l.lane % lanes : 0;
         hipStream_t sq = lane == 0 ? s : p->side[lane - 1];
         fm_status st = launch_one<T>(p, d, l, sq);
@@ -711,7 +717,9 @@ static void group_pipes(const fm_batch_plan *p, std::vector<PassList<T>> &pipes,
         const int lane = gi % lanes;
         const int phases = int(pipes[multi[q]].size());
         for (int ph = 0; ph < phases; ++ph) {
-            auto &pg = b.group(lane);
+            // the column transforms of the input (a first pass that is no
+            // convolution) need no spectrum
+            auto &pg = b.group(lane, ph == 0 && !pipes[multi[q]][0].conv);
             for (size_t t = q; t < r; ++t) pg.refs.push_back({0, b.add(pipes[multi[t]][ph])});
         }
         q = r;
@@ -1408,6 +1416,49 @@ extern "C" fm_status fm_batch_gen_apply(const fm_batch_plan *p, void *stream) {
                "batch generators used before fm_batch_gen_bind");
     if (p->prec == FM_COMPLEX64) return batch_spec_apply_t<float>(p, as_stream(stream));
     return batch_spec_apply_t<double>(p, as_stream(stream));
+}
+
+namespace fm {
+// The spectra from the generators and the forward apply, with the forward's
+// column transforms of the inputs (which need no spectrum) running on a
+// side stream while the spectra are formed: fork, early launches on the
+// last side stream, embedding + spectrum program on the caller's stream,
+// join, the rest of the forward program.  Without a free side stream it is
+// the two calls in sequence.
+template <typename T>
+static fm_status batch_gen_forward_t(const fm_batch_plan *p, hipStream_t s) {
+    const Program &d = p->dir[0];
+    constexpr int K = fm_batch_plan::kSide;
+    bool any_early = false;
+    for (const Launch &l : d.launches) any_early = any_early || l.early;
+    const bool overlap = p->has_side && any_early && d.lanes <= K && p->sprog.lanes <= K;
+    if (!overlap) {
+        fm_status st = batch_spec_apply_t<T>(p, s);
+        return st ? st : run_program<T>(p, d, s);
+    }
+    hipStream_t side = p->side[K - 1];
+    FM_CHECK_HIP(hipEventRecord(p->fork_ev, s));
+    FM_CHECK_HIP(hipStreamWaitEvent(side, p->fork_ev, 0));
+    for (const Launch &l : d.launches) {
+        if (!l.early) continue;
+        fm_status st = launch_one<T>(p, d, l, side);
+        if (st) return st;
+    }
+    FM_CHECK_HIP(hipEventRecord(p->join_ev[K - 1], side));
+    fm_status st = batch_spec_apply_t<T>(p, s);
+    if (st) return st;
+    FM_CHECK_HIP(hipStreamWaitEvent(s, p->join_ev[K - 1], 0));
+    return run_program<T>(p, d, s, true);
+}
+}  // namespace fm
+
+extern "C" fm_status fm_batch_gen_forward_apply(const fm_batch_plan *p, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->sprog.bound && p->stab, FM_ERR_VALUE,
+               "batch generators used before fm_batch_gen_bind");
+    FM_REQUIRE(p->dir[0].bound, FM_ERR_VALUE, "batch forward direction is not bound to buffers");
+    if (p->prec == FM_COMPLEX64) return batch_gen_forward_t<float>(p, as_stream(stream));
+    return batch_gen_forward_t<double>(p, as_stream(stream));
 }
 
 extern "C" fm_status fm_batch_plan_destroy(fm_batch_plan *p) {

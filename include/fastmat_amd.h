@@ -303,6 +303,15 @@ fm_status fm_batch_gen_workspace_size(const fm_batch_plan *plan, int64_t *bytes)
 fm_status fm_batch_gen_bind(fm_batch_plan *plan, int gen_dtype, const void *const *gens,
                             void *workspace, int64_t workspace_bytes, void *stream);
 fm_status fm_batch_gen_apply(const fm_batch_plan *plan, void *stream);
+/* fm_batch_gen_apply followed by fm_batch_apply(plan, 0), with the forward's
+ * column transforms of its inputs (the multi-pass operators' first pass,
+ * which needs no spectrum) issued on an internal side stream while the
+ * spectra are formed, joined back before the passes that read them; stream-
+ * ordered and capturable like the two calls it replaces (which it falls back
+ * to when the schedule leaves no side stream free).  The forward of a
+ * training step whose operators are recomputed from their generators
+ * (Circulant.pyx:130, Toeplitz.pyx:253-255). */
+fm_status fm_batch_gen_forward_apply(const fm_batch_plan *plan, void *stream);
 fm_status fm_batch_plan_destroy(fm_batch_plan *plan);
 
 /* --------------------------------------------------------------- dense

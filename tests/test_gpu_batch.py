@@ -272,6 +272,43 @@ def test_batched_generator_gradients_match_per_operator(torch_mod):
             assert accept.max_rel_error(r.cpu().numpy(), ref.cpu().numpy()) < 1e-5, (rep, op)
 
 
+def test_spectra_overlapped_with_forward_match_sequential(torch_mod):
+    """fm_batch_gen_forward_apply (the forward's column transforms issued on
+    a side stream while the spectra are formed) gives what load_generators
+    followed by the forward apply gives, bit for bit, on a mixed batch with
+    new generator values each time, and matches the oracle."""
+    torch = torch_mod
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(14)
+    spec = small_spec(14, count=8, hi=20000, kmax=4)
+    ops, ks = build(spec)
+    batch = ConvBatch(ops, ks)
+    xs = [torch.complex(torch.randn(k, op.numCols, device="cuda"),
+                        torch.randn(k, op.numCols, device="cuda")) for op, k in zip(ops, ks)]
+    ys = batch.new_outputs(False)
+    batch.bind(xs, ys, False)
+    gens = [torch.empty(e[4].size, dtype=torch.complex64, device="cuda") for e in spec]
+    for rep in range(2):
+        say("spectra overlapped with the forward, rep", rep)
+        new = [crandn(rng, e[4].size) for e in spec]
+        for g, h in zip(gens, new):
+            g.copy_(torch.from_numpy(h))
+        batch.load_generators(gens)
+        batch.launch(False)
+        ref = [y.clone() for y in ys]
+        for y in ys:
+            y.zero_()
+        batch.launch_with_generators(gens)
+        torch.cuda.synchronize()
+        for i, (y, r) in enumerate(zip(ys, ref)):
+            assert torch.equal(y, r), (rep, i)
+        for i, e in enumerate(spec):
+            kind, m, n, k, _ = e
+            entry = (kind, m, n, k, new[i])
+            want = oracle_apply(entry, xs[i].cpu().numpy().T.astype(np.complex128), False)
+            assert accept.max_rel_error(ys[i].cpu().numpy().T, want) < TOL32, (rep, i)
+
+
 def test_autograd_packed_batch_matches_per_tensor(torch_mod):
     """batch_conv_matvec_packed (all blocks in two flat tensors) gives the
     outputs and gradients of batch_conv_matvec, twice in a row."""
