@@ -349,7 +349,16 @@ def test_kron_fused_circulant_dense_tail(fm, lead, k):
     if lead:
         lead_b.append(lambda v: fast64.fourier(v, adjoint=True))
     opsb = lead_b + [lambda v: C64.conj().T @ v, lambda v: A.T.astype(np.float64) @ v]
-    assert relerr(K.backward(x), fast64.kron_apply(opsb, dims, x)) < 1e-5
+    got = K.backward(x)
+    want = fast64.kron_apply(opsb, dims, x)
+    err = relerr(got, want)
+    if err >= 1e-5:
+        # name the wrong 128 x 128 slabs (rows // 16384) and columns
+        bad = np.abs(got - want) > 1e-4 * np.abs(want).max()
+        rows, cols = np.nonzero(bad)
+        print("[diag] wrong elements", rows.size, "slabs", sorted(set((rows // 16384).tolist()))[:20],
+              "columns", sorted(set(cols.tolist())), flush=True)
+    assert err < 1e-5
 
 
 # ---------------------------------------------- composition and elementwise
