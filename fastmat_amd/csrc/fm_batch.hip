@@ -1065,10 +1065,16 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
     ProgBuilder<T> ba, bc, bp;
     auto &pa_x = ba.group(0);
     auto &pa_g = ba.group(p->lanes > 1 ? 1 : 0);
-    // the four-step operators' correlations and row inverses on lane 0, the
-    // one-pass operators' on a lane of their own (they share no data)
+    // the four-step operators' correlations: the longest rows on lane 0, the
+    // shorter ones on lane 2; the one-pass operators' on lane 1 (they share
+    // no data, and each alone leaves the chip partly idle)
     auto &pc = bc.group(0);
     auto &pc1 = bc.group(p->lanes > 1 ? 1 : 0);
+    auto &pc2 = bc.group(p->lanes > 1 ? 2 : 0);
+    int max_log2 = 0;
+    for (int i = 0; i < p->count; ++i)
+        if (p->plans[i]->geo.passes > 1 && p->plans[i]->geo.logN2 > max_log2)
+            max_log2 = p->plans[i]->geo.logN2;
     auto &pp = bp.group(0);
     auto &pp1 = bp.group(p->lanes > 1 ? 1 : 0);
     std::vector<GradEntry<T>> tab(p->count);
@@ -1136,7 +1142,7 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
             cd.g_valid = INT64_MAX;
             cd.k = k;
             cd.mode = CORR_FULL;
-            pc.refs.push_back({1, bc.add_corr(cd)});
+            (g.logN2 >= max_log2 ? pc : pc2).refs.push_back({1, bc.add_corr(cd)});
             // pass C of the rows (column inverse), natural order into r
             PipeIO io;
             memset(&io, 0, sizeof(io));
