@@ -683,6 +683,22 @@ def workload_batch_grad(args, rng):
         graph.replay()
         return loss_s
 
+    def isolated():
+        # the same eager step on a batch of its own with one lane, so that
+        # no launch shares the device with another and its HIP events time
+        # the kernel itself (the timed schedule overlaps two lanes)
+        holder = []
+
+        def s():
+            if not holder:
+                bi = ConvBatch(ops, ks)
+                bi.set_schedule(sched[0], sched[1], 1)
+                holder.append(bi)
+            for t in gens + [xp]:
+                t.grad = None
+            fa.batch_least_squares_packed(holder[0], gens, xp, yp).backward()
+        return s
+
     def generic_graph():
         holder = []
 
@@ -804,8 +820,8 @@ def workload_batch_grad(args, rng):
            "operators": len(ops), "total_columns": int(sum(ks)), "data": "packed",
            "schedule": {"workgroups_per_cu": sched[0], "group_mb": sched[1], "lanes": sched[2]},
            "schedule_candidates_ms": tuned, "hipgraph": True}
-    return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check,
-                plain=plain_torch(), per_op=per_op(), generic=generic_graph(), cpu=cpu,
+    return dict(step=step, eager_step=eager_step, isolated_step=isolated(), bytes=bytes_step,
+                check=check, plain=plain_torch(), per_op=per_op(), generic=generic_graph(), cpu=cpu,
                 cfg=cfg, dtype="c64", metric=METRIC)
 
 
@@ -957,6 +973,26 @@ def main():
                     "pass_bytes_per_launch": pass_bytes_per_launch,
                     "pass_GBps": round(pass_bytes_per_launch / (avg_ms * 1e-3) / 1e9, 1),
                     "timing": roof_note}
+    if w.get("isolated_step") is not None and roofline["bound"] == "hbm":
+        # the dominant kernel's launches one at a time (one lane): the
+        # kernel's own duration, without the other lane's kernels beside it
+        s = w["isolated_step"]
+        s()
+        torch.cuda.synchronize()
+        with _lib.Profiler() as prof3:
+            for _ in range(args.steps):
+                s()
+            torch.cuda.synchronize()
+        r3 = prof3.results.get(name)
+        if r3 and r3["launches"]:
+            iso_ms = r3["ms"] / r3["launches"]
+            iso = r3["bytes"] / r3["launches"] / (iso_ms * 1e-3) / 1e9
+            roofline["isolated"] = {
+                "avg_launch_us": round(iso_ms * 1e3, 2), "launches": r3["launches"],
+                "achieved": round(iso, 1), "frac": round(iso / HBM_PEAK_GBS, 4),
+                "pass_GBps": round(r3.get("pass_bytes", r3["bytes"]) / r3["launches"]
+                                   / (iso_ms * 1e-3) / 1e9, 1),
+                "timing": "HIP events, the same launches on one lane (no overlap)"}
     kernels = {k: {"launches": v["launches"], "avg_us": round(v["ms"] / v["launches"] * 1e3, 2),
                    "alg_GBps": round(v["bytes"] / v["ms"] / 1e6, 1),
                    "pass_GBps": round(v.get("pass_bytes", v["bytes"]) / v["ms"] / 1e6, 1),

@@ -284,9 +284,15 @@ class _BatchLeastSquares(torch.autograd.Function):
             # forward again (its last pass holds the adjoint's column
             # transforms of the residual)
             batch.launch_with_generators(gens)
-        batch.launch(True)
+        gxp = None
+        if need_x:
+            # the adjoint writes gl * A^H r straight into a new tensor (the
+            # bound gradient buffer stays as it is)
+            real_t = torch.float32 if xp.dtype == torch.complex64 else torch.float64
+            glr = gl.detach().to(real_t).reshape(1).contiguous()
+            gxp = torch.empty_like(gx_buf)
+            batch.launch(True, out_shift=gxp.data_ptr() - gx_buf.data_ptr(), out_scale=glr)
         scale = gl.to(xp.dtype)
-        gxp = gx_buf * scale if need_x else None
         if need_g:
             raw = batch.generator_grads(X, R, reuse=True, scale=scale)
             ggen = [(r if r.dtype == dt else r.to(dt)) if cplx else r.real.to(dt)

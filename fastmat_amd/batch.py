@@ -456,9 +456,26 @@ class ConvBatch:
             self._napply = [0, 0]
         return self._napply
 
-    def launch(self, adjoint=False):
-        """Launch the bound direction (kernels only; graph-capturable)."""
-        _lib.call("fm_batch_apply", self._h, int(adjoint), D.stream())
+    def launch(self, adjoint=False, out_shift=0, out_scale=None):
+        """Launch the bound direction (kernels only; graph-capturable).
+        out_shift (bytes, a whole number of elements) moves every output
+        block the direction writes to the same place in another buffer, and
+        out_scale (a one-element real device tensor: float32 for complex64,
+        float64 for complex128) multiplies what is written
+        (fm_batch_apply_redirected)."""
+        if out_shift or out_scale is not None:
+            sp = None
+            if out_scale is not None:
+                want = torch.float32 if self.dtype == np.complex64 else torch.float64
+                if (out_scale.dtype != want or out_scale.numel() != 1 or
+                        not out_scale.is_contiguous() or out_scale.device.type != "cuda"):
+                    raise TypeError("out_scale must be one contiguous %s element on the device"
+                                    % want)
+                sp = out_scale.data_ptr()
+            _lib.call("fm_batch_apply_redirected", self._h, int(adjoint), int(out_shift), sp,
+                      D.stream())
+        else:
+            _lib.call("fm_batch_apply", self._h, int(adjoint), D.stream())
         self.apply_count[int(adjoint)] += 1
 
     def apply(self, ins, outs=None, adjoint=False):

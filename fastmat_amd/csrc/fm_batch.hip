@@ -70,12 +70,19 @@ template <typename T> struct CorrDesc {
 // class's (descriptor, tile) items.
 template <typename T, int LOGN, bool CONV, bool I32>
 __global__ __launch_bounds__((PassCfg<T, LOGN>::NT)) void batch_pass_kernel(
-    const BatchDesc<T> *__restrict__ descs, const int2 *__restrict__ items, int64_t nitems) {
+    const BatchDesc<T> *__restrict__ descs, const int2 *__restrict__ items, int64_t nitems,
+    int64_t out_shift, const T *__restrict__ out_mul) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem_raw);
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 w = items[it];
-        const TileArgs<T> a = descs[w.x].a;
+        TileArgs<T> a = descs[w.x].a;
+        if (a.redirect && (out_shift != 0 || out_mul != nullptr)) {
+            // redirected apply: the output block moved by out_shift bytes,
+            // scaled by the device scalar *out_mul
+            a.y = reinterpret_cast<cpx<T> *>(reinterpret_cast<char *>(a.y) + out_shift);
+            if (out_mul) a.out_scale *= *out_mul;
+        }
         fft_pass_body<T, LOGN, PassCfg<T, LOGN>::NT, CONV, I32, PassCfg<T, LOGN>::LE>(a, w.y, lds);
         __syncthreads();  // the next item reuses the LDS tile
     }
@@ -420,7 +427,7 @@ static fm_status finalize(ProgBuilder<T> &b, Program &prog, int ncu_lanes_cap, h
 // ------------------------------------------------------------- launching
 template <typename T, int LOGN, bool CONV>
 static fm_status launch_pass_class(const fm_batch_plan *p, const Program &d, const Launch &l,
-                                   hipStream_t s) {
+                                   hipStream_t s, int64_t out_shift, const T *out_mul) {
     constexpr int N = 1 << LOGN;
     const size_t lds = size_t(tile_batch<T>(LOGN)) * (N + (N >> 5) + 1) * sizeof(cpx<T>);
     static bool attr = false;
@@ -438,11 +445,12 @@ static fm_status launch_pass_class(const fm_batch_plan *p, const Program &d, con
     constexpr int NT = PassCfg<T, LOGN>::NT;
     if (l.idx32)
         hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, true>), dim3(unsigned(grid)), dim3(NT),
-                           lds, s, (const BatchDesc<T> *)d.descs, (const int2 *)l.items, l.nitems);
+                           lds, s, (const BatchDesc<T> *)d.descs, (const int2 *)l.items, l.nitems,
+                           out_shift, out_mul);
     else
         hipLaunchKernelGGL((batch_pass_kernel<T, LOGN, CONV, false>), dim3(unsigned(grid)),
                            dim3(NT), lds, s, (const BatchDesc<T> *)d.descs, (const int2 *)l.items,
-                           l.nitems);
+                           l.nitems, out_shift, out_mul);
     FM_CHECK_LAUNCH();
     if (tok.active) {
         char name[80];
@@ -506,13 +514,13 @@ static fm_status launch_corr(const fm_batch_plan *p, const Program &d, const Lau
 
 template <typename T>
 static fm_status launch_one(const fm_batch_plan *p, const Program &d, const Launch &l,
-                            hipStream_t s) {
+                            hipStream_t s, int64_t out_shift = 0, const T *out_mul = nullptr) {
     fm_status st = FM_ERR_VALUE;
 #define FM_BATCH_CASE(L)                                                                     \
     case L:                                                                                  \
         st = l.kind == LK_CORR   ? launch_corr<T, L>(p, d, l, s)                             \
-             : l.kind == LK_CONV ? launch_pass_class<T, L, true>(p, d, l, s)                 \
-                                 : launch_pass_class<T, L, false>(p, d, l, s);               \
+             : l.kind == LK_CONV ? launch_pass_class<T, L, true>(p, d, l, s, out_shift, out_mul) \
+                                 : launch_pass_class<T, L, false>(p, d, l, s, out_shift, out_mul); \
         break;
     switch (l.logN) {
         FM_BATCH_CASE(0) FM_BATCH_CASE(1) FM_BATCH_CASE(2) FM_BATCH_CASE(3)
@@ -522,8 +530,9 @@ static fm_status launch_one(const fm_batch_plan *p, const Program &d, const Laun
     case 13:
         if constexpr (sizeof(T) == 4)
             st = l.kind == LK_CORR   ? launch_corr<T, 13>(p, d, l, s)
-                 : l.kind == LK_CONV ? launch_pass_class<T, 13, true>(p, d, l, s)
-                                     : launch_pass_class<T, 13, false>(p, d, l, s);
+                 : l.kind == LK_CONV ? launch_pass_class<T, 13, true>(p, d, l, s, out_shift, out_mul)
+                                     : launch_pass_class<T, 13, false>(p, d, l, s, out_shift,
+                                                                       out_mul);
         break;
     default:
         break;
@@ -539,7 +548,8 @@ static fm_status launch_one(const fm_batch_plan *p, const Program &d, const Laun
 // skip_early leaves out the launches of the groups that need no spectrum.
 template <typename T>
 static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream_t s,
-                             bool skip_early = false) {
+                             bool skip_early = false, int64_t out_shift = 0,
+                             const T *out_mul = nullptr) {
     const int lanes = p->has_side ? (d.lanes < fm_batch_plan::kSide + 1 ? d.lanes
                                                                        : fm_batch_plan::kSide + 1)
                                   : 1;
@@ -551,7 +561,7 @@ static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream
         if (skip_early && l.early) continue;
         const int lane = lanes > 1 ? l.lane % lanes : 0;
         hipStream_t sq = lane == 0 ? s : p->side[lane - 1];
-        fm_status st = launch_one<T>(p, d, l, sq);
+        fm_status st = launch_one<T>(p, d, l, sq, out_shift, out_mul);
         if (st) return st;
     }
     if (lanes > 1) {
@@ -763,6 +773,7 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
             pipes[i][1].a.y = t2;
             pipes[i][2].a.x = t2;
         }
+        pipes[i].back().a.redirect = 1;  // writes outs[i]
     }
     if (targets) {
         // residual mode: the last pass of every operator stores
@@ -913,7 +924,9 @@ static fm_status batch_bind_lsq_t(fm_batch_plan *p, const void *const *ins, cons
             TileArgs<T> &a = fw[i][0].a;
             a.target = (const cpx<T> *)targets[i];
             a.out_scale = T(scale);
+            a.redirect = 1;  // writes routs[i]
         }
+        ad[i].back().a.redirect = 1;  // writes gouts[i]
     }
     // residual slots: one per tile of every operator's residual pass
     int64_t slots = 0;
@@ -998,6 +1011,23 @@ extern "C" fm_status fm_batch_apply(const fm_batch_plan *p, int adjoint, void *s
     FM_REQUIRE(d.bound, FM_ERR_VALUE, "batch direction %d is not bound to buffers", adjoint);
     if (p->prec == FM_COMPLEX64) return run_program<float>(p, d, as_stream(stream));
     return run_program<double>(p, d, as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_apply_redirected(const fm_batch_plan *p, int adjoint,
+                                               int64_t shift_bytes, const void *scale,
+                                               void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    const Program &d = p->dir[adjoint ? 1 : 0];
+    FM_REQUIRE(d.bound, FM_ERR_VALUE, "batch direction %d is not bound to buffers", adjoint);
+    const int64_t esz = p->prec == FM_COMPLEX64 ? 8 : 16;
+    FM_REQUIRE(shift_bytes % esz == 0, FM_ERR_VALUE,
+               "output shift of %lld bytes is not a whole number of elements",
+               (long long)shift_bytes);
+    if (p->prec == FM_COMPLEX64)
+        return run_program<float>(p, d, as_stream(stream), false, shift_bytes,
+                                  (const float *)scale);
+    return run_program<double>(p, d, as_stream(stream), false, shift_bytes,
+                               (const double *)scale);
 }
 
 // ======================================================================

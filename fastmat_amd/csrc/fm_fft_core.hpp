@@ -190,6 +190,10 @@ template <typename T> struct TileArgs {
     int conj_out;
     T out_scale;
     int seqfast_out;
+    // batch engine: this pass writes the caller's output of its direction;
+    // a redirected apply moves y by a byte shift and multiplies out_scale by
+    // a device scalar for such passes only
+    int redirect;
     int logJ;                       // log2(J) when J is a power of two, else -1
     int64_t o_group;                // > 1: tiles walk groups of o_group o's (see tile_first_seq)
     // optional residual epilogue (target != NULL): the store writes

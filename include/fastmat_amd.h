@@ -235,6 +235,17 @@ fm_status fm_batch_bind(fm_batch_plan *plan, int adjoint, const void *const *inp
                         const int64_t *ld_out, void *workspace,
                         int64_t workspace_bytes, void *stream);
 fm_status fm_batch_apply(const fm_batch_plan *plan, int adjoint, void *stream);
+
+/* fm_batch_apply with the direction's outputs redirected: every element the
+ * bound direction writes to its output blocks lands shift_bytes further on
+ * (the same layout in another buffer; a whole number of elements) and is
+ * multiplied by the device scalar *scale (float for complex64 plans, double
+ * for complex128; NULL: 1).  The bound buffers themselves are not written.
+ * The backward of a loss writes the input gradient, scaled by the loss's
+ * incoming gradient, straight into a new tensor this way (Matrix.pyx:1844
+ * applies the adjoint per operator; the scaling is torch autograd's). */
+fm_status fm_batch_apply_redirected(const fm_batch_plan *plan, int adjoint, int64_t shift_bytes,
+                                    const void *scale, void *stream);
 /* Residual mode of the forward direction (the least-squares step
  * sum_i ||A_i x_i - t_i||^2 of a training loop): like fm_batch_bind for the
  * forward direction, but output i receives scale * (A_i x_i - t_i) (t_i a

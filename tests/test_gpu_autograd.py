@@ -237,7 +237,9 @@ def test_least_squares_function_matches_composition(torch_mod, dtype):
     forward apply) against the generic composition
     ((batch_conv_matvec_packed(...) - y).abs() ** 2).sum() and the FP64
     oracle: loss, x gradient and every generator gradient, on a mixed batch of
-    one- and multi-pass operators; twice (bindings reused)."""
+    one- and multi-pass operators; twice (bindings reused), the second time
+    with an upstream gradient of 2.5 (the adjoint writes the x gradient,
+    scaled by it, straight into a new tensor)."""
     torch = torch_mod
     import fastmat_amd.autograd as fa
     from fastmat_amd.batch import ConvBatch
@@ -255,14 +257,15 @@ def test_least_squares_function_matches_composition(torch_mod, dtype):
         gens = [torch.from_numpy(g.copy()).cuda().requires_grad_(True) for g in gens0]
         xp = torch.from_numpy(crandn(rng, n_in, dtype=dtype)).cuda().requires_grad_(True)
         yp = torch.from_numpy(crandn(rng, n_out, dtype=dtype)).cuda()
-        say("least squares", np.dtype(dtype).name, "rep", rep)
+        w = 1.0 if rep == 0 else 2.5
+        say("least squares", np.dtype(dtype).name, "rep", rep, "upstream", w)
         loss = fa.batch_least_squares_packed(batch, gens, xp, yp)
-        loss.backward()
+        (loss * w).backward()
         gens2 = [g.detach().clone().requires_grad_(True) for g in gens]
         xp2 = xp.detach().clone().requires_grad_(True)
         out = fa.batch_conv_matvec_packed(batch, gens2, xp2)
         loss2 = ((out - yp).abs() ** 2).sum()
-        loss2.backward()
+        (loss2 * w).backward()
         assert abs(float(loss) - float(loss2)) <= tol * abs(float(loss2))
         assert accept.max_rel_error(xp.grad.cpu().numpy(), xp2.grad.cpu().numpy()) < tol
         for g, g2 in zip(gens, gens2):
@@ -277,10 +280,11 @@ def test_least_squares_function_matches_composition(torch_mod, dtype):
             y = yv[i].cpu().numpy().T.astype(np.complex128)
             r = oracle_apply(k_, m, g0.astype(np.complex128), x) - y
             tot += float(np.sum(np.abs(r) ** 2))
-            gx = 2 * oracle_apply(k_, m, g0.astype(np.complex128), r, adjoint=True)
+            gx = 2 * w * oracle_apply(k_, m, g0.astype(np.complex128), r, adjoint=True)
             assert accept.max_rel_error(gxv[i].cpu().numpy().T, gx) < 10 * tol, i
             assert accept.max_rel_error(gens[i].grad.cpu().numpy(),
-                                        fast64.conv_generator_grad(k_, m, n, x, 2 * r)) < 10 * tol
+                                        fast64.conv_generator_grad(k_, m, n, x, 2 * w * r)) \
+                < 10 * tol
         assert abs(float(loss) - tot) <= 10 * tol * tot
 
 

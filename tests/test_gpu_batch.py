@@ -99,6 +99,47 @@ def test_batch_matches_oracle(torch_mod):
                                    rtol=0, atol=1e-5 * float(y.abs().max()))
 
 
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_redirected_apply_writes_scaled_outputs_elsewhere(torch_mod, adjoint):
+    """fm_batch_apply_redirected: the bound direction's outputs land at the
+    same places of another buffer (a byte shift), multiplied by a device
+    scalar, and the bound output buffer is not written; the plain apply
+    afterwards writes the bound buffer again."""
+    torch = torch_mod
+    from fastmat_amd.batch import ConvBatch
+    spec = small_spec(3)
+    ops, ks = build(spec)
+    assert any(op.fftLength > 8192 for op in ops)
+    batch = ConvBatch(ops, ks)
+    n_in, n_out = batch.packed_sizes()
+    if adjoint:
+        n_in, n_out = n_out, n_in
+    rng = np.random.default_rng(4)
+    xin = torch.from_numpy(crandn(rng, n_in)).cuda()
+    out = torch.empty(n_out, dtype=torch.complex64, device="cuda")
+    xs = batch.packed_views(xin, adjoint=adjoint, inputs=True)
+    ys = batch.packed_views(out, adjoint=adjoint, inputs=False)
+    batch.bind(xs, ys, adjoint)
+    say("redirected apply", "adjoint" if adjoint else "forward", len(ops))
+    batch.launch(adjoint)
+    ref = out.clone()
+    out.fill_(7.0)
+    other = torch.full((n_out,), 3.0, dtype=torch.complex64, device="cuda")
+    s = torch.tensor([2.5], dtype=torch.float32, device="cuda")
+    batch.launch(adjoint, out_shift=other.data_ptr() - out.data_ptr(), out_scale=s)
+    torch.cuda.synchronize()
+    assert bool((out == 7.0).all())
+    assert accept.max_rel_error(other.cpu().numpy(), 2.5 * ref.cpu().numpy()) < 1e-6
+    batch.launch(adjoint)
+    torch.cuda.synchronize()
+    assert bool((out == ref).all())
+    # against the oracle, per operator
+    for e, x, y in zip(spec, xs, batch.packed_views(other, adjoint=adjoint, inputs=False)):
+        xr = x.cpu().numpy().T.astype(np.complex128)
+        assert accept.max_rel_error(y.cpu().numpy().T, 2.5 * oracle_apply(e, xr, adjoint)) \
+            < TOL32, e[:4]
+
+
 def test_batch_graph_replays_and_tuner(torch_mod, tmp_path):
     torch = torch_mod
     from fastmat_amd.batch import Autotuner, ConvBatch
