@@ -495,11 +495,12 @@ class ConvBatch:
     # (workgroups per CU, group budget MB, lanes): ungrouped schedules (the
     # multi-pass operators' passes as one launch per length class), groups
     # sized for the memory-side cache on one or two lanes, and budgets of
-    # 1-1.5 GB that split the longest operators from the rest so that the
-    # two groups' pass chains run side by side on two lanes
+    # 0.5-1.5 GB that split the longest operators from the rest so that two
+    # or three groups' pass chains run side by side on as many lanes
     DEFAULT_SCHEDULES = ((2, 0, 2), (3, 0, 2), (4, 0, 2), (2, 48, 2), (3, 48, 2),
                          (2, 96, 2), (3, 96, 2), (4, 96, 2), (2, 128, 1), (3, 192, 1),
-                         (4, 1024, 2), (4, 1536, 2), (3, 1024, 2))
+                         (4, 1024, 2), (4, 1536, 2), (3, 1024, 2), (6, 1536, 2),
+                         (4, 512, 3), (4, 768, 3), (4, 1024, 3))
 
     def tune(self, ins, outs, adjoint=False, tuner=None, candidates=(1, 2, 3, 4)):
         """Pick the persistent grid (workgroups per CU) of one direction."""
