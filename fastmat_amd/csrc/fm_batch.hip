@@ -1062,12 +1062,12 @@ __global__ void batch_fold_kernel(const GradEntry<T> *__restrict__ tab) {
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < e.gen_len; i += stride) {
         cpx<T> v;
         if (e.kind == 0) {
-            v = e.r[i];
-            if (e.tail > 0 && i >= 1) v = v + e.r[e.L - e.tail + i - 1];
+            v = ldg(e.r, i);
+            if (e.tail > 0 && i >= 1) v = v + ldg(e.r, int64_t(e.L - e.tail + i - 1));
         } else {
-            v = i < e.head ? e.r[i] : e.r[e.L - e.tail + (i - e.head)];
+            v = i < e.head ? ldg(e.r, i) : ldg(e.r, int64_t(e.L - e.tail + (i - e.head)));
         }
-        e.out[i] = scale(v, s);
+        stg(e.out, i, scale(v, s));
     }
 }
 
@@ -1341,7 +1341,7 @@ __global__ void batch_embed_kernel(const GenEntry *__restrict__ tab) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < e.L; i += stride) {
         const cpx<double> v = embedded_at((const G *)e.gen, i, e.L, e.head, e.tail, e.tail_src);
-        ((cpx<T> *)e.t)[i] = mk(T(v.x), T(v.y));
+        stg((cpx<T> *)e.t, i, mk(T(v.x), T(v.y)));
     }
 }
 

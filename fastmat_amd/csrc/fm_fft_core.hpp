@@ -223,22 +223,33 @@ template <typename T> struct TileArgs {
 // (32-bit offsets from a uniform base become saddr + voffset addressing).
 template <bool I32> using IdxT = typename std::conditional<I32, uint32_t, int64_t>::type;
 
-// Load / store element `idx` of a uniform base pointer.
+// Load / store element `idx` of a uniform base pointer, as global memory:
+// every data pointer of the library is device memory, and kernels that read
+// their pointers from descriptors in memory (the batch engine) would
+// otherwise get flat loads and stores, which count against the LDS wait
+// counter too and so serialise the tile's LDS exchanges behind its
+// outstanding HBM traffic.
+// (the host pass of the same source sees plain pointers)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FM_GLOBAL __attribute__((address_space(1)))
+#else
+#define FM_GLOBAL
+#endif
 template <typename Ix, typename V>
 __device__ __forceinline__ V ldg(const V *base, Ix idx) {
+    const FM_GLOBAL char *b = (const FM_GLOBAL char *)base;
     if constexpr (sizeof(Ix) == 4)
-        return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) +
-                                            uint32_t(idx * uint32_t(sizeof(V))));
+        return *(const FM_GLOBAL V *)(b + uint32_t(idx * uint32_t(sizeof(V))));
     else
-        return base[idx];
+        return ((const FM_GLOBAL V *)b)[idx];
 }
 template <typename Ix, typename V>
 __device__ __forceinline__ void stg(V *base, Ix idx, V val) {
+    FM_GLOBAL char *b = (FM_GLOBAL char *)base;
     if constexpr (sizeof(Ix) == 4)
-        *reinterpret_cast<V *>(reinterpret_cast<char *>(base) +
-                               uint32_t(idx * uint32_t(sizeof(V)))) = val;
+        *(FM_GLOBAL V *)(b + uint32_t(idx * uint32_t(sizeof(V)))) = val;
     else
-        base[idx] = val;
+        ((FM_GLOBAL V *)b)[idx] = val;
 }
 
 // Bound of a logical index in the launch's index type.
@@ -318,7 +329,7 @@ __device__ __forceinline__ void fft_stage2(cpx<T> *v, cpx<T> *lds,
             // W_{pR}^{r k} = W_N^{r k N / (p R)}: one FP64 lookup of
             // W_{pR}^k, its powers by FP64 products, rounded once to T
             const int step = k << (LOGN - LOGP - LOGR);
-            const cpx<double> w1 = tw[step];
+            const cpx<double> w1 = ldg(tw, uint32_t(step));
             cpx<double> w = w1;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
@@ -436,7 +447,7 @@ __device__ __forceinline__ void tile_residual_sum(const TileArgs<T> &a, int64_t 
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int q = 0; q < NT / 64; ++q) t += red[q];
-        a.res_part[a.res_base + tile] = t;
+        stg(a.res_part, int64_t(a.res_base + tile), t);
     }
     __syncthreads();  // red is read before the LDS is written again
 }

@@ -360,18 +360,19 @@ namespace fm {
 // Generator element as complex128 (real generators have zero imaginary part).
 template <typename G> __device__ __forceinline__ cpx<double> gen_elem(const G *g, int64_t i);
 template <> __device__ __forceinline__ cpx<double> gen_elem<float>(const float *g, int64_t i) {
-    return mk(double(g[i]), 0.0);
+    return mk(double(ldg(g, i)), 0.0);
 }
 template <> __device__ __forceinline__ cpx<double> gen_elem<double>(const double *g, int64_t i) {
-    return mk(g[i], 0.0);
+    return mk(ldg(g, i), 0.0);
 }
 template <>
 __device__ __forceinline__ cpx<double> gen_elem<cpx<float>>(const cpx<float> *g, int64_t i) {
-    return mk(double(g[i].x), double(g[i].y));
+    const cpx<float> v = ldg(g, i);
+    return mk(double(v.x), double(v.y));
 }
 template <>
 __device__ __forceinline__ cpx<double> gen_elem<cpx<double>>(const cpx<double> *g, int64_t i) {
-    return g[i];
+    return ldg(g, i);
 }
 
 // Embedded generator t'[i] (see fm_conv_plan) for i < L.
