@@ -79,11 +79,15 @@ def device_block(a, dtype):
     return torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda").to(dtype)
 
 
-def traffic_from_profiles(kernel, config):
+def traffic_from_profiles(kernel, config, alg_bytes_per_launch=None):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), if recorded for this
     kernel on this workload: a line `<kernel label> | bytes_per_launch` in
-    profiles/*<config>*traffic*.md (or .txt)."""
+    profiles/*<config>*traffic*.md (or .txt).  A third column, the profiled
+    launches' algorithmic bytes, rescales the figure to this run's launches
+    (a batch schedule can split a length class into more or fewer launches:
+    the class's bytes per algorithmic byte stay, its bytes per launch do
+    not)."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None
@@ -94,7 +98,10 @@ def traffic_from_profiles(kernel, config):
             parts = [p.strip() for p in line.split("|")]
             if len(parts) >= 2 and parts[0] == kernel:
                 try:
-                    return float(parts[1])
+                    t = float(parts[1])
+                    if len(parts) >= 3 and parts[2] and alg_bytes_per_launch:
+                        t *= alg_bytes_per_launch / float(parts[2])
+                    return round(t)
                 except ValueError:
                     pass
     return None
@@ -361,6 +368,21 @@ def workload_batch(args, rng):
             plain_torch_path(ops, gs, True)
         return s
 
+    def isolated():
+        # the same applies on a batch of its own with one lane (every launch
+        # alone on the device)
+        holder = []
+
+        def s():
+            if not holder:
+                bi = ConvBatch(ops, ks)
+                bi.set_schedule(sched[0], sched[1], 1)
+                holder.append((bi, bi.new_outputs(False), bi.new_outputs(True)))
+            bi, yi, zi = holder[0]
+            bi.apply(xs, yi, False)
+            bi.apply(gs, zi, True)
+        return s
+
     def per_op():
         def s():
             for op, x in zip(ops, xs):
@@ -375,8 +397,8 @@ def workload_batch(args, rng):
            "schedule": {"workgroups_per_cu": sched[0], "group_mb": sched[1],
                         "lanes": sched[2]}, "schedule_candidates_ms": tuned,
            "hipgraph": True}
-    return dict(step=step, eager_step=eager_step, bytes=bytes_step, check=check, plain=plain(),
-                per_op=per_op(), cpu=None, cfg=cfg, dtype="c64",
+    return dict(step=step, eager_step=eager_step, isolated_step=isolated(), bytes=bytes_step,
+                check=check, plain=plain(), per_op=per_op(), cpu=None, cfg=cfg, dtype="c64",
                 metric=METRIC)
 
 
@@ -945,8 +967,27 @@ def main():
         print(json.dumps({"metric": w["metric"], "value": round(w["bytes"] / (ms * 1e-3) / 1e9, 2),
                           "unit": "GB/s", "ms_per_step": round(ms, 4), "config": w["cfg"]}))
         return
-    # dominant kernel from the library's own events
-    name, rec = max(results.items(), key=lambda kv: kv[1]["ms"])
+    iso_results = {}
+    if w.get("isolated_step") is not None:
+        # the same launches on a batch bound with one lane, one at a time:
+        # each kernel's own duration, without the other lanes' kernels
+        # sharing the device with it
+        s = w["isolated_step"]
+        s()
+        torch.cuda.synchronize()
+        with _lib.Profiler() as prof3:
+            for _ in range(args.steps):
+                s()
+            torch.cuda.synchronize()
+        iso_results = prof3.results
+    # dominant kernel from the library's own events (by its own time when
+    # the one-lane leg ran: overlapped durations stretch with what shares
+    # the device)
+    if iso_results:
+        name = max(iso_results.items(), key=lambda kv: kv[1]["ms"])[0]
+        rec = results.get(name, iso_results[name])
+    else:
+        name, rec = max(results.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = rec["ms"] / rec["launches"]
     bytes_per_launch = rec["bytes"] / rec["launches"]
     pass_bytes_per_launch = rec.get("pass_bytes", rec["bytes"]) / rec["launches"]
@@ -967,23 +1008,15 @@ def main():
     else:
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic_from_profiles(name, args.config), "kernel": name,
+                    "traffic": traffic_from_profiles(name, args.config, bytes_per_launch),
+                    "kernel": name,
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": rec["launches"],
                     "algorithmic_bytes_per_launch": bytes_per_launch,
                     "pass_bytes_per_launch": pass_bytes_per_launch,
                     "pass_GBps": round(pass_bytes_per_launch / (avg_ms * 1e-3) / 1e9, 1),
                     "timing": roof_note}
-    if w.get("isolated_step") is not None and roofline["bound"] == "hbm":
-        # the dominant kernel's launches one at a time (one lane): the
-        # kernel's own duration, without the other lane's kernels beside it
-        s = w["isolated_step"]
-        s()
-        torch.cuda.synchronize()
-        with _lib.Profiler() as prof3:
-            for _ in range(args.steps):
-                s()
-            torch.cuda.synchronize()
-        r3 = prof3.results.get(name)
+    if iso_results and roofline["bound"] == "hbm":
+        r3 = iso_results.get(name)
         if r3 and r3["launches"]:
             iso_ms = r3["ms"] / r3["launches"]
             iso = r3["bytes"] / r3["launches"] / (iso_ms * 1e-3) / 1e9
@@ -998,6 +1031,9 @@ def main():
                    "pass_GBps": round(v.get("pass_bytes", v["bytes"]) / v["ms"] / 1e6, 1),
                    "ms_per_step": round(v["ms"] / args.steps, 4)}
                for k, v in results.items()}
+    for k, v in iso_results.items():
+        if k in kernels and v["launches"]:
+            kernels[k]["one_lane_avg_us"] = round(v["ms"] / v["launches"] * 1e3, 2)
 
     extra = {}
     if args.no_legs:

@@ -284,17 +284,21 @@ class _BatchLeastSquares(torch.autograd.Function):
             # forward again (its last pass holds the adjoint's column
             # transforms of the residual)
             batch.launch_with_generators(gens)
-        gxp = None
+        gxp = adj = None
         if need_x:
             # the adjoint writes gl * A^H r straight into a new tensor (the
             # bound gradient buffer stays as it is)
             real_t = torch.float32 if xp.dtype == torch.complex64 else torch.float64
             glr = gl.detach().to(real_t).reshape(1).contiguous()
             gxp = torch.empty_like(gx_buf)
-            batch.launch(True, out_shift=gxp.data_ptr() - gx_buf.data_ptr(), out_scale=glr)
+            adj = (gxp.data_ptr() - gx_buf.data_ptr(), glr)
+            if not need_g:
+                batch.launch(True, out_shift=adj[0], out_scale=adj[1])
         scale = gl.to(xp.dtype)
         if need_g:
-            raw = batch.generator_grads(X, R, reuse=True, scale=scale)
+            # with the x gradient: the adjoint and the generator gradient's
+            # correlations in one backward (fm_batch_lsq_backward)
+            raw = batch.generator_grads(X, R, reuse=True, scale=scale, lsq_adjoint=adj)
             ggen = [(r if r.dtype == dt else r.to(dt)) if cplx else r.real.to(dt)
                     for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]
         return (None, gxp, None, *ggen)

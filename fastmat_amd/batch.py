@@ -376,7 +376,7 @@ class ConvBatch:
         # while the same addresses are handed in again
         self._bound[int(adjoint)] = key
 
-    def generator_grads(self, xs, gs, reuse=False, scale=None):
+    def generator_grads(self, xs, gs, reuse=False, scale=None, lsq_adjoint=None):
         """Gradients of a real loss with respect to every operator's
         generator, given the forward inputs xs[i] (k_i, cols_i) and the
         output gradients gs[i] (k_i, rows_i) as contiguous blocks: what
@@ -384,7 +384,12 @@ class ConvBatch:
         in one batched program (fm_batch_grad_*).  reuse=True reads the
         column transforms the last forward apply (of xs) and adjoint apply
         (of gs) left in the workspace instead of recomputing them.  ``scale``
-        (a device scalar tensor) multiplies all gradients in one pass."""
+        (a device scalar tensor) multiplies all gradients in one pass.
+        lsq_adjoint=(out_shift, out_scale), on a least-squares binding
+        (bind_lsq, with xs its inputs and gs its residuals), also runs the
+        adjoint apply as launch(True, out_shift, out_scale) does, its
+        launches sharing the lanes with the gradient's
+        (fm_batch_lsq_backward)."""
         n = len(self.ops)
         for i, (op, k) in enumerate(zip(self.ops, self.ks)):
             if tuple(xs[i].shape) != (k, op.numCols) or tuple(gs[i].shape) != (k, op.numRows):
@@ -423,8 +428,14 @@ class ConvBatch:
             _lib.call("fm_batch_grad_bind", self._h, px, lx, pg, lg, po, _lib.ptr(self._gws),
                       self._gws_bytes, D.stream())
             self._gbound = key
-        _lib.call("fm_batch_grad_apply_reuse" if reuse else "fm_batch_grad_apply", self._h,
-                  D.stream())
+        if lsq_adjoint is not None:
+            shift, sc = lsq_adjoint
+            _lib.call("fm_batch_lsq_backward", self._h, int(shift), self._scale_ptr(sc),
+                      D.stream())
+            self.apply_count[1] += 1
+        else:
+            _lib.call("fm_batch_grad_apply_reuse" if reuse else "fm_batch_grad_apply", self._h,
+                      D.stream())
         out = self._gout.clone() if scale is None else self._gout * scale
         return [out[int(off[i]):int(off[i + 1])] for i in range(n)]
 
@@ -449,6 +460,17 @@ class ConvBatch:
         self.grid_per_cu, self.group_mb, self.lanes = (int(workgroups_per_cu), int(group_mb),
                                                        int(lanes))
 
+    def _scale_ptr(self, out_scale):
+        """Device address of a one-element real scale (float32 for complex64
+        batches, float64 for complex128), or None."""
+        if out_scale is None:
+            return None
+        want = torch.float32 if self.dtype == np.complex64 else torch.float64
+        if (out_scale.dtype != want or out_scale.numel() != 1 or
+                not out_scale.is_contiguous() or out_scale.device.type != "cuda"):
+            raise TypeError("out_scale must be one contiguous %s element on the device" % want)
+        return out_scale.data_ptr()
+
     @property
     def apply_count(self):
         """Launches issued per direction (forward, adjoint)."""
@@ -464,16 +486,8 @@ class ConvBatch:
         float64 for complex128) multiplies what is written
         (fm_batch_apply_redirected)."""
         if out_shift or out_scale is not None:
-            sp = None
-            if out_scale is not None:
-                want = torch.float32 if self.dtype == np.complex64 else torch.float64
-                if (out_scale.dtype != want or out_scale.numel() != 1 or
-                        not out_scale.is_contiguous() or out_scale.device.type != "cuda"):
-                    raise TypeError("out_scale must be one contiguous %s element on the device"
-                                    % want)
-                sp = out_scale.data_ptr()
-            _lib.call("fm_batch_apply_redirected", self._h, int(adjoint), int(out_shift), sp,
-                      D.stream())
+            _lib.call("fm_batch_apply_redirected", self._h, int(adjoint), int(out_shift),
+                      self._scale_ptr(out_scale), D.stream())
         else:
             _lib.call("fm_batch_apply", self._h, int(adjoint), D.stream())
         self.apply_count[int(adjoint)] += 1

@@ -288,6 +288,9 @@ struct fm_batch_plan {
     double *res_part;
     int64_t n_res;
     bool res_bound;
+    // both directions bound by fm_batch_bind_lsq (the adjoint starts at its
+    // row pass; its column transforms of the residual come from the forward)
+    bool lsq_bound;
     // side streams (default priority) for the lanes, forked from and joined
     // back into the caller's stream
     static constexpr int kSide = 3;
@@ -543,20 +546,17 @@ static fm_status launch_one(const fm_batch_plan *p, const Program &d, const Laun
     return st;
 }
 
-// Issue a program: lanes beyond the first fork from the caller's stream and
-// join back into it (event record / wait only, so it is graph-capturable).
-// skip_early leaves out the launches of the groups that need no spectrum.
 template <typename T>
-static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream_t s,
-                             bool skip_early = false, int64_t out_shift = 0,
-                             const T *out_mul = nullptr) {
-    const int lanes = p->has_side ? (d.lanes < fm_batch_plan::kSide + 1 ? d.lanes
-                                                                       : fm_batch_plan::kSide + 1)
-                                  : 1;
-    if (lanes > 1) {
-        FM_CHECK_HIP(hipEventRecord(p->fork_ev, s));
-        for (int q = 0; q < lanes - 1; ++q) FM_CHECK_HIP(hipStreamWaitEvent(p->side[q], p->fork_ev, 0));
-    }
+static int program_lanes(const fm_batch_plan *p, const Program &d) {
+    return p->has_side ? (d.lanes < fm_batch_plan::kSide + 1 ? d.lanes : fm_batch_plan::kSide + 1)
+                       : 1;
+}
+
+// Issue the launches of a program on `lanes` lanes (lane 0 is the caller's
+// stream) without forking or joining.
+template <typename T>
+static fm_status issue_launches(const fm_batch_plan *p, const Program &d, int lanes, hipStream_t s,
+                                bool skip_early, int64_t out_shift, const T *out_mul) {
     for (const Launch &l : d.launches) {
         if (skip_early && l.early) continue;
         const int lane = lanes > 1 ? l.lane % lanes : 0;
@@ -564,6 +564,18 @@ static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream
         fm_status st = launch_one<T>(p, d, l, sq, out_shift, out_mul);
         if (st) return st;
     }
+    return FM_SUCCESS;
+}
+
+static fm_status fork_lanes(const fm_batch_plan *p, int lanes, hipStream_t s) {
+    if (lanes > 1) {
+        FM_CHECK_HIP(hipEventRecord(p->fork_ev, s));
+        for (int q = 0; q < lanes - 1; ++q) FM_CHECK_HIP(hipStreamWaitEvent(p->side[q], p->fork_ev, 0));
+    }
+    return FM_SUCCESS;
+}
+
+static fm_status join_lanes(const fm_batch_plan *p, int lanes, hipStream_t s) {
     if (lanes > 1) {
         for (int q = 0; q < lanes - 1; ++q) {
             FM_CHECK_HIP(hipEventRecord(p->join_ev[q], p->side[q]));
@@ -571,6 +583,20 @@ static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream
         }
     }
     return FM_SUCCESS;
+}
+
+// Issue a program: lanes beyond the first fork from the caller's stream and
+// join back into it (event record / wait only, so it is graph-capturable).
+// skip_early leaves out the launches of the groups that need no spectrum.
+template <typename T>
+static fm_status run_program(const fm_batch_plan *p, const Program &d, hipStream_t s,
+                             bool skip_early = false, int64_t out_shift = 0,
+                             const T *out_mul = nullptr) {
+    const int lanes = program_lanes<T>(p, d);
+    fm_status st = fork_lanes(p, lanes, s);
+    if (!st) st = issue_launches<T>(p, d, lanes, s, skip_early, out_shift, out_mul);
+    if (!st) st = join_lanes(p, lanes, s);
+    return st;
 }
 
 static int64_t up256(int64_t b) { return (b + 255) / 256 * 256; }
@@ -808,6 +834,7 @@ static fm_status batch_bind_t(fm_batch_plan *p, int adjoint, const void *const *
     fm_status st = finalize<T>(b, p->dir[dd], fm_batch_plan::kSide + 1, s);
     if (st) return st;
     if (dd == 0) p->res_bound = targets != nullptr;
+    p->lsq_bound = false;
     p->ws_bound[dd] = workspace;
     p->ins_bound[dd].assign(ins, ins + p->count);
     p->ld_bound[dd].assign(ld_in, ld_in + p->count);
@@ -953,6 +980,7 @@ static fm_status batch_bind_lsq_t(fm_batch_plan *p, const void *const *ins, cons
     if (!st) st = finalize<T>(ba, p->dir[1], fm_batch_plan::kSide + 1, s);
     if (st) return st;
     p->res_bound = true;
+    p->lsq_bound = true;
     for (int d = 0; d < 2; ++d) p->ws_bound[d] = ws;
     p->ins_bound[0].assign(ins, ins + p->count);
     p->ld_bound[0].assign(ld_in, ld_in + p->count);
@@ -965,9 +993,18 @@ static fm_status batch_bind_lsq_t(fm_batch_plan *p, const void *const *ins, cons
 __global__ __launch_bounds__(256) void residual_sum_kernel(const double *__restrict__ part,
                                                            int64_t n, double *__restrict__ out) {
     __shared__ double red[256];
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
-    red[threadIdx.x] = s;
+    // eight independent running sums per thread keep eight loads in flight
+    // (one workgroup: the sum's order is fixed, so the loss is deterministic)
+    double s[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 7 * 256 < n; i += 8 * 256) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] += part[i + u * 256];
+    }
+    for (; i < n; i += 256) s[0] += part[i];
+    red[threadIdx.x] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -1256,10 +1293,27 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
 }
 
 template <typename T>
-static fm_status batch_grad_apply_t(const fm_batch_plan *p, bool run_pass_a, hipStream_t s) {
+static fm_status batch_grad_apply_t(const fm_batch_plan *p, bool run_pass_a, hipStream_t s,
+                                    bool with_adjoint = false, int64_t out_shift = 0,
+                                    const T *out_mul = nullptr) {
     fm_status st = FM_SUCCESS;
     if (run_pass_a) st = run_program<T>(p, p->gpassA, s);
-    if (!st) st = run_program<T>(p, p->gcorr, s);
+    if (!st && with_adjoint) {
+        // least-squares backward: the adjoint (from its row pass on) and the
+        // correlations read the same column transforms and write disjoint
+        // buffers, so they share one fork of the lanes; a lane that is done
+        // with its share of the adjoint goes on with its correlations while
+        // the others finish
+        const Program &ad = p->dir[1];
+        const int la = program_lanes<T>(p, ad), lc = program_lanes<T>(p, p->gcorr);
+        const int lanes = la > lc ? la : lc;
+        st = fork_lanes(p, lanes, s);
+        if (!st) st = issue_launches<T>(p, ad, lanes, s, false, out_shift, out_mul);
+        if (!st) st = issue_launches<T>(p, p->gcorr, lanes, s, false, 0, nullptr);
+        if (!st) st = join_lanes(p, lanes, s);
+    } else if (!st) {
+        st = run_program<T>(p, p->gcorr, s);
+    }
     if (!st) st = run_program<T>(p, p->gpassC, s);
     if (st) return st;
     int64_t gx = ceil_div(p->g_maxgen, 256);
@@ -1318,6 +1372,29 @@ extern "C" fm_status fm_batch_grad_apply_reuse(const fm_batch_plan *p, void *str
                "blocks and the adjoint direction to its g blocks");
     if (p->prec == FM_COMPLEX64) return batch_grad_apply_t<float>(p, false, as_stream(stream));
     return batch_grad_apply_t<double>(p, false, as_stream(stream));
+}
+
+extern "C" fm_status fm_batch_lsq_backward(const fm_batch_plan *p, int64_t shift_bytes,
+                                           const void *scale, void *stream) {
+    FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
+    FM_REQUIRE(p->lsq_bound, FM_ERR_VALUE,
+               "the least-squares backward needs both directions bound by fm_batch_bind_lsq");
+    FM_REQUIRE(p->g_bound && p->gtab, FM_ERR_VALUE,
+               "batch gradient used before fm_batch_grad_bind");
+    const bool ok = p->ws_bound[0] == p->g_ws && p->ws_bound[1] == p->g_ws &&
+                    p->ins_bound[0] == p->g_xs && p->ld_bound[0] == p->g_ldx &&
+                    p->ins_bound[1] == p->g_gs && p->ld_bound[1] == p->g_ldg;
+    FM_REQUIRE(ok, FM_ERR_VALUE,
+               "the gradient must be bound to the least-squares step's inputs and residuals");
+    const int64_t esz = p->prec == FM_COMPLEX64 ? 8 : 16;
+    FM_REQUIRE(shift_bytes % esz == 0, FM_ERR_VALUE,
+               "output shift of %lld bytes is not a whole number of elements",
+               (long long)shift_bytes);
+    if (p->prec == FM_COMPLEX64)
+        return batch_grad_apply_t<float>(p, false, as_stream(stream), true, shift_bytes,
+                                         (const float *)scale);
+    return batch_grad_apply_t<double>(p, false, as_stream(stream), true, shift_bytes,
+                                      (const double *)scale);
 }
 
 // ======================================================================

@@ -288,6 +288,40 @@ def test_least_squares_function_matches_composition(torch_mod, dtype):
         assert abs(float(loss) - tot) <= 10 * tol * tot
 
 
+@pytest.mark.parametrize("want", ["x", "gen"])
+def test_least_squares_one_kind_of_gradient(torch_mod, want):
+    """Only the x gradient (generators without requires_grad: the adjoint
+    alone, redirected) or only the generator gradients (x without
+    requires_grad: the correlations alone) against the composition through
+    the generic matvec Function."""
+    torch = torch_mod
+    import fastmat_amd.autograd as fa
+    from fastmat_amd.batch import ConvBatch
+    rng = np.random.default_rng(57)
+    spec = [("circulant", 300, 300, 2), ("toeplitz", 3000, 9000, 3),
+            ("circulant", 5000, 5000, 4), ("toeplitz", 200, 350, 1)]
+    gens0 = [crandn(rng, n if k_ == "circulant" else m + n - 1) for k_, m, n, _ in spec]
+    ops = [make_op(k_, m, g) for (k_, m, n, _), g in zip(spec, gens0)]
+    batch = ConvBatch(ops, [e[3] for e in spec])
+    n_in, n_out = batch.packed_sizes()
+    gens = [torch.from_numpy(g).cuda().requires_grad_(want == "gen") for g in gens0]
+    xp = torch.from_numpy(crandn(rng, n_in)).cuda().requires_grad_(want == "x")
+    yp = torch.from_numpy(crandn(rng, n_out)).cuda()
+    say("least squares, only", want)
+    (fa.batch_least_squares_packed(batch, gens, xp, yp) * 1.5).backward()
+    gens2 = [g.detach().clone().requires_grad_(want == "gen") for g in gens]
+    xp2 = xp.detach().clone().requires_grad_(want == "x")
+    out = fa.batch_conv_matvec_packed(ConvBatch(ops, [e[3] for e in spec]), gens2, xp2)
+    (((out - yp).abs() ** 2).sum() * 1.5).backward()
+    if want == "x":
+        assert all(g.grad is None for g in gens)
+        assert accept.max_rel_error(xp.grad.cpu().numpy(), xp2.grad.cpu().numpy()) < 1e-5
+    else:
+        assert xp.grad is None
+        for g, g2 in zip(gens, gens2):
+            assert accept.max_rel_error(g.grad.cpu().numpy(), g2.grad.cpu().numpy()) < 1e-5
+
+
 def test_least_squares_backward_after_another_forward(torch_mod):
     """Two least-squares forwards on one batch (different x and y), then
     the first one's backward, then the second's: each backward must give its
