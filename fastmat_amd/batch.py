@@ -514,7 +514,8 @@ class ConvBatch:
     DEFAULT_SCHEDULES = ((2, 0, 2), (3, 0, 2), (4, 0, 2), (2, 48, 2), (3, 48, 2),
                          (2, 96, 2), (3, 96, 2), (4, 96, 2), (2, 128, 1), (3, 192, 1),
                          (4, 1024, 2), (4, 1536, 2), (3, 1024, 2), (6, 1536, 2),
-                         (4, 512, 3), (4, 768, 3), (4, 1024, 3), (6, 768, 3), (6, 512, 3))
+                         (4, 512, 3), (4, 768, 3), (4, 1024, 3), (6, 768, 3), (6, 512, 3),
+                         (8, 512, 3), (8, 768, 3))
 
     def tune(self, ins, outs, adjoint=False, tuner=None, candidates=(1, 2, 3, 4)):
         """Pick the persistent grid (workgroups per CU) of one direction."""

@@ -1544,23 +1544,26 @@ static fm_status batch_gen_forward_t(const fm_batch_plan *p, hipStream_t s) {
     constexpr int K = fm_batch_plan::kSide;
     bool any_early = false;
     for (const Launch &l : d.launches) any_early = any_early || l.early;
-    const bool overlap = p->has_side && any_early && d.lanes <= K && p->sprog.lanes <= K;
+    // the spectrum program runs on the caller's stream and its first
+    // sprog.lanes - 1 side streams; the early launches go to the others
+    const int first = p->sprog.lanes > 1 ? p->sprog.lanes - 1 : 0;
+    const bool overlap = p->has_side && any_early && first < K;
     if (!overlap) {
         fm_status st = batch_spec_apply_t<T>(p, s);
         return st ? st : run_program<T>(p, d, s);
     }
-    hipStream_t side = p->side[K - 1];
+    const int nfree = K - first;
     FM_CHECK_HIP(hipEventRecord(p->fork_ev, s));
-    FM_CHECK_HIP(hipStreamWaitEvent(side, p->fork_ev, 0));
+    for (int q = first; q < K; ++q) FM_CHECK_HIP(hipStreamWaitEvent(p->side[q], p->fork_ev, 0));
     for (const Launch &l : d.launches) {
         if (!l.early) continue;
-        fm_status st = launch_one<T>(p, d, l, side);
+        fm_status st = launch_one<T>(p, d, l, p->side[first + l.lane % nfree]);
         if (st) return st;
     }
-    FM_CHECK_HIP(hipEventRecord(p->join_ev[K - 1], side));
+    for (int q = first; q < K; ++q) FM_CHECK_HIP(hipEventRecord(p->join_ev[q], p->side[q]));
     fm_status st = batch_spec_apply_t<T>(p, s);
     if (st) return st;
-    FM_CHECK_HIP(hipStreamWaitEvent(s, p->join_ev[K - 1], 0));
+    for (int q = first; q < K; ++q) FM_CHECK_HIP(hipStreamWaitEvent(s, p->join_ev[q], 0));
     return run_program<T>(p, d, s, true);
 }
 }  // namespace fm
