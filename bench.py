@@ -398,8 +398,8 @@ def workload_batch(args, rng):
                         "lanes": sched[2]}, "schedule_candidates_ms": tuned,
            "hipgraph": True}
     return dict(step=step, eager_step=eager_step, isolated_step=isolated(), bytes=bytes_step,
-                check=check, plain=plain(), per_op=per_op(), cpu=None, cfg=cfg, dtype="c64",
-                metric=METRIC)
+                check=check, plain=plain(), per_op=per_op(), cpu=lambda: batch_cpu_legs(spec),
+                cfg=cfg, dtype="c64", metric=METRIC)
 
 
 def workload_copy(args, rng):
@@ -814,27 +814,8 @@ def workload_batch_grad(args, rng):
         return s
 
     def cpu():
-        # the reference's CPU path (oracle.cpu_path: Partial(Product(F.H,
-        # Diag, F)) with the reference's embedding lengths), forward and
-        # backward of every 8th operator on its own columns
-        from oracle import cpu_path
-        legs = []
-        for i in range(0, len(spec), 8):
-            kind, m, n, k, gen = spec[i]
-            g = np.asarray(gen, dtype=np.complex128)
-            op = cpu_path.circulant(g) if kind == "circulant" else cpu_path.toeplitz(g[:m], g[m:])
-            x = crandn(np.random.default_rng(i), (n, k))
-            if kind == "toeplitz" and m != n:
-                # backward takes an m-row block: time each direction on its
-                # own input
-                legs.append((_OneWay(op, False), x, ((n + m) * k + gen.size) * 8))
-                legs.append((_OneWay(op, True), crandn(np.random.default_rng(i + 1), (m, k)),
-                             ((n + m) * k + gen.size) * 8))
-            else:
-                legs.append((op, x, 2 * ((n + m) * k + gen.size) * 8))
-        return legs, ("every 8th operator of the batch (32 of 256), forward + backward on "
-                      "its own columns, one core; the generator gradient has no counterpart "
-                      "in the reference")
+        legs, sample = batch_cpu_legs(spec)
+        return legs, sample + "; the generator gradient has no counterpart in the reference"
 
     # the generic leg replays its own graph; the timed step's graph is
     # replayed again before the measurement
@@ -845,6 +826,29 @@ def workload_batch_grad(args, rng):
     return dict(step=step, eager_step=eager_step, isolated_step=isolated(), bytes=bytes_step,
                 check=check, plain=plain_torch(), per_op=per_op(), generic=generic_graph(), cpu=cpu,
                 cfg=cfg, dtype="c64", metric=METRIC)
+
+
+def batch_cpu_legs(spec):
+    """The reference's CPU path (oracle.cpu_path: Partial(Product(F.H, Diag,
+    F)) with the reference's embedding lengths), forward and backward of
+    every 8th operator of the uneven batch on its own columns."""
+    from oracle import cpu_path
+    legs = []
+    for i in range(0, len(spec), 8):
+        kind, m, n, k, gen = spec[i]
+        g = np.asarray(gen, dtype=np.complex128)
+        op = cpu_path.circulant(g) if kind == "circulant" else cpu_path.toeplitz(g[:m], g[m:])
+        x = crandn(np.random.default_rng(i), (n, k))
+        if kind == "toeplitz" and m != n:
+            # backward takes an m-row block: time each direction on its own
+            # input
+            legs.append((_OneWay(op, False), x, ((n + m) * k + gen.size) * 8))
+            legs.append((_OneWay(op, True), crandn(np.random.default_rng(i + 1), (m, k)),
+                         ((n + m) * k + gen.size) * 8))
+        else:
+            legs.append((op, x, 2 * ((n + m) * k + gen.size) * 8))
+    return legs, ("every 8th operator of the batch (32 of 256), forward + backward on its own "
+                  "columns, one core")
 
 
 class _OneWay:

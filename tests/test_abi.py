@@ -61,3 +61,15 @@ def test_errors_map_to_reference_exceptions():
         fm.Eye(3) // 2
     with pytest.raises(ValueError):
         fm.Partial(fm.Eye(3), rows=np.array([5]))
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_batch_backward_entry_points_reject_a_null_plan():
+    """The redirected apply and the least-squares backward validate their
+    arguments on the host before any launch (no device is touched)."""
+    from fastmat_amd import _lib
+    lib = _lib.load()
+    assert lib.fm_batch_apply_redirected(None, 1, 0, None, None) != 0
+    assert b"NULL" in lib.fm_get_last_error()
+    assert lib.fm_batch_lsq_backward(None, 0, None, None) != 0
+    assert b"NULL" in lib.fm_get_last_error()
