@@ -1304,12 +1304,41 @@ static fm_status batch_grad_apply_t(const fm_batch_plan *p, bool run_pass_a, hip
         // buffers, so they share one fork of the lanes; a lane that is done
         // with its share of the adjoint goes on with its correlations while
         // the others finish
-        const Program &ad = p->dir[1];
-        const int la = program_lanes<T>(p, ad), lc = program_lanes<T>(p, p->gcorr);
+        const Program &ad = p->dir[1], &gc = p->gcorr;
+        const int la = program_lanes<T>(p, ad), lc = program_lanes<T>(p, gc);
         const int lanes = la > lc ? la : lc;
+        constexpr int NL = fm_batch_plan::kSide + 1;
+        // each correlation lane (its launches depend on one another in
+        // stream order, so they move together) goes to the group lane with
+        // the least adjoint work so far, largest first (bytes moved as the
+        // measure of work); not to the one-pass operators' lane, whose few
+        // bytes take long while the persistent group launches hold the CUs
+        double load[NL] = {}, cl[NL] = {};
+        for (const Launch &l : ad.launches) load[lanes > 1 ? l.lane % lanes : 0] += l.pass_bytes;
+        for (const Launch &l : gc.launches) cl[lanes > 1 ? l.lane % lanes : 0] += l.pass_bytes;
+        int lane_of[NL];
+        bool placed[NL] = {};
+        for (int u = 0; u < NL; ++u) lane_of[u] = u;
+        for (int n = 0; n < lanes; ++n) {
+            int u = -1;
+            for (int v = 0; v < lanes; ++v)
+                if (!placed[v] && cl[v] > 0 && (u < 0 || cl[v] > cl[u])) u = v;
+            if (u < 0) break;
+            const int skip = p->lanes < lanes ? p->lanes : -1;  // the one-pass lane
+            int q = -1;
+            for (int v = 0; v < lanes; ++v)
+                if (v != skip && (q < 0 || load[v] < load[q])) q = v;
+            placed[u] = true;
+            lane_of[u] = q;
+            load[q] += cl[u];
+        }
         st = fork_lanes(p, lanes, s);
         if (!st) st = issue_launches<T>(p, ad, lanes, s, false, out_shift, out_mul);
-        if (!st) st = issue_launches<T>(p, p->gcorr, lanes, s, false, 0, nullptr);
+        for (const Launch &l : gc.launches) {
+            if (st) break;
+            const int q = lane_of[lanes > 1 ? l.lane % lanes : 0];
+            st = launch_one<T>(p, gc, l, q == 0 ? s : p->side[q - 1]);
+        }
         if (!st) st = join_lanes(p, lanes, s);
     } else if (!st) {
         st = run_program<T>(p, p->gcorr, s);
