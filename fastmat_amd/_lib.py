@@ -81,7 +81,7 @@ _PROTOTYPES = {
                                ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _vp, _c_i64, _vp]),
     "fm_batch_apply": (_c_int, [_vp, _c_int, _vp]),
     "fm_batch_apply_redirected": (_c_int, [_vp, _c_int, _c_i64, _vp, _vp]),
-    "fm_batch_lsq_backward": (_c_int, [_vp, _c_i64, _vp, _vp]),
+    "fm_batch_lsq_backward": (_c_int, [_vp, _c_i64, _vp, _c_i64, _vp]),
     "fm_batch_bind_residual": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
                                         ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),
                                         ctypes.POINTER(_vp), ctypes.POINTER(_c_i64),

@@ -294,10 +294,11 @@ class _BatchLeastSquares(torch.autograd.Function):
             adj = (gxp.data_ptr() - gx_buf.data_ptr(), glr)
             if not need_g:
                 batch.launch(True, out_shift=adj[0], out_scale=adj[1])
-        scale = gl.to(xp.dtype)
         if need_g:
             # with the x gradient: the adjoint and the generator gradient's
-            # correlations in one backward (fm_batch_lsq_backward)
+            # correlations in one backward (fm_batch_lsq_backward), both
+            # scaled by gl on the device; without it, the gradient alone
+            scale = gl.to(xp.dtype) if adj is None else None
             raw = batch.generator_grads(X, R, reuse=True, scale=scale, lsq_adjoint=adj)
             ggen = [(r if r.dtype == dt else r.to(dt)) if cplx else r.real.to(dt)
                     for r, cplx, dt in zip(raw, ctx.gen_complex, ctx.gen_dtype)]

@@ -389,7 +389,8 @@ class ConvBatch:
         (bind_lsq, with xs its inputs and gs its residuals), also runs the
         adjoint apply as launch(True, out_shift, out_scale) does, its
         launches sharing the lanes with the gradient's
-        (fm_batch_lsq_backward)."""
+        (fm_batch_lsq_backward), and scales the gradients by out_scale (a
+        real device scalar; ``scale`` is not used then)."""
         n = len(self.ops)
         for i, (op, k) in enumerate(zip(self.ops, self.ks)):
             if tuple(xs[i].shape) != (k, op.numCols) or tuple(gs[i].shape) != (k, op.numRows):
@@ -429,13 +430,16 @@ class ConvBatch:
                       self._gws_bytes, D.stream())
             self._gbound = key
         if lsq_adjoint is not None:
+            # the fold writes the gradients, scaled by the same device
+            # scalar as the adjoint's output, straight into a new tensor
             shift, sc = lsq_adjoint
+            out = torch.empty_like(self._gout)
             _lib.call("fm_batch_lsq_backward", self._h, int(shift), self._scale_ptr(sc),
-                      D.stream())
+                      out.data_ptr() - self._gout.data_ptr(), D.stream())
             self.apply_count[1] += 1
-        else:
-            _lib.call("fm_batch_grad_apply_reuse" if reuse else "fm_batch_grad_apply", self._h,
-                      D.stream())
+            return [out[int(off[i]):int(off[i + 1])] for i in range(n)]
+        _lib.call("fm_batch_grad_apply_reuse" if reuse else "fm_batch_grad_apply", self._h,
+                  D.stream())
         out = self._gout.clone() if scale is None else self._gout * scale
         return [out[int(off[i]):int(off[i + 1])] for i in range(n)]
 

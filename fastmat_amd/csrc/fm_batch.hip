@@ -1092,9 +1092,16 @@ template <typename T> struct GradEntry {
 };
 
 template <typename T>
-__global__ void batch_fold_kernel(const GradEntry<T> *__restrict__ tab) {
-    const GradEntry<T> e = tab[blockIdx.y];
-    const T s = T(1.0 / double(e.L));
+__global__ void batch_fold_kernel(const GradEntry<T> *__restrict__ tab, int64_t out_shift,
+                                  const T *__restrict__ out_mul) {
+    GradEntry<T> e = tab[blockIdx.y];
+    T s = T(1.0 / double(e.L));
+    if (out_shift != 0 || out_mul != nullptr) {
+        // least-squares backward: into another buffer of the same layout,
+        // scaled by the device scalar *out_mul
+        e.out = reinterpret_cast<cpx<T> *>(reinterpret_cast<char *>(e.out) + out_shift);
+        if (out_mul) s *= *out_mul;
+    }
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < e.gen_len; i += stride) {
         cpx<T> v;
@@ -1295,7 +1302,7 @@ static fm_status batch_grad_bind_t(fm_batch_plan *p, const void *const *xs, cons
 template <typename T>
 static fm_status batch_grad_apply_t(const fm_batch_plan *p, bool run_pass_a, hipStream_t s,
                                     bool with_adjoint = false, int64_t out_shift = 0,
-                                    const T *out_mul = nullptr) {
+                                    const T *out_mul = nullptr, int64_t gen_shift = 0) {
     fm_status st = FM_SUCCESS;
     if (run_pass_a) st = run_program<T>(p, p->gpassA, s);
     if (!st && with_adjoint) {
@@ -1348,7 +1355,8 @@ static fm_status batch_grad_apply_t(const fm_batch_plan *p, bool run_pass_a, hip
     int64_t gx = ceil_div(p->g_maxgen, 256);
     if (gx > 256) gx = 256;
     hipLaunchKernelGGL(batch_fold_kernel<T>, dim3(unsigned(gx), unsigned(p->count)), dim3(256), 0,
-                       s, (const GradEntry<T> *)p->gtab);
+                       s, (const GradEntry<T> *)p->gtab, gen_shift,
+                       with_adjoint ? out_mul : (const T *)nullptr);
     FM_CHECK_LAUNCH();
     return FM_SUCCESS;
 }
@@ -1404,7 +1412,8 @@ extern "C" fm_status fm_batch_grad_apply_reuse(const fm_batch_plan *p, void *str
 }
 
 extern "C" fm_status fm_batch_lsq_backward(const fm_batch_plan *p, int64_t shift_bytes,
-                                           const void *scale, void *stream) {
+                                           const void *scale, int64_t gen_shift_bytes,
+                                           void *stream) {
     FM_REQUIRE(p != nullptr, FM_ERR_VALUE, "plan is NULL");
     FM_REQUIRE(p->lsq_bound, FM_ERR_VALUE,
                "the least-squares backward needs both directions bound by fm_batch_bind_lsq");
@@ -1416,14 +1425,14 @@ extern "C" fm_status fm_batch_lsq_backward(const fm_batch_plan *p, int64_t shift
     FM_REQUIRE(ok, FM_ERR_VALUE,
                "the gradient must be bound to the least-squares step's inputs and residuals");
     const int64_t esz = p->prec == FM_COMPLEX64 ? 8 : 16;
-    FM_REQUIRE(shift_bytes % esz == 0, FM_ERR_VALUE,
-               "output shift of %lld bytes is not a whole number of elements",
-               (long long)shift_bytes);
+    FM_REQUIRE(shift_bytes % esz == 0 && gen_shift_bytes % esz == 0, FM_ERR_VALUE,
+               "output shift of %lld / %lld bytes is not a whole number of elements",
+               (long long)shift_bytes, (long long)gen_shift_bytes);
     if (p->prec == FM_COMPLEX64)
         return batch_grad_apply_t<float>(p, false, as_stream(stream), true, shift_bytes,
-                                         (const float *)scale);
+                                         (const float *)scale, gen_shift_bytes);
     return batch_grad_apply_t<double>(p, false, as_stream(stream), true, shift_bytes,
-                                      (const double *)scale);
+                                      (const double *)scale, gen_shift_bytes);
 }
 
 // ======================================================================

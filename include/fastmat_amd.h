@@ -308,14 +308,15 @@ fm_status fm_batch_grad_apply_reuse(const fm_batch_plan *plan, void *stream);
  * call: fm_batch_apply_redirected(plan, 1, shift_bytes, scale) (the input
  * gradients, scaled by the device scalar *scale, written shift_bytes past
  * the bound gradient blocks) and fm_batch_grad_apply_reuse (the generator
- * gradients into the blocks fm_batch_grad_bind gave, which must be the
- * step's inputs and residuals).  Neither needs the other's results, so their
- * launches share one fork of the lanes.  The reference differentiates
+ * gradients, also scaled by *scale, written gen_shift_bytes past the blocks
+ * fm_batch_grad_bind gave; that binding must be to the step's inputs and
+ * residuals).  Neither needs the other's results, so their launches share
+ * one fork of the lanes.  The reference differentiates
  * nothing; this is the backward of sum_i ||A_i x_i - t_i||^2 through
  * Matrix.backward (Matrix.pyx:1844) and the generator's spectrum
  * (Circulant.pyx:130, Toeplitz.pyx:253-260). */
 fm_status fm_batch_lsq_backward(const fm_batch_plan *plan, int64_t shift_bytes, const void *scale,
-                                void *stream);
+                                int64_t gen_shift_bytes, void *stream);
 /* Batched fm_conv_set_generator: every operator's spectrum from its device
  * generator gens[i] (element type gen_dtype for all), as one embedding launch
  * and one batched FFT program.  fm_batch_gen_bind builds the program for one

@@ -71,5 +71,5 @@ def test_batch_backward_entry_points_reject_a_null_plan():
     lib = _lib.load()
     assert lib.fm_batch_apply_redirected(None, 1, 0, None, None) != 0
     assert b"NULL" in lib.fm_get_last_error()
-    assert lib.fm_batch_lsq_backward(None, 0, None, None) != 0
+    assert lib.fm_batch_lsq_backward(None, 0, None, 0, None) != 0
     assert b"NULL" in lib.fm_get_last_error()
