@@ -194,6 +194,8 @@ template <typename T> struct TileArgs {
     // a redirected apply moves y by a byte shift and multiplies out_scale by
     // a device scalar for such passes only
     int redirect;
+    // non-temporal (streaming) output stores (the batch engine's passes)
+    int stream_out;
     int logJ;                       // log2(J) when J is a power of two, else -1
     int64_t o_group;                // > 1: tiles walk groups of o_group o's (see tile_first_seq)
     // optional residual epilogue (target != NULL): the store writes
@@ -250,6 +252,20 @@ __device__ __forceinline__ void stg(V *base, Ix idx, V val) {
         *(FM_GLOBAL V *)(b + uint32_t(idx * uint32_t(sizeof(V)))) = val;
     else
         ((FM_GLOBAL V *)b)[idx] = val;
+}
+
+// Non-temporal store of one complex element (see stg).
+template <typename Ix, typename T>
+__device__ __forceinline__ void stg_stream(cpx<T> *base, Ix idx, cpx<T> val) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    FM_GLOBAL char *b = (FM_GLOBAL char *)base;
+    FM_GLOBAL v2 *q;
+    if constexpr (sizeof(Ix) == 4)
+        q = (FM_GLOBAL v2 *)(b + uint32_t(idx * uint32_t(sizeof(cpx<T>))));
+    else
+        q = (FM_GLOBAL v2 *)(b + idx * int64_t(sizeof(cpx<T>)));
+    v2 v = {val.x, val.y};
+    __builtin_nontemporal_store(v, q);
 }
 
 // Bound of a logical index in the launch's index type.
@@ -538,7 +554,10 @@ __device__ __forceinline__ void fft_tile_store(const TileArgs<T> &a, int64_t til
                 res2 += double(d.x) * double(d.x) + double(d.y) * double(d.y);
                 u = d;
             }
-            stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
+            if (a.stream_out)
+                stg_stream(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
+            else
+                stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
         }
     }
     if (a.target) tile_residual_sum<T, NT>(a, tile, res2, lds);
