@@ -330,10 +330,10 @@ static fm_status finalize(ProgBuilder<T> &b, Program &prog, int ncu_lanes_cap, h
         fm_status st = normalize_pass(d.a);
         if (st) return st;
         // the batch's passes write gigabytes per launch that the next pass
-        // reads back long after: non-temporal stores (the 256-point conv
-        // pass 126 -> 121 us alone, the least-squares step about 1% faster;
-        // the single-operator Fourier passes got slower with them)
-        d.a.stream_out = 1;
+        // reads back long after: non-temporal stores and loads (the
+        // 256-point conv pass 126 -> 121 -> 119 us alone; the
+        // single-operator Fourier passes got slower with such stores)
+        d.a.stream_out = 3;
     }
     for (auto &c : b.cdescs) {
         // the correlation tiles use corr_loge elements per thread: set the

@@ -194,7 +194,8 @@ template <typename T> struct TileArgs {
     // a redirected apply moves y by a byte shift and multiplies out_scale by
     // a device scalar for such passes only
     int redirect;
-    // non-temporal (streaming) output stores (the batch engine's passes)
+    // non-temporal (streaming) accesses (the batch engine's passes): bit 0
+    // the output stores, bit 1 the input loads
     int stream_out;
     int logJ;                       // log2(J) when J is a power of two, else -1
     int64_t o_group;                // > 1: tiles walk groups of o_group o's (see tile_first_seq)
@@ -266,6 +267,20 @@ __device__ __forceinline__ void stg_stream(cpx<T> *base, Ix idx, cpx<T> val) {
         q = (FM_GLOBAL v2 *)(b + idx * int64_t(sizeof(cpx<T>)));
     v2 v = {val.x, val.y};
     __builtin_nontemporal_store(v, q);
+}
+
+// Non-temporal load of one complex element (see ldg).
+template <typename Ix, typename T>
+__device__ __forceinline__ cpx<T> ldg_stream(const cpx<T> *base, Ix idx) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const FM_GLOBAL char *b = (const FM_GLOBAL char *)base;
+    const FM_GLOBAL v2 *q;
+    if constexpr (sizeof(Ix) == 4)
+        q = (const FM_GLOBAL v2 *)(b + uint32_t(idx * uint32_t(sizeof(cpx<T>))));
+    else
+        q = (const FM_GLOBAL v2 *)(b + idx * int64_t(sizeof(cpx<T>)));
+    const v2 v = __builtin_nontemporal_load(q);
+    return mk(v.x, v.y);
 }
 
 // Bound of a logical index in the launch's index type.
@@ -431,19 +446,27 @@ __device__ __forceinline__ void fft_tile_load(const TileArgs<T> &a, int64_t tile
     const int64_t s0 = tile_first_seq<T, S::LOGB>(a, tile);
     const Ix nv = idx_bound<Ix>(a.n_valid);
     const Ix in_gn = Ix(a.in_gn), in_gj = Ix(a.in_gj), in_ns = Ix(a.in_ns);
+    // one loop per load kind (uniform), so that all the tile's loads issue
+    // before the first use
+    auto body = [&](auto load) {
 #pragma unroll
-    for (int e = 0; e < NB; ++e) {
-        int b, i;
-        bfly_pos<S::LOGB, LOGNR>(threadIdx.x + e * NT, a.seqfast_in, b, i);
-        const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
+        for (int e = 0; e < NB; ++e) {
+            int b, i;
+            bfly_pos<S::LOGB, LOGNR>(threadIdx.x + e * NT, a.seqfast_in, b, i);
+            const SeqPos<Ix> sp = seq_pos<Ix>(a, s0 + b);
 #pragma unroll
-        for (int r = 0; r < S::R0; ++r) {
-            const Ix n = Ix(i + r * NR);
-            const Ix gidx = n * in_gn + sp.j * in_gj;
-            const bool ok = sp.ok && gidx < nv;
-            v[e * S::R0 + r] = ok ? ldg(a.x, Ix(sp.in_base + n * in_ns)) : mk(T(0), T(0));
+            for (int r = 0; r < S::R0; ++r) {
+                const Ix n = Ix(i + r * NR);
+                const Ix gidx = n * in_gn + sp.j * in_gj;
+                const bool ok = sp.ok && gidx < nv;
+                v[e * S::R0 + r] = ok ? load(Ix(sp.in_base + n * in_ns)) : mk(T(0), T(0));
+            }
         }
-    }
+    };
+    if (a.stream_out & 2)
+        body([&](Ix k) { return ldg_stream(a.x, k); });
+    else
+        body([&](Ix k) { return ldg(a.x, k); });
 }
 
 // The tile's sum of the threads' res2 (sums of |u - t|^2) into
@@ -554,7 +577,7 @@ __device__ __forceinline__ void fft_tile_store(const TileArgs<T> &a, int64_t til
                 res2 += double(d.x) * double(d.x) + double(d.y) * double(d.y);
                 u = d;
             }
-            if (a.stream_out)
+            if (a.stream_out & 1)
                 stg_stream(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
             else
                 stg(a.y, Ix(sp.out_base + n * out_ns), scale(u, a.out_scale));
