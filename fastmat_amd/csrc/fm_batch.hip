@@ -336,6 +336,9 @@ static fm_status finalize(ProgBuilder<T> &b, Program &prog, int ncu_lanes_cap, h
         d.a.stream_out = 3;
     }
     for (auto &c : b.cdescs) {
+        // the correlations read each column transform once: non-temporal
+        // loads (the 256-point row correlation 304 -> 289 us alone)
+        c.a.stream_out = 2;
         // the correlation tiles use corr_loge elements per thread: set the
         // tile shape (B sequences) for that shape
         const int loge = corr_loge<T>(c.a.logN);
