@@ -189,7 +189,21 @@ __device__ __forceinline__ void wht_all(C *v, C *lds, bool sf0) {
     }
 }
 
-template <typename S, int LOGN, int NT>
+// Element load / store of a pass; STREAM bit 1 makes the loads and bit 0 the
+// stores non-temporal (the data is touched once per pass, so it need not
+// displace other lines from the caches).
+template <int STREAM, typename S> __device__ __forceinline__ S wht_ld(const S *x, int64_t i) {
+    const FM_GLOBAL S *q = (const FM_GLOBAL S *)x + i;
+    if constexpr (STREAM & 2) return __builtin_nontemporal_load(q);
+    else return *q;
+}
+template <int STREAM, typename S> __device__ __forceinline__ void wht_st(S *y, int64_t i, S v) {
+    FM_GLOBAL S *q = (FM_GLOBAL S *)y + i;
+    if constexpr (STREAM & 1) __builtin_nontemporal_store(v, q);
+    else *q = v;
+}
+
+template <typename S, int LOGN, int NT, int STREAM = 0>
 __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
     using C = typename WhtTraits<S>::C;
     constexpr int LOGE = wht_loge<C>(), E = 1 << LOGE;
@@ -220,7 +234,8 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
             wht_seq(a, s0 + b, ok, base);
 #pragma unroll
             for (int t = 0; t < R; ++t)
-                v[e * R + t] = ok ? to_c<S, C>(x[base + int64_t(wht_pos<0, LR>(u, t)) * a.ns]) : C{};
+                v[e * R + t] = ok ? to_c<S, C>(wht_ld<STREAM>(x, base + int64_t(wht_pos<0, LR>(u, t)) * a.ns))
+                                  : C{};
         }
         wht_all<C, LOGN, LOGE, NT, LOGB, 0>(v, lds, true);
     } else {
@@ -232,7 +247,7 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
             bool ok;
             int64_t base;
             wht_seq(a, s0 + b, ok, base);
-            v[e] = ok ? to_c<S, C>(x[base + int64_t(n) * a.ns]) : C{};
+            v[e] = ok ? to_c<S, C>(wht_ld<STREAM>(x, base + int64_t(n) * a.ns)) : C{};
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -270,7 +285,7 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 wht_restage_pos<LOGB, NT>(threadIdx.x, e, b, p);
-                y[base + int64_t(p) * a.ns] = to_s<S, C>(lds[b * SB + lpad(p)]);
+                wht_st<STREAM>(y, base + int64_t(p) * a.ns, to_s<S, C>(lds[b * SB + lpad(p)]));
             }
         }
         return;
@@ -286,11 +301,11 @@ __global__ __launch_bounds__(NT) void wht_pass_kernel(const WhtArgs a) {
         if (!ok) continue;
 #pragma unroll
         for (int t = 0; t < RL; ++t)
-            y[base + int64_t(wht_pos<QL, LRL>(u, t)) * a.ns] = to_s<S, C>(v[e * RL + t]);
+            wht_st<STREAM>(y, base + int64_t(wht_pos<QL, LRL>(u, t)) * a.ns, to_s<S, C>(v[e * RL + t]));
     }
 }
 
-template <typename S, int LOGN, int NT>
+template <typename S, int LOGN, int NT, int STREAM = 0>
 static fm_status wht_launch_nt(WhtArgs a, hipStream_t stream) {
     using C = typename WhtTraits<S>::C;
     constexpr int LOGE = wht_loge<C>();
@@ -300,7 +315,7 @@ static fm_status wht_launch_nt(WhtArgs a, hipStream_t stream) {
     const size_t lds = size_t(B) * Sh::SB * sizeof(C);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)wht_pass_kernel<S, LOGN, NT>,
+        (void)hipFuncSetAttribute((const void *)wht_pass_kernel<S, LOGN, NT, STREAM>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
@@ -310,7 +325,7 @@ static fm_status wht_launch_nt(WhtArgs a, hipStream_t stream) {
                "tile count out of range");
     ProfToken tok;
     prof_begin(stream, &tok);
-    hipLaunchKernelGGL((wht_pass_kernel<S, LOGN, NT>), dim3(unsigned(a.ntiles)), dim3(NT), lds,
+    hipLaunchKernelGGL((wht_pass_kernel<S, LOGN, NT, STREAM>), dim3(unsigned(a.ntiles)), dim3(NT), lds,
                        stream, a);
     FM_CHECK_LAUNCH();
     if (tok.active) {
@@ -324,6 +339,15 @@ static fm_status wht_launch_nt(WhtArgs a, hipStream_t stream) {
     return FM_SUCCESS;
 }
 
+// Launch with plain accesses, or (4-byte numbers, nt) non-temporal ones.
+template <typename S, int LOGN, int NT>
+static fm_status wht_launch_sel(WhtArgs a, hipStream_t stream, bool nt) {
+    if constexpr (sizeof(S) == 4 && std::is_arithmetic<S>::value) {
+        if (nt) return wht_launch_nt<S, LOGN, NT, 3>(a, stream);
+    }
+    return wht_launch_nt<S, LOGN, NT>(a, stream);
+}
+
 template <typename S, int LOGN>
 static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
     using C = typename WhtTraits<S>::C;
@@ -335,6 +359,12 @@ static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
     // default, else 512 (two tiles per CU).  Order 20, 64 columns, int32 and
     // float32 forward + adjoint: 1.177 ms per step with 512, 1.016 with 1024.
     // FASTMAT_AMD_WHT_STRIDED_NT=512 selects the narrower tile.
+    // 4-byte numbers: non-temporal loads and stores in the wide strided
+    // pass (mode 2) or in every pass (mode 3); mode 1 plain accesses.  Order
+    // 20, 64 columns, int32 + float32 forward + adjoint: 1.02 ms per step
+    // plain, 0.91 with mode 2, 0.95 with mode 3 (the contiguous pass is
+    // slower with non-temporal accesses).  FASTMAT_AMD_WHT_STREAM selects it.
+    const int stream_mode = knob("FASTMAT_AMD_WHT_STREAM", 2);
     if constexpr (NT == 256) {
         const int want = knob("FASTMAT_AMD_WHT_STRIDED_NT", 1024);
         if (a.seqfast && want >= 1024) {
@@ -342,14 +372,15 @@ static fm_status wht_launch_t(WhtArgs a, hipStream_t stream) {
             if constexpr (S4::LOGB >= 0) {
                 constexpr size_t lds4 = size_t(S4::B) * S4::SB * sizeof(C);
                 if (a.J >= S4::B && lds4 <= 160 * 1024)
-                    return wht_launch_nt<S, LOGN, 1024>(a, stream);
+                    return wht_launch_sel<S, LOGN, 1024>(a, stream, stream_mode >= 2);
             }
         }
         using S2 = WhtShape<LOGN, 512, LOGE>;
         constexpr size_t lds2 = size_t(S2::B) * S2::SB * sizeof(C);
-        if (a.seqfast && a.J >= S2::B && lds2 <= 80 * 1024) return wht_launch_nt<S, LOGN, 512>(a, stream);
+        if (a.seqfast && a.J >= S2::B && lds2 <= 80 * 1024)
+            return wht_launch_sel<S, LOGN, 512>(a, stream, stream_mode >= 3);
     }
-    return wht_launch_nt<S, LOGN, NT>(a, stream);
+    return wht_launch_sel<S, LOGN, NT>(a, stream, stream_mode >= 3);
 }
 
 // Host-side check of a tile's index mapping (the formulas the kernel runs):

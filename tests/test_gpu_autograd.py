@@ -266,7 +266,7 @@ def test_least_squares_function_matches_composition(torch_mod, dtype):
         out = fa.batch_conv_matvec_packed(batch, gens2, xp2)
         loss2 = ((out - yp).abs() ** 2).sum()
         (loss2 * w).backward()
-        assert abs(float(loss) - float(loss2)) <= tol * abs(float(loss2))
+        assert abs(float(loss.detach()) - float(loss2.detach())) <= tol * abs(float(loss2.detach()))
         assert accept.max_rel_error(xp.grad.cpu().numpy(), xp2.grad.cpu().numpy()) < tol
         for g, g2 in zip(gens, gens2):
             assert accept.max_rel_error(g.grad.cpu().numpy(), g2.grad.cpu().numpy()) < tol
