@@ -285,7 +285,7 @@ def test_least_squares_function_matches_composition(torch_mod, dtype):
             assert accept.max_rel_error(gens[i].grad.cpu().numpy(),
                                         fast64.conv_generator_grad(k_, m, n, x, 2 * w * r)) \
                 < 10 * tol
-        assert abs(float(loss) - tot) <= 10 * tol * tot
+        assert abs(float(loss.detach()) - tot) <= 10 * tol * tot
 
 
 @pytest.mark.parametrize("want", ["x", "gen"])
